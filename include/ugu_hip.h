@@ -1,0 +1,259 @@
+/*
+ * ugu_hip.h -- C ABI of the MI355X (gfx950) rasterizer.
+ *
+ * Plain C, POD descriptors and pointers only: no Eigen, STL or torch types
+ * cross this boundary. The C++ drop-in `ugu::Rasterizer`
+ * (include/ugu/renderer/cpu/rasterizer.h) and the Python package `ugu_amd`
+ * (ctypes) are both written on top of it.
+ *
+ * Reference interface each entry point replaces (UGU, unclearness/ugu):
+ *   ugu_hip_create / ugu_hip_destroy
+ *       Rasterizer::Rasterizer(const RendererCpuOption&) / ~Rasterizer
+ *       (src/renderer/cpu/rasterizer.cc:351-356)
+ *   ugu_hip_prepare_mesh
+ *       Rasterizer::set_mesh + Rasterizer::PrepareMesh
+ *       (src/renderer/cpu/rasterizer.cc:77-99; include/ugu/renderer/cpu/renderer.h:49-54)
+ *   ugu_hip_render
+ *       Rasterizer::Render(color, depth, normal, mask, face_id)
+ *       (src/renderer/cpu/rasterizer.cc:105-307, :372-375)
+ *   ugu_hip_render_batch
+ *       no single reference call: replaces the per-view loop of Render calls
+ *       (examples/ex02_renderer.cc:135-169)
+ *   ugu_hip_depth_to_u16
+ *       the ConvertTo step of Rasterizer::RenderW (src/renderer/cpu/rasterizer.cc:329-344,
+ *       include/ugu/image_proc.h:453-520)
+ *   ugu_hip_diff_forward / ugu_hip_diff_backward
+ *       no reference counterpart: differentiable rendering (gradients of an
+ *       image loss with respect to vertex colours, texture and positions)
+ *
+ * Error convention: every int-returning call returns UGU_HIP_OK (0) or a
+ * negative code; ugu_hip_last_error() gives the message. The validation codes
+ * -1..-10 correspond one-to-one to the reference's LOGE branches in
+ * ValidateAndInitBeforeRender (src/renderer/cpu/util_private.cc:19-66) and
+ * RenderW (src/renderer/cpu/rasterizer.cc:331-334).
+ *
+ * Threading: one context = one HIP stream; calls on one context must not
+ * overlap (the reference's Render is likewise not reentrant). Every call is
+ * synchronous: outputs are complete when it returns.
+ */
+#ifndef UGU_HIP_H_
+#define UGU_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UGU_HIP_ABI_VERSION 1
+
+typedef struct ugu_hip_ctx ugu_hip_ctx;
+
+enum UguHipStatus {
+  UGU_HIP_OK = 0,
+  UGU_HIP_ERR_NO_CAMERA = -1,               /* util_private.cc:19-22 */
+  UGU_HIP_ERR_MESH_NOT_PREPARED = -2,       /* util_private.cc:23-26 */
+  UGU_HIP_ERR_CULLING_NO_FACE_NORMAL = -3,  /* util_private.cc:27-30 */
+  UGU_HIP_ERR_TEXTURE_EMPTY = -4,           /* util_private.cc:31-35 */
+  UGU_HIP_ERR_TEXTURE_NTH_EMPTY = -5,       /* util_private.cc:36-44 */
+  UGU_HIP_ERR_VERTEX_COLOR_EMPTY = -6,      /* util_private.cc:45-50 */
+  UGU_HIP_ERR_FACE_NORMAL_EMPTY = -7,       /* util_private.cc:51-55 */
+  UGU_HIP_ERR_VERTEX_NORMAL_EMPTY = -8,     /* util_private.cc:56-62 */
+  UGU_HIP_ERR_ALL_NULL = -9,                /* util_private.cc:63-66 */
+  UGU_HIP_ERR_DEPTH_NULL = -10,             /* rasterizer.cc:331-334 */
+  UGU_HIP_ERR_INVALID_ARGUMENT = -20,
+  UGU_HIP_ERR_HIP = -21,
+  UGU_HIP_ERR_OUT_OF_MEMORY = -22,
+  UGU_HIP_ERR_NO_MESH = -23                 /* rasterizer.cc:91-94 */
+};
+
+/* Where output (and differentiable-input) pointers live. */
+enum UguHipMemory { UGU_HIP_MEMORY_HOST = 0, UGU_HIP_MEMORY_DEVICE = 1 };
+
+/* One RGB8 texture (ObjMaterial::diffuse_tex, an Image3b: rows x cols x 3). */
+typedef struct {
+  int32_t width;
+  int32_t height;
+  const uint8_t* data; /* host, height*width*3 bytes, row-major HWC */
+} UguTexture;
+
+/* Host-side mesh, the arrays of ugu::Mesh (include/ugu/mesh.h:83-102,137-151).
+ * Absent attributes: count 0 and null pointer. Indices are validated. */
+typedef struct {
+  int32_t num_vertices;
+  const float* vertices; /* V x 3 */
+  int32_t num_faces;
+  const int32_t* faces;        /* F x 3 (vertex_indices) */
+  const float* face_normals;   /* F x 3 or null */
+  int32_t num_normals;
+  const float* normals;        /* N x 3 or null */
+  const int32_t* normal_indices; /* F x 3 or null */
+  int32_t num_vertex_colors;
+  const float* vertex_colors;  /* V x 3, 0..255 scale, or null */
+  int32_t num_uv;
+  const float* uv;             /* U x 2 or null */
+  const int32_t* uv_indices;   /* F x 3 or null */
+  const int32_t* material_ids; /* F or null */
+  int32_t num_materials;
+  const UguTexture* textures;  /* num_materials entries or null */
+} UguMeshDesc;
+
+/* Pinhole camera, everything already reduced to float on the host:
+ *   w2c_R = camera->w2c().rotation().cast<float>()   (rasterizer.cc:118)
+ *   w2c_t = camera->w2c().translation().cast<float>() (rasterizer.cc:119)
+ *   c2w_R = c2w.matrix().block<3,3>(0,0).cast<float>() (camera.h:322)
+ * Matrices are row-major. (cx, cy) is the principal point, (fx, fy) the
+ * focal length, both in pixels (camera.h:380-387). */
+typedef struct {
+  int32_t width;
+  int32_t height;
+  float fx, fy, cx, cy;
+  float w2c_R[9];
+  float w2c_t[3];
+  float c2w_R[9];
+} UguCamera;
+
+/* RendererCpuOption (include/ugu/renderer/cpu/renderer.h:17-38) with the
+ * enum values of include/ugu/renderer/base.h:11-32 and common.h:50-53.
+ * As in the reference, depth_scale and oren_nayar_sigma have no effect on
+ * Render (RendererCpuOption::CopyTo never copies the sigma). */
+typedef struct {
+  int32_t diffuse_color;   /* 0 kNone, 1 kTexture, 2 kVertex */
+  int32_t interp;          /* 0 kNn, 1 kBilinear */
+  int32_t shading_normal;  /* 0 kFace, 1 kVertex */
+  int32_t diffuse_shading; /* 0 kNone, 1 kLambertian, 2 kOrenNayar */
+  float depth_scale;
+  int32_t backface_culling;
+  float oren_nayar_sigma;
+} UguOptions;
+
+/* Output images for one view; null = not requested (as the reference's
+ * nullptr arguments). Layouts are interleaved HWC, row-major:
+ * color W*H*3 u8 (RGB), depth W*H f32, normal W*H*3 f32, mask W*H u8,
+ * face_id W*H i32 (-1 = no face). */
+typedef struct {
+  uint8_t* color;
+  float* depth;
+  float* normal;
+  uint8_t* mask;
+  int32_t* face_id;
+} UguOutputs;
+
+/* Tile geometry of the binning rasterizer. tile_w x tile_h must be one of
+ * 8x8, 16x16, 32x8. bin_capacity = face slots per tile (grown automatically
+ * when a render overflows it). */
+typedef struct {
+  int32_t tile_w;
+  int32_t tile_h;
+  int32_t bin_capacity;
+  int32_t persistent; /* 1: batched raster pulls (view, tile) items from a queue */
+} UguTiling;
+
+/* Per-kernel GPU time of the last render call, from hipEvents recorded on the
+ * context's stream (filled only when profiling is enabled). */
+typedef struct {
+  float clear_ms;
+  float project_ms;
+  float setup_bin_ms;
+  float raster_ms;
+  float total_ms;
+  int32_t max_bin_count;
+  int32_t retries;
+} UguTimings;
+
+/* Context lifetime. device = HIP device ordinal. Returns null on failure. */
+ugu_hip_ctx* ugu_hip_create(int device);
+void ugu_hip_destroy(ugu_hip_ctx* ctx);
+const char* ugu_hip_last_error(const ugu_hip_ctx* ctx);
+int ugu_hip_abi_version(void);
+
+/* Use a caller-provided hipStream_t (e.g. torch's current stream); null
+ * restores the context's own stream. */
+int ugu_hip_set_stream(ugu_hip_ctx* ctx, void* hip_stream);
+
+/* set_mesh + PrepareMesh: validates indices and uploads the mesh. The device
+ * copy stays valid until the next call (the reference's contract "don't
+ * modify mesh after PrepareMesh", renderer.h:52-53). */
+int ugu_hip_prepare_mesh(ugu_hip_ctx* ctx, const UguMeshDesc* mesh);
+
+/* Render one view (Rasterizer::Render). memory = where the output pointers
+ * live. */
+int ugu_hip_render(ugu_hip_ctx* ctx, const UguCamera* camera,
+                   const UguOptions* options, const UguOutputs* outputs,
+                   int memory);
+
+/* Render n views of possibly different resolutions in one batched pipeline
+ * (clear, project, setup+bin, raster launched back to back, captured once in
+ * a hipGraph and replayed). outputs[i] belongs to cameras[i]. Results are
+ * bitwise identical to n separate ugu_hip_render calls. */
+int ugu_hip_render_batch(ugu_hip_ctx* ctx, const UguCamera* cameras, int n,
+                         const UguOptions* options, const UguOutputs* outputs,
+                         int memory);
+
+/* Same as ugu_hip_render_batch without the graph: the per-kernel launches are
+ * issued directly on the stream (used for per-kernel timing). */
+int ugu_hip_render_batch_eager(ugu_hip_ctx* ctx, const UguCamera* cameras,
+                               int n, const UguOptions* options,
+                               const UguOutputs* outputs, int memory);
+
+/* Tiling control and the autotuner. ugu_hip_autotune renders the given views
+ * with every tile size, sizes the bin capacity from the measured maximum bin
+ * occupancy, keeps the fastest and returns it in *chosen. */
+int ugu_hip_set_tiling(ugu_hip_ctx* ctx, const UguTiling* tiling);
+int ugu_hip_get_tiling(const ugu_hip_ctx* ctx, UguTiling* tiling);
+int ugu_hip_autotune(ugu_hip_ctx* ctx, const UguCamera* cameras, int n,
+                     const UguOptions* options, int repeats,
+                     UguTiling* chosen);
+
+/* Profiling (hipEvents around each kernel on the context's stream). */
+int ugu_hip_enable_profiling(ugu_hip_ctx* ctx, int enable);
+int ugu_hip_get_timings(const ugu_hip_ctx* ctx, UguTimings* timings);
+
+/* RenderW's depth conversion: out[i] = uint16(1.0 * (double)depth[i]),
+ * truncating, as ConvertTo (image_proc.h:453-520). memory as above. */
+int ugu_hip_depth_to_u16(ugu_hip_ctx* ctx, const float* depth, int64_t n,
+                         uint16_t* out, int memory);
+
+/* ---- differentiable rendering (no reference counterpart) ----------------
+ * Uses the prepared mesh's topology (faces, uv, uv_indices) with parameters
+ * given as device float32 arrays:
+ *   vertices      V x 3   positions (visibility is computed from these)
+ *   vertex_colors V x 3   or null
+ *   texture       Th x Tw x 3 (row-major, same orientation as an Image3b)
+ *                 or null
+ * Image model per covered pixel (fid > 0 and not culled, as the reference's
+ * shading rule), with u = w1, v = w2 the perspective-correct barycentrics:
+ *   c   = ((1-u)-v)*C0 + u*C1 + v*C2                     (vertex colour term)
+ *   t   = bilinear(texture, uv(u, v))                    (texture term)
+ *   img = c*t if both present, else whichever is present; 0 elsewhere.
+ * Face normals for the backface test are recomputed from `vertices`. */
+typedef struct {
+  const float* vertices;
+  const float* vertex_colors;
+  const float* texture;
+  int32_t tex_width;
+  int32_t tex_height;
+  int32_t backface_culling;
+} UguDiffParams;
+
+/* images[i]: device H_i x W_i x 3 f32; face_ids[i]: device H_i x W_i i32 (the
+ * visibility kept for the backward pass). */
+int ugu_hip_diff_forward(ugu_hip_ctx* ctx, const UguCamera* cameras, int n,
+                         const UguDiffParams* params, float* const* images,
+                         int32_t* const* face_ids);
+
+/* Accumulates (+=) dL/d(vertices), dL/d(vertex_colors), dL/d(texture) given
+ * dL/d(images) (device, same layout as images) and the face_ids of the
+ * forward pass, at fixed visibility. Null gradient pointers are skipped. */
+int ugu_hip_diff_backward(ugu_hip_ctx* ctx, const UguCamera* cameras, int n,
+                          const UguDiffParams* params,
+                          const float* const* grad_images,
+                          const int32_t* const* face_ids, float* grad_vertices,
+                          float* grad_vertex_colors, float* grad_texture);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UGU_HIP_H_ */

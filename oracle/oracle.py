@@ -1,0 +1,204 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY. Imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg -- as the checker or the CPU baseline, never by
+the product package ugu_amd.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_P = ctypes.c_void_p
+_i = ctypes.c_int32
+_f = ctypes.c_float
+
+
+class OracleMesh(ctypes.Structure):
+    _fields_ = [
+        ("num_vertices", _i), ("vertices", _P),
+        ("num_faces", _i), ("faces", _P),
+        ("face_normals", _P),
+        ("num_normals", _i), ("normals", _P), ("normal_indices", _P),
+        ("num_vertex_colors", _i), ("vertex_colors", _P),
+        ("num_uv", _i), ("uv", _P), ("uv_indices", _P),
+        ("material_ids", _P),
+        ("num_materials", _i), ("tex_width", _P), ("tex_height", _P), ("tex_data", _P),
+    ]
+
+
+class OracleCamera(ctypes.Structure):
+    _fields_ = [("width", _i), ("height", _i), ("fx", _f), ("fy", _f), ("cx", _f), ("cy", _f),
+                ("w2c_R", _f * 9), ("w2c_t", _f * 3), ("c2w_R", _f * 9)]
+
+
+class OracleOptions(ctypes.Structure):
+    _fields_ = [("diffuse_color", _i), ("interp", _i), ("shading_normal", _i),
+                ("diffuse_shading", _i), ("depth_scale", _f), ("backface_culling", _i),
+                ("oren_nayar_sigma", _f)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_render.restype = ctypes.c_int
+        L.oracle_render.argtypes = [ctypes.POINTER(OracleMesh), ctypes.POINTER(OracleCamera), _P,
+                                    ctypes.POINTER(OracleOptions), _P, _P, _P, _P, _P,
+                                    ctypes.POINTER(ctypes.c_double)]
+        L.oracle_depth_to_u16.restype = None
+        L.oracle_depth_to_u16.argtypes = [_P, ctypes.c_int64, _P]
+        L.oracle_edge_function.restype = ctypes.c_float
+        L.oracle_edge_function.argtypes = [_P, _P, _P]
+        L.oracle_ray_w.restype = None
+        L.oracle_ray_w.argtypes = [ctypes.POINTER(OracleCamera), _f, _f, _P]
+        L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_num_threads.argtypes = []
+        if hasattr(L, "oracle_diff_forward_f64"):
+            L.oracle_diff_forward_f64.restype = ctypes.c_int
+            L.oracle_diff_backward_f64.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def camera_struct(w2c_R, w2c_t, c2w_R, width, height, fx, fy, cx, cy):
+    c = OracleCamera()
+    c.width, c.height = int(width), int(height)
+    c.fx, c.fy, c.cx, c.cy = float(fx), float(fy), float(cx), float(cy)
+    for i, x in enumerate(np.asarray(w2c_R, np.float32).reshape(-1)):
+        c.w2c_R[i] = float(x)
+    for i, x in enumerate(np.asarray(w2c_t, np.float32).reshape(-1)):
+        c.w2c_t[i] = float(x)
+    for i, x in enumerate(np.asarray(c2w_R, np.float32).reshape(-1)):
+        c.c2w_R[i] = float(x)
+    return c
+
+
+def camera_from_abi(abi_cam):
+    """Same float values as an ugu_amd._abi.UguCamera."""
+    c = OracleCamera()
+    c.width, c.height = abi_cam.width, abi_cam.height
+    c.fx, c.fy, c.cx, c.cy = abi_cam.fx, abi_cam.fy, abi_cam.cx, abi_cam.cy
+    for i in range(9):
+        c.w2c_R[i] = abi_cam.w2c_R[i]
+        c.c2w_R[i] = abi_cam.c2w_R[i]
+    for i in range(3):
+        c.w2c_t[i] = abi_cam.w2c_t[i]
+    return c
+
+
+class _MeshHold:
+    def __init__(self, mesh):
+        self.keep = []
+
+        def arr(a, dt):
+            if a is None or len(a) == 0:
+                return None
+            a = np.ascontiguousarray(a, dt)
+            self.keep.append(a)
+            return a
+
+        m = OracleMesh()
+        v = arr(mesh.vertices, np.float32)
+        f = arr(mesh.vertex_indices, np.int32)
+        m.num_vertices = 0 if v is None else len(v)
+        m.vertices = _p(v)
+        m.num_faces = 0 if f is None else len(f)
+        m.faces = _p(f)
+        m.face_normals = _p(arr(mesh.face_normals, np.float32))
+        n = arr(mesh.normals, np.float32)
+        m.num_normals = 0 if n is None else len(n)
+        m.normals = _p(n)
+        m.normal_indices = _p(arr(mesh.normal_indices, np.int32))
+        vc = arr(mesh.vertex_colors, np.float32)
+        m.num_vertex_colors = 0 if vc is None else len(vc)
+        m.vertex_colors = _p(vc)
+        uv = arr(mesh.uv, np.float32)
+        m.num_uv = 0 if uv is None else len(uv)
+        m.uv = _p(uv)
+        m.uv_indices = _p(arr(mesh.uv_indices, np.int32))
+        m.material_ids = _p(arr(mesh.material_ids, np.int32))
+        nm = len(mesh.materials)
+        m.num_materials = nm
+        if nm:
+            tw = np.array([t.shape[1] if np.size(t) else 0 for t in mesh.materials], np.int32)
+            th = np.array([t.shape[0] if np.size(t) else 0 for t in mesh.materials], np.int32)
+            texs = [np.ascontiguousarray(t, np.uint8) for t in mesh.materials]
+            ptrs = (ctypes.c_void_p * nm)(*[t.ctypes.data if t.size else None for t in texs])
+            self.keep += [tw, th, texs, ptrs]
+            m.tex_width, m.tex_height = _p(tw), _p(th)
+            m.tex_data = ctypes.cast(ptrs, _P)
+        self.m = m
+
+
+def options_struct(o):
+    s = OracleOptions()
+    s.diffuse_color = int(o.diffuse_color)
+    s.interp = int(o.interp)
+    s.shading_normal = int(o.shading_normal)
+    s.diffuse_shading = int(o.diffuse_shading)
+    s.depth_scale = float(o.depth_scale)
+    s.backface_culling = 1 if o.backface_culling else 0
+    s.oren_nayar_sigma = float(o.oren_nayar_sigma)
+    return s
+
+
+def render(mesh, cam, options, c2w_t=None, outputs=("color", "depth", "normal", "mask", "face_id")):
+    """Oracle Render(): returns (rc, dict of numpy outputs, elapsed_ms).
+    `cam` is an OracleCamera (see camera_from_abi / camera_struct)."""
+    h, w = cam.height, cam.width
+    out = {}
+    if "color" in outputs:
+        out["color"] = np.zeros((h, w, 3), np.uint8)
+    if "depth" in outputs:
+        out["depth"] = np.zeros((h, w), np.float32)
+    if "normal" in outputs:
+        out["normal"] = np.zeros((h, w, 3), np.float32)
+    if "mask" in outputs:
+        out["mask"] = np.zeros((h, w), np.uint8)
+    if "face_id" in outputs:
+        out["face_id"] = np.zeros((h, w), np.int32)
+    hold = _MeshHold(mesh)
+    opt = options_struct(options)
+    ct = None if c2w_t is None else np.ascontiguousarray(c2w_t, np.float32)
+    ms = ctypes.c_double(0.0)
+    rc = lib().oracle_render(ctypes.byref(hold.m), ctypes.byref(cam), _p(ct), ctypes.byref(opt),
+                             _p(out.get("color")), _p(out.get("depth")), _p(out.get("normal")),
+                             _p(out.get("mask")), _p(out.get("face_id")), ctypes.byref(ms))
+    return rc, out, ms.value
+
+
+def depth_to_u16(depth):
+    d = np.ascontiguousarray(depth, np.float32)
+    out = np.empty(d.shape, np.uint16)
+    lib().oracle_depth_to_u16(_p(d), d.size, _p(out))
+    return out
+
+
+def edge_function(a, b, c):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    c = np.asarray(c, np.float32)
+    return lib().oracle_edge_function(_p(a), _p(b), _p(c))
+
+
+def ray_w(cam, x, y):
+    out = np.zeros(3, np.float32)
+    lib().oracle_ray_w(ctypes.byref(cam), float(x), float(y), _p(out))
+    return out
+
+
+def num_threads():
+    return lib().oracle_num_threads()

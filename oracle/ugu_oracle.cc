@@ -1,0 +1,574 @@
+// CPU restatement of UGU's software rasterizer, used ONLY as the test oracle
+// and as the CPU baseline of the benchmark. Nothing in the product library
+// links or calls this file: tests, the smoke check and the benchmark's
+// cpu_baseline leg load it as a checker.
+//
+// It restates, expression by expression, with fp-contract off:
+//   src/renderer/cpu/rasterizer.cc:105-307          Rasterizer::Impl::Render
+//   src/renderer/cpu/rasterizer.cc:25-28            EdgeFunction
+//   src/renderer/cpu/util_private.cc:12-89          ValidateAndInitBeforeRender
+//   include/ugu/renderer/cpu/pixel_shader.h:143-422 colourizers and shaders
+//   include/ugu/camera.h:380-387                    PinholeCamera::Project
+//   include/ugu/camera.h:438-448,482-513            ray_c/ray_w and InitRayTable
+//   include/ugu/image_proc.h:453-520                ConvertTo (RenderW)
+//
+// Eigen evaluation order. Fixed-size 3-term reductions (dot, squaredNorm,
+// each row of Matrix3f*Vector3f) are evaluated by Eigen's unrolled redux as
+// x0 + (x1 + x2); this is isolated in sum3() below. Element-wise expressions
+// (a*x + b*y + c*z per coefficient) are plain left-to-right.
+//
+// Float->int conversions follow what GCC emits on x86-64 for the reference's
+// static_casts (cvttss2si: truncation, INT32_MIN for NaN/out of range; the
+// narrow casts keep the low bits). Two documented departures, both where the
+// reference has undefined behaviour:
+//   * the pixel bbox upper bound uses a saturating conversion (the reference's
+//     int32 overflow turns into a 2^31-long loop writing wrapped pixels);
+//   * texture reads whose linear index falls outside the image are clamped to
+//     the image (the reference reads outside the buffer there).
+//
+// Parity pins for this file (closed forms, brute force, invariants) live in
+// tests/test_oracle_*.py.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <chrono>
+#include <limits>
+#include <vector>
+#include <algorithm>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#if defined(__FP_FAST_FMAF) || defined(__FAST_MATH__)
+#error "oracle must be compiled without FMA contraction or fast-math"
+#endif
+
+extern "C" {
+
+struct OracleMesh {
+  int32_t num_vertices;
+  const float* vertices;        // V x 3
+  int32_t num_faces;
+  const int32_t* faces;         // F x 3
+  const float* face_normals;    // F x 3 or null
+  int32_t num_normals;
+  const float* normals;         // N x 3 or null
+  const int32_t* normal_indices;// F x 3 or null
+  int32_t num_vertex_colors;
+  const float* vertex_colors;   // V x 3 or null (0..255 scale)
+  int32_t num_uv;
+  const float* uv;              // U x 2 or null
+  const int32_t* uv_indices;    // F x 3 or null
+  const int32_t* material_ids;  // F or null
+  int32_t num_materials;
+  const int32_t* tex_width;     // per material
+  const int32_t* tex_height;    // per material
+  const uint8_t* const* tex_data;  // per material, H x W x 3 RGB
+};
+
+struct OracleCamera {
+  int32_t width, height;
+  float fx, fy, cx, cy;
+  float w2c_R[9];  // row-major, = camera->w2c().rotation().cast<float>()
+  float w2c_t[3];  // = camera->w2c().translation().cast<float>()
+  float c2w_R[9];  // row-major, = c2w.matrix().block<3,3>(0,0).cast<float>()
+};
+
+struct OracleOptions {
+  int32_t diffuse_color;    // 0 none, 1 texture, 2 vertex
+  int32_t interp;           // 0 nn, 1 bilinear
+  int32_t shading_normal;   // 0 face, 1 vertex
+  int32_t diffuse_shading;  // 0 none, 1 lambertian, 2 oren-nayar
+  float depth_scale;
+  int32_t backface_culling;
+  float oren_nayar_sigma;
+};
+
+}  // extern "C"
+
+namespace {
+
+// error codes mirror the reference's LOGE branches (util_private.cc:19-66)
+enum {
+  kOk = 0,
+  kErrNoCamera = -1,
+  kErrMeshNotInitialized = -2,
+  kErrCullingNoFaceNormal = -3,
+  kErrTextureEmpty = -4,
+  kErrTextureNthEmpty = -5,
+  kErrVertexColorEmpty = -6,
+  kErrFaceNormalEmpty = -7,
+  kErrVertexNormalEmpty = -8,
+  kErrAllNull = -9,
+  kErrDepthNull = -10,
+};
+
+struct V3 {
+  float v[3];
+  float& operator[](int i) { return v[i]; }
+  const float& operator[](int i) const { return v[i]; }
+};
+
+// Eigen's unrolled 3-term reduction: x0 + (x1 + x2)
+inline float sum3(float a, float b, float c) { return a + (b + c); }
+
+inline float dot3(const V3& a, const V3& b) {
+  return sum3(a[0] * b[0], a[1] * b[1], a[2] * b[2]);
+}
+
+inline V3 matvec(const float* R, const V3& x) {
+  V3 r;
+  for (int i = 0; i < 3; i++)
+    r[i] = sum3(R[i * 3 + 0] * x[0], R[i * 3 + 1] * x[1], R[i * 3 + 2] * x[2]);
+  return r;
+}
+
+// x86-64 cvttss2si: truncate, INT32_MIN for NaN / out of range.
+inline int32_t f2i_x86(float f) {
+  if (f >= -2147483648.0f && f < 2147483648.0f) return static_cast<int32_t>(f);
+  return std::numeric_limits<int32_t>::min();
+}
+inline int32_t d2i_x86(double f) {
+  if (f >= -2147483648.0 && f < 2147483648.0) return static_cast<int32_t>(f);
+  return std::numeric_limits<int32_t>::min();
+}
+// saturating conversion used for the pixel bbox (see header comment)
+inline int32_t f2i_sat(float f) {
+  if (f >= 2147483648.0f) return std::numeric_limits<int32_t>::max();
+  if (f >= -2147483648.0f) return static_cast<int32_t>(f);
+  return std::numeric_limits<int32_t>::min();  // also NaN
+}
+inline uint8_t f2u8_x86(float f) {
+  return static_cast<uint8_t>(static_cast<uint32_t>(f2i_x86(f)) & 0xffu);
+}
+
+// rasterizer.cc:25-28
+inline float EdgeFunction(const V3& a, const V3& b, const V3& c) {
+  return (c[0] - a[0]) * (b[1] - a[1]) - (c[1] - a[1]) * (b[0] - a[0]);
+}
+
+// std::min({a,b,c}) / std::max({a,b,c}) (min_element/max_element order)
+inline float min3_std(float a, float b, float c) {
+  float m = a;
+  if (b < m) m = b;
+  if (c < m) m = c;
+  return m;
+}
+inline float max3_std(float a, float b, float c) {
+  float m = a;
+  if (m < b) m = b;
+  if (m < c) m = c;
+  return m;
+}
+
+V3 load3(const float* p, int64_t i) {
+  V3 r;
+  r[0] = p[i * 3 + 0];
+  r[1] = p[i * 3 + 1];
+  r[2] = p[i * 3 + 2];
+  return r;
+}
+
+struct Tex {
+  int32_t w, h;
+  const uint8_t* d;
+  // ImageBase::at (image.h:416-427) with the linear index clamped to the image
+  const uint8_t* at(int32_t row, int32_t col) const {
+    int64_t lin = static_cast<int64_t>(row) * w + col;
+    int64_t n = static_cast<int64_t>(w) * h;
+    if (lin < 0) lin = 0;
+    if (lin >= n) lin = n - 1;
+    return d + lin * 3;
+  }
+};
+
+// camera.h:438-448 (ray_c then c2w_R_f * dir), evaluated for one pixel
+inline V3 ray_w_at(const OracleCamera& cam, float x, float y) {
+  V3 d;
+  d[0] = (x - cam.cx) / cam.fx;
+  d[1] = (y - cam.cy) / cam.fy;
+  d[2] = 1.0f;
+  // Eigen normalize(): z = squaredNorm(); if (z > 0) *this /= sqrt(z)
+  float z = sum3(d[0] * d[0], d[1] * d[1], d[2] * d[2]);
+  if (z > 0.0f) {
+    float s = std::sqrt(z);
+    d[0] = d[0] / s;
+    d[1] = d[1] / s;
+    d[2] = d[2] / s;
+  }
+  return matvec(cam.c2w_R, d);
+}
+
+// Per-pixel tables exactly as PinholeCamera::InitRayTable (camera.h:482-513):
+// all four tables are filled (OpenMP over rows) so the CPU timing matches
+// the reference's work.
+struct RayTables {
+  std::vector<V3> org_c, org_w, ray_c, ray_w;
+};
+
+void InitRayTable(const OracleCamera& cam, const V3& c2w_t, RayTables* t) {
+  const size_t n = static_cast<size_t>(cam.width) * cam.height;
+  t->org_c.resize(n);
+  t->org_w.resize(n);
+  t->ray_c.resize(n);
+  t->ray_w.resize(n);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int y = 0; y < cam.height; y++) {
+    for (int x = 0; x < cam.width; x++) {
+      size_t i = static_cast<size_t>(y) * cam.width + x;
+      t->org_c[i] = V3{{0.0f, 0.0f, 0.0f}};
+      t->org_w[i] = c2w_t;
+      V3 d;
+      d[0] = (static_cast<float>(x) - cam.cx) / cam.fx;
+      d[1] = (static_cast<float>(y) - cam.cy) / cam.fy;
+      d[2] = 1.0f;
+      float z = sum3(d[0] * d[0], d[1] * d[1], d[2] * d[2]);
+      if (z > 0.0f) {
+        float s = std::sqrt(z);
+        d[0] = d[0] / s;
+        d[1] = d[1] / s;
+        d[2] = d[2] / s;
+      }
+      t->ray_c[i] = d;
+      t->ray_w[i] = ray_w_at(cam, static_cast<float>(x), static_cast<float>(y));
+    }
+  }
+}
+
+// pixel_shader.h:143-148
+struct OrenNayarParam {
+  float sigma, A, B;
+  explicit OrenNayarParam(float s) : sigma(s) {
+    float sigma_2 = sigma * sigma;
+    A = 1.0f - (0.5f * sigma_2 / (sigma_2 + 0.33f));
+    B = 0.45f * sigma_2 / (sigma_2 + 0.09f);
+  }
+};
+
+struct ShadeCtx {
+  const OracleMesh* m;
+  const OracleOptions* o;
+  std::vector<Tex> tex;
+  OrenNayarParam on;
+  explicit ShadeCtx(float sigma) : on(sigma) {}
+};
+
+// interp of a 2-vector: (1-u-v)*a + u*b + v*c per coefficient (pixel_shader.h:277-279)
+inline void interp_uv(const OracleMesh& m, int fid, float u, float v, float out[2]) {
+  const int32_t* ui = m.uv_indices + static_cast<int64_t>(fid) * 3;
+  const float* a = m.uv + static_cast<int64_t>(ui[0]) * 2;
+  const float* b = m.uv + static_cast<int64_t>(ui[1]) * 2;
+  const float* c = m.uv + static_cast<int64_t>(ui[2]) * 2;
+  float w = 1.0f - u - v;
+  for (int k = 0; k < 2; k++) out[k] = w * a[k] + u * b[k] + v * c[k];
+}
+
+// Colourizers (pixel_shader.h:222-359) and shaders (:363-422) for one pixel.
+void ShadePixel(const ShadeCtx& s, uint8_t* c, float u, float v, int fid,
+                const V3& ray_w, const V3& light_dir, const V3& n) {
+  const OracleMesh& m = *s.m;
+  // colourizer
+  if (s.o->diffuse_color == 2) {
+    // DiffuseVertexColorColorizer (:234-256)
+    const int32_t* f = m.faces + static_cast<int64_t>(fid) * 3;
+    V3 c0 = load3(m.vertex_colors, f[0]);
+    V3 c1 = load3(m.vertex_colors, f[1]);
+    V3 c2 = load3(m.vertex_colors, f[2]);
+    float w = 1.0f - u - v;
+    for (int k = 0; k < 3; k++) {
+      float ic = w * c0[k] + u * c1[k] + v * c2[k];
+      c[k] = f2u8_x86(ic);
+    }
+  } else if (s.o->diffuse_color == 1) {
+    const Tex& t = s.tex[m.material_ids[fid]];
+    float iuv[2];
+    interp_uv(m, fid, u, v, iuv);
+    float fx = iuv[0] * static_cast<float>(t.w - 1);
+    float fy = (1.0f - iuv[1]) * static_cast<float>(t.h - 1);
+    if (s.o->interp == 0) {
+      // DiffuseTextureNnColorizer (:260-298)
+      int tx = f2i_x86(std::round(fx));
+      int ty = f2i_x86(std::round(fy));
+      const uint8_t* dt = t.at(ty, tx);
+      for (int k = 0; k < 3; k++) {
+        float ic = static_cast<float>(dt[k]);
+        c[k] = f2u8_x86(ic);
+      }
+    } else {
+      // DiffuseTextureBilinearColorizer (:302-359)
+      int x0 = f2i_x86(std::floor(fx));
+      int y0 = f2i_x86(std::floor(fy));
+      int x1 = x0 + 1;
+      int y1 = y0 + 1;
+      float lu = fx - static_cast<float>(x0);
+      float lv = fy - static_cast<float>(y0);
+      const uint8_t* a = t.at(y0, x0);
+      const uint8_t* b = t.at(y0, x1);
+      const uint8_t* cc = t.at(y1, x0);
+      const uint8_t* d = t.at(y1, x1);
+      for (int k = 0; k < 3; k++) {
+        float ic = (1.0f - lu) * (1.0f - lv) * static_cast<float>(a[k]) +
+                   lu * (1.0f - lv) * static_cast<float>(b[k]) +
+                   (1.0f - lu) * lv * static_cast<float>(cc[k]) +
+                   lu * lv * static_cast<float>(d[k]);
+        c[k] = f2u8_x86(ic);
+      }
+    }
+  } else {
+    // DiffuseDefaultColorizer (:222-230)
+    c[0] = c[1] = c[2] = 255;
+  }
+  // shader
+  if (s.o->diffuse_shading == 1) {
+    // DiffuseLambertianShader (:370-388)
+    float coeff = -dot3(light_dir, n);
+    if (coeff < 0.0f) coeff = 0.0f;
+    for (int k = 0; k < 3; k++) c[k] = f2u8_x86(coeff * static_cast<float>(c[k]));
+  } else if (s.o->diffuse_shading == 2) {
+    // DiffuseOrenNayarShader (:392-422)
+    float dot_light = -dot3(light_dir, n);
+    float theta_i = std::acos(dot_light);
+    float dot_ray = -dot3(ray_w, n);
+    float theta_r = std::acos(dot_ray);
+    V3 bl, br;
+    for (int k = 0; k < 3; k++) {
+      bl[k] = -n[k] * dot_light - light_dir[k];
+      br[k] = -n[k] * dot_light - ray_w[k];
+    }
+    float phi_diff_cos = std::max(0.0f, dot3(bl, br));
+    float alpha = std::max(theta_i, theta_r);
+    float beta = std::min(theta_i, theta_r);
+    float A = s.on.A, B = s.on.B;
+    float coeff = std::max(0.0f, dot_light) *
+                  (A + (B * phi_diff_cos * std::sin(alpha) * std::tan(beta)));
+    for (int k = 0; k < 3; k++) c[k] = f2u8_x86(coeff * static_cast<float>(c[k]));
+  }
+}
+
+int Validate(const OracleMesh& m, const OracleOptions& o, bool any_output) {
+  // util_private.cc:12-66 (camera and PrepareMesh are always present here)
+  if (o.backface_culling && m.face_normals == nullptr) return kErrCullingNoFaceNormal;
+  if (o.diffuse_color == 1 && m.num_materials <= 0) return kErrTextureEmpty;
+  if (o.diffuse_color == 1) {
+    for (int i = 0; i < m.num_materials; i++)
+      if (m.tex_width[i] < 1 || m.tex_height[i] < 1 || m.tex_data[i] == nullptr)
+        return kErrTextureNthEmpty;
+  }
+  if (o.diffuse_color == 2 && m.vertex_colors == nullptr) return kErrVertexColorEmpty;
+  if (o.shading_normal == 0 && m.face_normals == nullptr) return kErrFaceNormalEmpty;
+  if (o.shading_normal == 1 && m.normals == nullptr) return kErrVertexNormalEmpty;
+  if (!any_output) return kErrAllNull;
+  return kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Restatement of Rasterizer::Impl::Render (rasterizer.cc:105-307).
+// Output arrays are caller-owned, sized W*H*{3,1,3,1,1}; null = not requested.
+// c2w_t (camera position) only feeds the org_ray_w table, as in InitRayTable.
+// Returns 0 or a negative error code; *elapsed_ms receives the wall time of
+// the main loop (rasterizer.cc:121-122,303-304), ray table included.
+int oracle_render(const OracleMesh* mesh, const OracleCamera* cam,
+                  const float* c2w_t, const OracleOptions* opt, uint8_t* color,
+                  float* depth, float* normal, uint8_t* mask, int32_t* face_id,
+                  double* elapsed_ms) {
+  if (cam == nullptr) return kErrNoCamera;
+  if (mesh == nullptr) return kErrMeshNotInitialized;
+  const OracleMesh& m = *mesh;
+  const OracleOptions& o = *opt;
+  int err = Validate(m, o, color || depth || normal || mask || face_id);
+  if (err) return err;
+
+  const int W = cam->width, H = cam->height;
+  const size_t npx = static_cast<size_t>(W) * H;
+  // ValidateAndInitBeforeRender: colour 0, depth 0, normal 0, mask 0, fid -1
+  if (color) std::memset(color, 0, npx * 3);
+  if (depth) std::fill(depth, depth + npx, 0.0f);
+  if (normal) std::fill(normal, normal + npx * 3, 0.0f);
+  if (mask) std::memset(mask, 0, npx);
+  if (face_id) std::fill(face_id, face_id + npx, -1);
+
+  ShadeCtx sc(0.3f);  // CopyTo never copies oren_nayar_sigma (renderer.h:30-37)
+  sc.m = mesh;
+  sc.o = opt;
+  for (int i = 0; i < m.num_materials; i++)
+    sc.tex.push_back(Tex{m.tex_width[i], m.tex_height[i], m.tex_data[i]});
+
+  auto t0 = std::chrono::steady_clock::now();
+
+  RayTables rt;
+  V3 ct{{c2w_t ? c2w_t[0] : 0.0f, c2w_t ? c2w_t[1] : 0.0f, c2w_t ? c2w_t[2] : 0.0f}};
+  InitRayTable(*cam, ct, &rt);
+
+  // vertex loop (rasterizer.cc:125-136)
+  const int V = m.num_vertices;
+  std::vector<V3> camera_vertices(V), camera_normals(V), image_vertices(V);
+  std::vector<float> camera_depth_list(V);
+  for (int i = 0; i < V; i++) {
+    V3 rv = matvec(cam->w2c_R, load3(m.vertices, i));
+    for (int k = 0; k < 3; k++) camera_vertices[i][k] = rv[k] + cam->w2c_t[k];
+    // the reference also rotates normals[i] (unused); only valid indices here
+    if (m.normals && i < m.num_normals)
+      camera_normals[i] = matvec(cam->w2c_R, load3(m.normals, i));
+    camera_depth_list[i] = camera_vertices[i][2];
+    // PinholeCamera::Project (camera.h:380-387): divide first, then multiply
+    const V3& p = camera_vertices[i];
+    image_vertices[i][0] = cam->fx / p[2] * p[0] + cam->cx;
+    image_vertices[i][1] = cam->fy / p[2] * p[1] + cam->cy;
+    image_vertices[i][2] = p[2];
+  }
+
+  std::vector<float> depth_internal;
+  float* depth_ = depth;
+  if (depth_ == nullptr) {
+    depth_internal.assign(npx, 0.0f);
+    depth_ = depth_internal.data();
+  }
+  std::vector<int32_t> fid_internal;
+  int32_t* face_id_ = face_id;
+  if (face_id_ == nullptr) {
+    fid_internal.assign(npx, -1);
+    face_id_ = fid_internal.data();
+  }
+  std::vector<uint8_t> backface_image(npx, 0);
+  std::vector<float> weight_image(npx * 3, 0.0f);
+
+  // face loop (rasterizer.cc:161-241)
+  const float flt_min = std::numeric_limits<float>::min();
+  for (int i = 0; i < m.num_faces; i++) {
+    const int32_t* face = m.faces + static_cast<int64_t>(i) * 3;
+    const V3& v0_i = image_vertices[face[0]];
+    const V3& v1_i = image_vertices[face[1]];
+    const V3& v2_i = image_vertices[face[2]];
+    if (v0_i[2] < 0.0f || v1_i[2] < 0.0f || v2_i[2] < 0.0f) continue;
+    float xmin = min3_std(v0_i[0], v1_i[0], v2_i[0]);
+    float ymin = min3_std(v0_i[1], v1_i[1], v2_i[1]);
+    float xmax = max3_std(v0_i[0], v1_i[0], v2_i[0]);
+    float ymax = max3_std(v0_i[1], v1_i[1], v2_i[1]);
+    if (xmin > static_cast<float>(W - 1) || xmax < 0 ||
+        ymin > static_cast<float>(H - 1) || ymax < 0)
+      continue;
+    int32_t x0 = std::max(int32_t(0), f2i_x86(std::ceil(xmin)));
+    int32_t x1 = std::min(W - 1, f2i_sat(std::floor(xmax)));
+    int32_t y0 = std::max(int32_t(0), f2i_x86(std::ceil(ymin)));
+    int32_t y1 = std::min(H - 1, f2i_sat(std::floor(ymax)));
+    float area = EdgeFunction(v0_i, v1_i, v2_i);
+    if (std::abs(area) < flt_min) continue;
+    const V3 fn = load3(m.face_normals ? m.face_normals : m.vertices, i);
+    for (int32_t y = y0; y <= y1; ++y) {
+      for (int32_t x = x0; x <= x1; ++x) {
+        const V3& ray_w = rt.ray_w[static_cast<size_t>(y) * W + x];
+        bool backface = m.face_normals ? dot3(fn, ray_w) > 0 : false;
+        V3 pixel_sample{{static_cast<float>(x), static_cast<float>(y), 0.0f}};
+        float w0 = EdgeFunction(v1_i, v2_i, pixel_sample);
+        float w1 = EdgeFunction(v2_i, v0_i, pixel_sample);
+        float w2 = EdgeFunction(v0_i, v1_i, pixel_sample);
+        if ((!backface && (w0 >= 0 && w1 >= 0 && w2 >= 0)) ||
+            (backface && (w0 <= 0 && w1 <= 0 && w2 <= 0))) {
+          w0 /= area;
+          w1 /= area;
+          w2 /= area;
+          w0 /= v0_i[2];
+          w1 /= v1_i[2];
+          w2 /= v2_i[2];
+          pixel_sample[2] = 1.0f / (w0 + w1 + w2);
+          w0 = w0 * pixel_sample[2];
+          w1 = w1 * pixel_sample[2];
+          w2 = w2 * pixel_sample[2];
+          size_t pi = static_cast<size_t>(y) * W + x;
+          float& d = depth_[pi];
+          if (d < flt_min || pixel_sample[2] < d) {
+            d = pixel_sample[2];
+            face_id_[pi] = i;
+            weight_image[pi * 3 + 0] = w0;
+            weight_image[pi * 3 + 1] = w1;
+            weight_image[pi * 3 + 2] = w2;
+            backface_image[pi] = backface ? 255 : 0;
+          }
+        }
+      }
+    }
+  }
+
+  // resolve loop (rasterizer.cc:243-301)
+  for (int y = 0; y < H; y++) {
+    for (int x = 0; x < W; x++) {
+      size_t pi = static_cast<size_t>(y) * W + x;
+      const uint8_t bf = backface_image[pi];
+      int32_t& fid = face_id_[pi];
+      if (o.backface_culling && bf == 255) {
+        depth_[pi] = 0.0f;
+        fid = -1;
+        continue;
+      }
+      if (fid > 0) {
+        const V3& ray_w = rt.ray_w[pi];
+        float w0 = weight_image[pi * 3 + 0];
+        float w1 = weight_image[pi * 3 + 1];
+        float w2 = weight_image[pi * 3 + 2];
+        if (mask) mask[pi] = 255;
+        V3 sn{{0.0f, 0.0f, 0.0f}};
+        if (o.shading_normal == 0) {
+          sn = load3(m.face_normals, fid);
+        } else {
+          const int32_t* ni = m.normal_indices + static_cast<int64_t>(fid) * 3;
+          V3 na = load3(m.normals, ni[0]);
+          V3 nb = load3(m.normals, ni[1]);
+          V3 nc = load3(m.normals, ni[2]);
+          for (int k = 0; k < 3; k++) sn[k] = w0 * na[k] + w1 * nb[k] + w2 * nc[k];
+        }
+        if (normal) {
+          V3 snc = matvec(cam->w2c_R, sn);
+          for (int k = 0; k < 3; k++) normal[pi * 3 + k] = snc[k];
+        }
+        if (color) {
+          V3 light_dir = ray_w;
+          ShadePixel(sc, color + pi * 3, w1, w2, fid, ray_w, light_dir, sn);
+        }
+      }
+    }
+  }
+
+  auto t1 = std::chrono::steady_clock::now();
+  if (elapsed_ms)
+    *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  return kOk;
+}
+
+// RenderW / RenderDepthW (rasterizer.cc:329-348) depth conversion:
+// ConvertTo(f_depth, depth) with scale 1 -> static_cast<uint16_t>(1.0 * (double)d)
+void oracle_depth_to_u16(const float* depth, int64_t n, uint16_t* out) {
+  for (int64_t i = 0; i < n; i++) {
+    double val = static_cast<double>(depth[i]);
+    double s = static_cast<double>(1.0f) * val;
+    out[i] = static_cast<uint16_t>(static_cast<uint32_t>(d2i_x86(s)) & 0xffffu);
+  }
+}
+
+// Exposed pieces for unit pins.
+float oracle_edge_function(const float* a, const float* b, const float* c) {
+  V3 A{{a[0], a[1], 0}}, B{{b[0], b[1], 0}}, C{{c[0], c[1], 0}};
+  return EdgeFunction(A, B, C);
+}
+
+void oracle_ray_w(const OracleCamera* cam, float x, float y, float* out) {
+  V3 r = ray_w_at(*cam, x, y);
+  out[0] = r[0];
+  out[1] = r[1];
+  out[2] = r[2];
+}
+
+int oracle_num_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

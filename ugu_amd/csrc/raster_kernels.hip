@@ -1,0 +1,1021 @@
+// HIP kernels of the gfx950 rasterizer.
+//
+// Pipeline for a batch of views (a single view is a batch of one):
+//   k_clear       zero per-tile bin counters and the status words
+//   k_project     vertex loop of Rasterizer::Impl::Render
+//                 (src/renderer/cpu/rasterizer.cc:125-136, camera.h:380-387)
+//   k_setup_bin   per-face setup of the face loop (rasterizer.cc:162-193):
+//                 z<0 reject, bbox, off-screen cull, pixel bbox, area; then
+//                 appends the face id to every screen tile its pixel bbox
+//                 touches (wavefront-aggregated atomics)
+//   k_raster      one workgroup per screen tile: stages the tile's faces in
+//                 LDS, expands (face, pixel) pairs with a block prefix sum so
+//                 every lane tests one pair, resolves depth with a packed
+//                 64-bit (depth bits << 32 | face id) LDS atomicMin, then
+//                 shades and writes every output pixel of the tile once
+//                 (resolve loop rasterizer.cc:243-301, pixel_shader.h).
+//   k_backward    differentiable backward pass (no reference counterpart).
+//
+// Bit-exactness: all coverage arithmetic follows the reference's expression
+// order; this file is compiled with -ffp-contract=off and correctly rounded
+// fp32 division/sqrt (see the Makefile), and the pragma below forbids
+// contraction locally as well.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+#include <limits.h>
+
+#include "raster_types.h"
+
+#pragma clang fp contract(off)
+
+namespace ugu_hip {
+
+#define UGU_DEV __device__ __forceinline__
+
+// ---------------------------------------------------------------------------
+// scalar helpers
+
+// Eigen's unrolled reduction of a fixed 3-vector: x0 + (x1 + x2)
+UGU_DEV float sum3(float a, float b, float c) { return a + (b + c); }
+
+// EdgeFunction(a, b, c) = (c0-a0)(b1-a1) - (c1-a1)(b0-a0)  (rasterizer.cc:25-28)
+UGU_DEV float edge_fn(float ax, float ay, float bx, float by, float cx, float cy) {
+  return (cx - ax) * (by - ay) - (cy - ay) * (bx - ax);
+}
+
+// float -> int32 as x86-64 cvttss2si (what the reference's static_cast
+// compiles to): truncation, INT32_MIN for NaN and out-of-range values.
+UGU_DEV int32_t f2i_x86(float f) {
+  return (f >= -2147483648.0f && f < 2147483648.0f) ? static_cast<int32_t>(f)
+                                                    : INT32_MIN;
+}
+// saturating float -> int32 for the pixel bbox upper bound
+UGU_DEV int32_t f2i_sat(float f) {
+  if (f >= 2147483648.0f) return INT32_MAX;
+  if (f >= -2147483648.0f) return static_cast<int32_t>(f);
+  return INT32_MIN;
+}
+UGU_DEV uint8_t f2u8_x86(float f) {
+  return static_cast<uint8_t>(static_cast<uint32_t>(f2i_x86(f)) & 0xffu);
+}
+
+// std::min({a,b,c}) / std::max({a,b,c}) including their NaN behaviour
+UGU_DEV float min3_std(float a, float b, float c) {
+  float m = a;
+  if (b < m) m = b;
+  if (c < m) m = c;
+  return m;
+}
+UGU_DEV float max3_std(float a, float b, float c) {
+  float m = a;
+  if (m < b) m = b;
+  if (m < c) m = c;
+  return m;
+}
+
+// PinholeCamera::ray_w(float x, float y) (camera.h:438-448):
+// d = ((x-cx)/fx, (y-cy)/fy, 1); d.normalize(); ray = c2w_R_f * d
+UGU_DEV void ray_w_at(const DevView& v, float x, float y, float& r0, float& r1,
+                      float& r2) {
+  float d0 = (x - v.cx) / v.fx;
+  float d1 = (y - v.cy) / v.fy;
+  float d2 = 1.0f;
+  float z = sum3(d0 * d0, d1 * d1, d2 * d2);
+  if (z > 0.0f) {
+    float s = sqrtf(z);
+    d0 = d0 / s;
+    d1 = d1 / s;
+    d2 = d2 / s;
+  }
+  r0 = sum3(v.c2w_R[0] * d0, v.c2w_R[1] * d1, v.c2w_R[2] * d2);
+  r1 = sum3(v.c2w_R[3] * d0, v.c2w_R[4] * d1, v.c2w_R[5] * d2);
+  r2 = sum3(v.c2w_R[6] * d0, v.c2w_R[7] * d1, v.c2w_R[8] * d2);
+}
+
+// w2c_R * p + w2c_t, then PinholeCamera::Project (camera.h:380-387):
+// x = fx / z * x + cx (divide first), y likewise, z kept.
+UGU_DEV float4 project_point(const DevView& v, float px, float py, float pz) {
+  float qx = sum3(v.w2c_R[0] * px, v.w2c_R[1] * py, v.w2c_R[2] * pz) + v.w2c_t[0];
+  float qy = sum3(v.w2c_R[3] * px, v.w2c_R[4] * py, v.w2c_R[5] * pz) + v.w2c_t[1];
+  float qz = sum3(v.w2c_R[6] * px, v.w2c_R[7] * py, v.w2c_R[8] * pz) + v.w2c_t[2];
+  float4 r;
+  r.x = v.fx / qz * qx + v.cx;
+  r.y = v.fy / qz * qy + v.cy;
+  r.z = qz;
+  r.w = 0.0f;
+  return r;
+}
+
+// Per-face setup of rasterizer.cc:164-193. Returns false when the
+// reference skips the face or its pixel bbox is empty.
+UGU_DEV bool face_setup(const float4& a, const float4& b, const float4& c,
+                        int W, int H, int& x0, int& x1, int& y0, int& y1,
+                        float& area) {
+  if (a.z < 0.0f || b.z < 0.0f || c.z < 0.0f) return false;
+  float xmin = min3_std(a.x, b.x, c.x);
+  float ymin = min3_std(a.y, b.y, c.y);
+  float xmax = max3_std(a.x, b.x, c.x);
+  float ymax = max3_std(a.y, b.y, c.y);
+  if (xmin > static_cast<float>(W - 1) || xmax < 0.0f ||
+      ymin > static_cast<float>(H - 1) || ymax < 0.0f)
+    return false;
+  x0 = max(0, f2i_x86(ceilf(xmin)));
+  x1 = min(W - 1, f2i_sat(floorf(xmax)));
+  y0 = max(0, f2i_x86(ceilf(ymin)));
+  y1 = min(H - 1, f2i_sat(floorf(ymax)));
+  area = edge_fn(a.x, a.y, b.x, b.y, c.x, c.y);
+  if (fabsf(area) < FLT_MIN) return false;
+  return x0 <= x1 && y0 <= y1;
+}
+
+// Coverage, perspective-correct weights and depth of one (face, pixel) pair
+// (rasterizer.cc:196-226). Returns true when the pixel is covered.
+UGU_DEV bool cover_test(float ax, float ay, float az, float bx, float by,
+                        float bz, float cx, float cy, float cz, float area,
+                        bool bf, float x, float y, float& z, float& w0o,
+                        float& w1o, float& w2o) {
+  float w0 = edge_fn(bx, by, cx, cy, x, y);
+  float w1 = edge_fn(cx, cy, ax, ay, x, y);
+  float w2 = edge_fn(ax, ay, bx, by, x, y);
+  bool cov = (!bf && (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f)) ||
+             (bf && (w0 <= 0.0f && w1 <= 0.0f && w2 <= 0.0f));
+  if (!cov) return false;
+  w0 /= area;
+  w1 /= area;
+  w2 /= area;
+  w0 /= az;
+  w1 /= bz;
+  w2 /= cz;
+  z = 1.0f / (w0 + w1 + w2);
+  w0o = w0 * z;
+  w1o = w1 * z;
+  w2o = w2 * z;
+  return true;
+}
+
+// backface = face_normal . ray_w > 0 (rasterizer.cc:200)
+UGU_DEV bool backface_of(const float* fn, int64_t fid, float r0, float r1, float r2) {
+  if (fn == nullptr) return false;
+  const float* n = fn + fid * 3;
+  return sum3(n[0] * r0, n[1] * r1, n[2] * r2) > 0.0f;
+}
+
+// ImageBase::at on an RGB8 texture (image.h:416-427) with the linear index
+// clamped to the image.
+UGU_DEV const uint8_t* tex_at(const uint8_t* base, int w, int h, int row, int col) {
+  int64_t lin = static_cast<int64_t>(row) * w + col;
+  int64_t n = static_cast<int64_t>(w) * h;
+  lin = lin < 0 ? 0 : (lin >= n ? n - 1 : lin);
+  return base + lin * 3;
+}
+
+UGU_DEV int xcd_remap(int b, int nb) {
+  // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous run of tiles so neighbouring tiles share its L2.
+  const int q = nb >> 3, r = nb & 7;
+  const int xcd = b & 7, idx = b >> 3;
+  const int start = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+  return start + idx;
+}
+
+// ---------------------------------------------------------------------------
+// k_clear
+
+__global__ void k_clear(int32_t* counts, int32_t n, int32_t* status) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) counts[i] = 0;
+  if (i < 4) status[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// k_project: one thread per (vertex, view)
+
+__global__ void k_project(const DevView* __restrict__ views,
+                          const float* __restrict__ verts, int32_t V,
+                          float4* __restrict__ proj) {
+  const DevView& v = views[blockIdx.y];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V) return;
+  float px = verts[int64_t(i) * 3 + 0];
+  float py = verts[int64_t(i) * 3 + 1];
+  float pz = verts[int64_t(i) * 3 + 2];
+  proj[v.proj_offset + i] = project_point(v, px, py, pz);
+}
+
+// ---------------------------------------------------------------------------
+// k_face_normals: Mesh::CalcFaceNormal (src/mesh.cc:339-349) on device, for
+// the differentiable path where positions change between calls.
+
+UGU_DEV void normalize3(float& x, float& y, float& z) {
+  float n = sum3(x * x, y * y, z * z);
+  if (n > 0.0f) {
+    float s = sqrtf(n);
+    x = x / s;
+    y = y / s;
+    z = z / s;
+  }
+}
+
+__global__ void k_face_normals(const float* __restrict__ verts,
+                               const int32_t* __restrict__ faces, int32_t F,
+                               float* __restrict__ fn) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const int32_t* fi = faces + int64_t(f) * 3;
+  const float* p0 = verts + int64_t(fi[0]) * 3;
+  const float* p1 = verts + int64_t(fi[1]) * 3;
+  const float* p2 = verts + int64_t(fi[2]) * 3;
+  float ax = p1[0] - p0[0], ay = p1[1] - p0[1], az = p1[2] - p0[2];
+  float bx = p2[0] - p0[0], by = p2[1] - p0[1], bz = p2[2] - p0[2];
+  normalize3(ax, ay, az);
+  normalize3(bx, by, bz);
+  float cx = ay * bz - az * by;
+  float cy = az * bx - ax * bz;
+  float cz = ax * by - ay * bx;
+  normalize3(cx, cy, cz);
+  fn[int64_t(f) * 3 + 0] = cx;
+  fn[int64_t(f) * 3 + 1] = cy;
+  fn[int64_t(f) * 3 + 2] = cz;
+}
+
+// ---------------------------------------------------------------------------
+// k_setup_bin: one thread per (face, view)
+
+// Appends fid to bin `tile` for every lane with `has`; lanes that target the
+// same tile share one atomicAdd (ballot + popcount rank).
+UGU_DEV void wave_append(const DevBins& bins, int tile, int fid, bool has) {
+  uint64_t todo = __ballot(has);
+  const int lane = __lane_id();
+  while (todo) {
+    const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+    const int lt = __builtin_amdgcn_readlane(tile, leader);
+    const bool mine = has && tile == lt;
+    const uint64_t grp = __ballot(mine);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&bins.counts[lt], __popcll(grp));
+    base = __builtin_amdgcn_readlane(base, leader);
+    if (mine) {
+      const int slot = base + __popcll(grp & ((1ull << lane) - 1ull));
+      if (slot < bins.cap) bins.bins[int64_t(lt) * bins.cap + slot] = fid;
+    }
+    todo &= ~grp;
+  }
+}
+
+template <int TW, int TH>
+__global__ __launch_bounds__(256) void k_setup_bin(
+    const DevView* __restrict__ views, DevMesh mesh,
+    const float4* __restrict__ proj, DevBins bins) {
+  const DevView& v = views[blockIdx.y];
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  bool has = false;
+  int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+  if (f < mesh.F) {
+    const int32_t* fi = mesh.faces + int64_t(f) * 3;
+    const float4* pv = proj + v.proj_offset;
+    float4 a = pv[fi[0]], b = pv[fi[1]], c = pv[fi[2]];
+    int x0, x1, y0, y1;
+    float area;
+    if (face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) {
+      has = true;
+      tx0 = x0 / TW;
+      tx1 = x1 / TW;
+      ty0 = y0 / TH;
+      ty1 = y1 / TH;
+    }
+  }
+  int tx = tx0, ty = ty0;
+  while (__any(has)) {
+    const int tile = has ? v.tile_base + ty * v.tiles_x + tx : -1;
+    wave_append(bins, tile, f, has);
+    if (has) {
+      if (++tx > tx1) {
+        tx = tx0;
+        if (++ty > ty1) has = false;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_raster
+
+template <int NT>
+struct RasterLds {
+  unsigned long long key[NT];  // packed (depth bits << 32 | face id)
+  float ray[3][NT];            // ray_w of each tile pixel
+  float fv[13][NT];            // staged faces: 3 projected verts, area, face normal
+  int32_t fid[NT];
+  int32_t box[NT];             // lx0 | ly0 << 8 | nx << 16 (tile-local pixel bbox)
+  int32_t pref[NT + 1];        // exclusive prefix of pair counts
+  int32_t wsum[NT / 64 + 1];
+  int32_t flag[NT];            // pixel saw a depth outside [FLT_MIN, inf]
+  int32_t any_flag;
+  int32_t item;
+};
+
+// Exact sequential z-buffer for one pixel (slow path): visits the covering
+// faces of the tile's bin in increasing face id and applies the reference's
+// update rule `if (d < FLT_MIN || z < d)` (rasterizer.cc:228-237). Used only
+// for pixels where some covering face produced a depth below FLT_MIN or NaN,
+// where the packed atomicMin order is not the sequential order.
+UGU_DEV void sequential_pixel(const DevView& v, const DevMesh& mesh,
+                              const float4* pv, const int32_t* bin, int n,
+                              int gx, int gy, float r0, float r1, float r2,
+                              int& out_fid) {
+  int cur = -1;
+  float d = 0.0f;
+  int dfid = -1;
+  const float x = static_cast<float>(gx), y = static_cast<float>(gy);
+  for (;;) {
+    int best = INT_MAX;
+    float bestz = 0.0f;
+    for (int i = 0; i < n; i++) {
+      const int f = bin[i];
+      if (f <= cur || f >= best) continue;
+      const int32_t* fi = mesh.faces + int64_t(f) * 3;
+      float4 a = pv[fi[0]], b = pv[fi[1]], c = pv[fi[2]];
+      int x0, x1, y0, y1;
+      float area;
+      if (!face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) continue;
+      if (gx < x0 || gx > x1 || gy < y0 || gy > y1) continue;
+      const bool bf = backface_of(mesh.fnormals, f, r0, r1, r2);
+      float z, w0, w1, w2;
+      if (cover_test(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, area, bf, x,
+                     y, z, w0, w1, w2)) {
+        best = f;
+        bestz = z;
+      }
+    }
+    if (best == INT_MAX) break;
+    if (d < FLT_MIN || bestz < d) {
+      d = bestz;
+      dfid = best;
+    }
+    cur = best;
+  }
+  out_fid = dfid;
+}
+
+// Colour of one shaded pixel of the reference path: colourizer then shader
+// (pixel_shader.h:215-218 with the classes at :222-422).
+UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, float u,
+                      float vv, float r0, float r1, float r2, float n0,
+                      float n1, float n2, uint8_t c[3]) {
+  if (o.diffuse_color == 2) {
+    // DiffuseVertexColorColorizer (pixel_shader.h:234-256)
+    const int32_t* f = m.faces + int64_t(fid) * 3;
+    const float* c0 = m.vcolors + int64_t(f[0]) * 3;
+    const float* c1 = m.vcolors + int64_t(f[1]) * 3;
+    const float* c2 = m.vcolors + int64_t(f[2]) * 3;
+    const float w = 1.0f - u - vv;
+    for (int k = 0; k < 3; k++) c[k] = f2u8_x86(w * c0[k] + u * c1[k] + vv * c2[k]);
+  } else if (o.diffuse_color == 1) {
+    const TexInfo t = m.tex[m.matid[fid]];
+    const uint8_t* base = m.texdata + t.offset;
+    const int32_t* ui = m.uvidx + int64_t(fid) * 3;
+    const float* a = m.uv + int64_t(ui[0]) * 2;
+    const float* b = m.uv + int64_t(ui[1]) * 2;
+    const float* cc = m.uv + int64_t(ui[2]) * 2;
+    const float w = 1.0f - u - vv;
+    const float iu = w * a[0] + u * b[0] + vv * cc[0];
+    const float iv = w * a[1] + u * b[1] + vv * cc[1];
+    const float fx = iu * static_cast<float>(t.w - 1);
+    const float fy = (1.0f - iv) * static_cast<float>(t.h - 1);
+    if (o.interp == 0) {
+      // DiffuseTextureNnColorizer (pixel_shader.h:260-298)
+      const uint8_t* dt = tex_at(base, t.w, t.h, f2i_x86(roundf(fy)), f2i_x86(roundf(fx)));
+      c[0] = dt[0];
+      c[1] = dt[1];
+      c[2] = dt[2];
+    } else {
+      // DiffuseTextureBilinearColorizer (pixel_shader.h:302-359)
+      const int x0 = f2i_x86(floorf(fx));
+      const int y0 = f2i_x86(floorf(fy));
+      const int x1 = x0 + 1, y1 = y0 + 1;
+      const float lu = fx - static_cast<float>(x0);
+      const float lv = fy - static_cast<float>(y0);
+      const uint8_t* p00 = tex_at(base, t.w, t.h, y0, x0);
+      const uint8_t* p01 = tex_at(base, t.w, t.h, y0, x1);
+      const uint8_t* p10 = tex_at(base, t.w, t.h, y1, x0);
+      const uint8_t* p11 = tex_at(base, t.w, t.h, y1, x1);
+      for (int k = 0; k < 3; k++) {
+        float ic = (1.0f - lu) * (1.0f - lv) * static_cast<float>(p00[k]) +
+                   lu * (1.0f - lv) * static_cast<float>(p01[k]) +
+                   (1.0f - lu) * lv * static_cast<float>(p10[k]) +
+                   lu * lv * static_cast<float>(p11[k]);
+        c[k] = f2u8_x86(ic);
+      }
+    }
+  } else {
+    // DiffuseDefaultColorizer (pixel_shader.h:222-230)
+    c[0] = c[1] = c[2] = 255;
+  }
+  if (o.diffuse_shading == 1) {
+    // DiffuseLambertianShader (pixel_shader.h:370-388); light_dir = ray_w
+    float coeff = -sum3(r0 * n0, r1 * n1, r2 * n2);
+    if (coeff < 0.0f) coeff = 0.0f;
+    for (int k = 0; k < 3; k++) c[k] = f2u8_x86(coeff * static_cast<float>(c[k]));
+  } else if (o.diffuse_shading == 2) {
+    // DiffuseOrenNayarShader (pixel_shader.h:392-422); light_dir = ray_w
+    const float dot_light = -sum3(r0 * n0, r1 * n1, r2 * n2);
+    const float theta_i = acosf(dot_light);
+    const float dot_ray = -sum3(r0 * n0, r1 * n1, r2 * n2);
+    const float theta_r = acosf(dot_ray);
+    const float bl0 = -n0 * dot_light - r0, bl1 = -n1 * dot_light - r1,
+                bl2 = -n2 * dot_light - r2;
+    const float phi_diff_cos = fmaxf(0.0f, sum3(bl0 * bl0, bl1 * bl1, bl2 * bl2));
+    const float alpha = fmaxf(theta_i, theta_r);
+    const float beta = fminf(theta_i, theta_r);
+    const float coeff = fmaxf(0.0f, dot_light) *
+                        (o.on_A + (o.on_B * phi_diff_cos * sinf(alpha) * tanf(beta)));
+    for (int k = 0; k < 3; k++) c[k] = f2u8_x86(coeff * static_cast<float>(c[k]));
+  }
+}
+
+// Bilinear sample of an f32 RGB texture with clamp-to-edge addressing
+// (differentiable path). Returns the top-left texel and the local weights.
+UGU_DEV void ftex_coords(int tw, int th, float iu, float iv, int& x0, int& y0,
+                         int& x1, int& y1, float& lu, float& lv) {
+  const float fx = iu * static_cast<float>(tw - 1);
+  const float fy = (1.0f - iv) * static_cast<float>(th - 1);
+  const float flx = floorf(fx), fly = floorf(fy);
+  lu = fx - flx;
+  lv = fy - fly;
+  int ix = f2i_sat(flx), iy = f2i_sat(fly);
+  x0 = min(max(ix, 0), tw - 1);
+  y0 = min(max(iy, 0), th - 1);
+  x1 = min(max(ix + 1, 0), tw - 1);
+  y1 = min(max(iy + 1, 0), th - 1);
+}
+
+template <int TW, int TH, int MODE>
+UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
+                         const DevMesh& mesh, const float4* proj,
+                         const DevBins& bins, const DevOptions& opt) {
+  constexpr int NT = TW * TH;
+  const int t = threadIdx.x;
+
+  // view of this tile: largest v with tile_base[v] <= gtile
+  int lo = 0, hi = bins.n_views - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bins.tile_base[mid] <= gtile) lo = mid; else hi = mid - 1;
+  }
+  const DevView& v = views[lo];
+  const int lt = gtile - v.tile_base;
+  const int tyi = lt / v.tiles_x;
+  const int txi = lt - tyi * v.tiles_x;
+  const int px0 = txi * TW, py0 = tyi * TH;
+  const float4* pv = proj + v.proj_offset;
+
+  // this thread's pixel
+  const int lx = t % TW, ly = t / TW;
+  const int gx = px0 + lx, gy = py0 + ly;
+  float r0, r1, r2;
+  ray_w_at(v, static_cast<float>(gx), static_cast<float>(gy), r0, r1, r2);
+  s.key[t] = ~0ull;
+  s.flag[t] = 0;
+  s.ray[0][t] = r0;
+  s.ray[1][t] = r1;
+  s.ray[2][t] = r2;
+  if (t == 0) s.any_flag = 0;
+
+  const int count = bins.counts[gtile];
+  const int n = count < bins.cap ? count : bins.cap;
+  if (t == 0 && count > 0) atomicMax(&bins.status[0], count);
+  const int32_t* bin = bins.bins + int64_t(gtile) * bins.cap;
+  __syncthreads();
+
+  for (int base = 0; base < n; base += NT) {
+    // stage one chunk of faces
+    int c = 0;
+    const int i = base + t;
+    if (i < n) {
+      const int f = bin[i];
+      const int32_t* fi = mesh.faces + int64_t(f) * 3;
+      const float4 a = pv[fi[0]], b = pv[fi[1]], cc = pv[fi[2]];
+      int x0, x1, y0, y1;
+      float area;
+      if (face_setup(a, b, cc, v.W, v.H, x0, x1, y0, y1, area)) {
+        const int lx0 = max(x0, px0) - px0, lx1 = min(x1, px0 + TW - 1) - px0;
+        const int ly0 = max(y0, py0) - py0, ly1 = min(y1, py0 + TH - 1) - py0;
+        if (lx0 <= lx1 && ly0 <= ly1) {
+          const int nx = lx1 - lx0 + 1;
+          c = nx * (ly1 - ly0 + 1);
+          s.box[t] = lx0 | (ly0 << 8) | (nx << 16);
+        }
+      }
+      s.fv[0][t] = a.x; s.fv[1][t] = a.y; s.fv[2][t] = a.z;
+      s.fv[3][t] = b.x; s.fv[4][t] = b.y; s.fv[5][t] = b.z;
+      s.fv[6][t] = cc.x; s.fv[7][t] = cc.y; s.fv[8][t] = cc.z;
+      s.fv[9][t] = area;
+      if (mesh.fnormals) {
+        const float* fn = mesh.fnormals + int64_t(f) * 3;
+        s.fv[10][t] = fn[0]; s.fv[11][t] = fn[1]; s.fv[12][t] = fn[2];
+      }
+      s.fid[t] = f;
+    }
+    // block exclusive scan of the pair counts
+    {
+      const int lane = t & 63, wid = t >> 6;
+      int x = c;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+      }
+      if (lane == 63) s.wsum[wid] = x;
+      __syncthreads();
+      int wb = 0;
+      for (int w = 0; w < wid; w++) wb += s.wsum[w];
+      s.pref[t] = wb + x - c;
+      if (t == NT - 1) s.pref[NT] = wb + x;
+      __syncthreads();
+    }
+    const int total = s.pref[NT];
+    const bool has_fn = mesh.fnormals != nullptr;
+    for (int p = t; p < total; p += NT) {
+      int jl = 0, jh = NT - 1;
+      while (jl < jh) {
+        const int mid = (jl + jh + 1) >> 1;
+        if (s.pref[mid] <= p) jl = mid; else jh = mid - 1;
+      }
+      const int j = jl;
+      const int r = p - s.pref[j];
+      const int bx = s.box[j];
+      const int nx = (bx >> 16) & 0xff;
+      const int qy = r / nx;
+      const int pxl = (bx & 0xff) + (r - qy * nx);
+      const int pyl = ((bx >> 8) & 0xff) + qy;
+      const int pix = pyl * TW + pxl;
+      const float rx = s.ray[0][pix], ry = s.ray[1][pix], rz = s.ray[2][pix];
+      const bool bf = has_fn && sum3(s.fv[10][j] * rx, s.fv[11][j] * ry, s.fv[12][j] * rz) > 0.0f;
+      float z, w0, w1, w2;
+      if (cover_test(s.fv[0][j], s.fv[1][j], s.fv[2][j], s.fv[3][j], s.fv[4][j],
+                     s.fv[5][j], s.fv[6][j], s.fv[7][j], s.fv[8][j], s.fv[9][j],
+                     bf, static_cast<float>(px0 + pxl), static_cast<float>(py0 + pyl),
+                     z, w0, w1, w2)) {
+        if (z >= FLT_MIN) {
+          const unsigned long long key =
+              (static_cast<unsigned long long>(__float_as_uint(z)) << 32) |
+              static_cast<unsigned int>(s.fid[j]);
+          atomicMin(&s.key[pix], key);
+        } else {
+          s.flag[pix] = 1;
+          s.any_flag = 1;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- resolve (rasterizer.cc:243-301) for this thread's pixel
+  int fid = -1;
+  if (s.any_flag && s.flag[t]) {
+    sequential_pixel(v, mesh, pv, bin, n, gx, gy, r0, r1, r2, fid);
+  } else {
+    const unsigned long long key = s.key[t];
+    if (key != ~0ull) fid = static_cast<int>(key & 0xffffffffull);
+  }
+  __syncthreads();  // LDS is reused by the next tile of a persistent block
+  if (gx >= v.W || gy >= v.H) return;
+
+  float z = 0.0f, w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
+  bool bf = false;
+  if (fid >= 0) {
+    const int32_t* fi = mesh.faces + int64_t(fid) * 3;
+    const float4 a = pv[fi[0]], b = pv[fi[1]], c = pv[fi[2]];
+    const float area = edge_fn(a.x, a.y, b.x, b.y, c.x, c.y);
+    bf = backface_of(mesh.fnormals, fid, r0, r1, r2);
+    cover_test(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, area, bf,
+               static_cast<float>(gx), static_cast<float>(gy), z, w0, w1, w2);
+    if (opt.culling && bf) {
+      fid = -1;
+      z = 0.0f;
+    }
+  }
+  const int64_t pidx = int64_t(gy) * v.W + gx;
+
+  if (MODE == 0) {
+    uint8_t col[3] = {0, 0, 0};
+    float nrm[3] = {0.0f, 0.0f, 0.0f};
+    uint8_t msk = 0;
+    if (fid > 0) {
+      msk = 255;
+      float n0, n1, n2;
+      if (opt.shading_normal == 0) {
+        const float* fn = mesh.fnormals + int64_t(fid) * 3;
+        n0 = fn[0]; n1 = fn[1]; n2 = fn[2];
+      } else {
+        const int32_t* ni = mesh.nidx + int64_t(fid) * 3;
+        const float* na = mesh.normals + int64_t(ni[0]) * 3;
+        const float* nb = mesh.normals + int64_t(ni[1]) * 3;
+        const float* nc = mesh.normals + int64_t(ni[2]) * 3;
+        n0 = w0 * na[0] + w1 * nb[0] + w2 * nc[0];
+        n1 = w0 * na[1] + w1 * nb[1] + w2 * nc[1];
+        n2 = w0 * na[2] + w1 * nb[2] + w2 * nc[2];
+      }
+      if (v.normal) {
+        nrm[0] = sum3(v.w2c_R[0] * n0, v.w2c_R[1] * n1, v.w2c_R[2] * n2);
+        nrm[1] = sum3(v.w2c_R[3] * n0, v.w2c_R[4] * n1, v.w2c_R[5] * n2);
+        nrm[2] = sum3(v.w2c_R[6] * n0, v.w2c_R[7] * n1, v.w2c_R[8] * n2);
+      }
+      if (v.color) shade_u8(mesh, opt, fid, w1, w2, r0, r1, r2, n0, n1, n2, col);
+    }
+    if (v.color) {
+      uint8_t* cp = v.color + pidx * 3;
+      cp[0] = col[0];
+      cp[1] = col[1];
+      cp[2] = col[2];
+    }
+    if (v.depth) v.depth[pidx] = z;
+    if (v.normal) {
+      float* np = v.normal + pidx * 3;
+      np[0] = nrm[0];
+      np[1] = nrm[1];
+      np[2] = nrm[2];
+    }
+    if (v.mask) v.mask[pidx] = msk;
+    if (v.face_id) v.face_id[pidx] = fid;
+  } else {
+    // differentiable forward: float image = colour term * texture term
+    float img[3] = {0.0f, 0.0f, 0.0f};
+    if (fid > 0) {
+      const float u = w1, vv = w2, w = 1.0f - u - vv;
+      float cc[3] = {1.0f, 1.0f, 1.0f};
+      if (mesh.vcolors) {
+        const int32_t* f = mesh.faces + int64_t(fid) * 3;
+        const float* c0 = mesh.vcolors + int64_t(f[0]) * 3;
+        const float* c1 = mesh.vcolors + int64_t(f[1]) * 3;
+        const float* c2 = mesh.vcolors + int64_t(f[2]) * 3;
+        for (int k = 0; k < 3; k++) cc[k] = w * c0[k] + u * c1[k] + vv * c2[k];
+      }
+      float tt[3] = {1.0f, 1.0f, 1.0f};
+      if (mesh.ftex) {
+        const int32_t* ui = mesh.uvidx + int64_t(fid) * 3;
+        const float* a = mesh.uv + int64_t(ui[0]) * 2;
+        const float* b = mesh.uv + int64_t(ui[1]) * 2;
+        const float* c = mesh.uv + int64_t(ui[2]) * 2;
+        const float iu = w * a[0] + u * b[0] + vv * c[0];
+        const float iv = w * a[1] + u * b[1] + vv * c[1];
+        int x0, y0, x1, y1;
+        float lu, lv;
+        ftex_coords(mesh.ftex_w, mesh.ftex_h, iu, iv, x0, y0, x1, y1, lu, lv);
+        const int64_t tw = mesh.ftex_w;
+        const float* p00 = mesh.ftex + (int64_t(y0) * tw + x0) * 3;
+        const float* p01 = mesh.ftex + (int64_t(y0) * tw + x1) * 3;
+        const float* p10 = mesh.ftex + (int64_t(y1) * tw + x0) * 3;
+        const float* p11 = mesh.ftex + (int64_t(y1) * tw + x1) * 3;
+        for (int k = 0; k < 3; k++)
+          tt[k] = (1.0f - lu) * (1.0f - lv) * p00[k] + lu * (1.0f - lv) * p01[k] +
+                  (1.0f - lu) * lv * p10[k] + lu * lv * p11[k];
+      }
+      for (int k = 0; k < 3; k++) img[k] = cc[k] * tt[k];
+    }
+    float* ip = v.image + pidx * 3;
+    ip[0] = img[0];
+    ip[1] = img[1];
+    ip[2] = img[2];
+    if (v.face_id) v.face_id[pidx] = fid;
+  }
+}
+
+template <int TW, int TH, int MODE>
+__global__ __launch_bounds__(TW * TH) void k_raster(
+    const DevView* __restrict__ views, DevMesh mesh,
+    const float4* __restrict__ proj, DevBins bins, DevOptions opt,
+    int persistent) {
+  __shared__ RasterLds<TW * TH> s;
+  if (!persistent) {
+    raster_tile<TW, TH, MODE>(s, xcd_remap(blockIdx.x, gridDim.x), views, mesh,
+                              proj, bins, opt);
+    return;
+  }
+  // persistent: pull (view, tile) items until the queue is drained; every
+  // block exits once the head passes total_tiles.
+  for (;;) {
+    if (threadIdx.x == 0) s.item = atomicAdd(&bins.status[1], 1);
+    __syncthreads();
+    const int item = s.item;
+    __syncthreads();
+    if (item >= bins.total_tiles) break;
+    raster_tile<TW, TH, MODE>(s, item, views, mesh, proj, bins, opt);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_backward: dL/d(vertices, vertex colours, texture) at fixed visibility.
+// One thread per pixel (row-major, so a wavefront is 64 neighbouring pixels);
+// contributions of contiguous lanes that share a face (or a texel quad) are
+// summed across the wavefront before the atomics.
+
+// Sums `val` over contiguous runs of equal `run` ids; the run's first lane
+// ends up with the run total.
+UGU_DEV float run_sum(float val, int run) {
+  const int lane = __lane_id();
+  for (int off = 1; off < 64; off <<= 1) {
+    const float o = __shfl_down(val, off);
+    const int orun = __shfl_down(run, off);
+    if (lane + off < 64 && orun == run) val += o;
+  }
+  return val;
+}
+
+__global__ __launch_bounds__(256) void k_backward(
+    const DevView* __restrict__ views, DevMesh mesh,
+    const float4* __restrict__ proj, DevGrads g) {
+  const DevView& v = views[blockIdx.y];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int npx = v.W * v.H;
+  int fid = -1;
+  if (p < npx) fid = v.face_id[p];
+  const bool valid = fid > 0;
+
+  float gcol[9], gpos[9], gtex[12];
+  for (int k = 0; k < 9; k++) { gcol[k] = 0.0f; gpos[k] = 0.0f; }
+  for (int k = 0; k < 12; k++) gtex[k] = 0.0f;
+  int tx0 = 0, ty0 = 0, tx1 = 0, ty1 = 0;
+  int vid[3] = {0, 0, 0};
+
+  if (valid) {
+    const float x = static_cast<float>(p % v.W), y = static_cast<float>(p / v.W);
+    const float G[3] = {v.grad[int64_t(p) * 3 + 0], v.grad[int64_t(p) * 3 + 1],
+                        v.grad[int64_t(p) * 3 + 2]};
+    const int32_t* fi = mesh.faces + int64_t(fid) * 3;
+    vid[0] = fi[0]; vid[1] = fi[1]; vid[2] = fi[2];
+    const float4* pv = proj + v.proj_offset;
+    float X[3], Y[3], Z[3], Qx[3], Qy[3];
+    for (int i = 0; i < 3; i++) {
+      const float4 q = pv[vid[i]];
+      X[i] = q.x; Y[i] = q.y; Z[i] = q.z;
+      const float* P = mesh.verts + int64_t(vid[i]) * 3;
+      Qx[i] = sum3(v.w2c_R[0] * P[0], v.w2c_R[1] * P[1], v.w2c_R[2] * P[2]) + v.w2c_t[0];
+      Qy[i] = sum3(v.w2c_R[3] * P[0], v.w2c_R[4] * P[1], v.w2c_R[5] * P[2]) + v.w2c_t[1];
+    }
+    // forward quantities (same expressions as the rasterizer)
+    const float A = edge_fn(X[0], Y[0], X[1], Y[1], X[2], Y[2]);
+    const float e[3] = {edge_fn(X[1], Y[1], X[2], Y[2], x, y),
+                        edge_fn(X[2], Y[2], X[0], Y[0], x, y),
+                        edge_fn(X[0], Y[0], X[1], Y[1], x, y)};
+    float q[3];
+    for (int i = 0; i < 3; i++) q[i] = e[i] / A / Z[i];
+    const float ssum = q[0] + q[1] + q[2];
+    const float zp = 1.0f / ssum;
+    const float b1 = q[1] * zp, b2 = q[2] * zp;
+    const float b0 = 1.0f - b1 - b2;
+    const float beta[3] = {b0, b1, b2};
+
+    float cc[3] = {1.0f, 1.0f, 1.0f}, dcdu[3] = {0, 0, 0}, dcdv[3] = {0, 0, 0};
+    if (mesh.vcolors) {
+      const float* c0 = mesh.vcolors + int64_t(vid[0]) * 3;
+      const float* c1 = mesh.vcolors + int64_t(vid[1]) * 3;
+      const float* c2 = mesh.vcolors + int64_t(vid[2]) * 3;
+      for (int k = 0; k < 3; k++) {
+        cc[k] = b0 * c0[k] + b1 * c1[k] + b2 * c2[k];
+        dcdu[k] = c1[k] - c0[k];
+        dcdv[k] = c2[k] - c0[k];
+      }
+    }
+    float tt[3] = {1.0f, 1.0f, 1.0f}, dtdu[3] = {0, 0, 0}, dtdv[3] = {0, 0, 0};
+    float lu = 0.0f, lv = 0.0f;
+    if (mesh.ftex) {
+      const int32_t* ui = mesh.uvidx + int64_t(fid) * 3;
+      const float* a = mesh.uv + int64_t(ui[0]) * 2;
+      const float* b = mesh.uv + int64_t(ui[1]) * 2;
+      const float* c = mesh.uv + int64_t(ui[2]) * 2;
+      const float iu = b0 * a[0] + b1 * b[0] + b2 * c[0];
+      const float iv = b0 * a[1] + b1 * b[1] + b2 * c[1];
+      ftex_coords(mesh.ftex_w, mesh.ftex_h, iu, iv, tx0, ty0, tx1, ty1, lu, lv);
+      const int64_t tw = mesh.ftex_w;
+      const float* p00 = mesh.ftex + (int64_t(ty0) * tw + tx0) * 3;
+      const float* p01 = mesh.ftex + (int64_t(ty0) * tw + tx1) * 3;
+      const float* p10 = mesh.ftex + (int64_t(ty1) * tw + tx0) * 3;
+      const float* p11 = mesh.ftex + (int64_t(ty1) * tw + tx1) * 3;
+      // d(lu)/du etc. through uv = b0*uv0 + u*uv1 + v*uv2
+      const float sx = static_cast<float>(mesh.ftex_w - 1);
+      const float sy = -static_cast<float>(mesh.ftex_h - 1);
+      const float dlu_du = sx * (b[0] - a[0]), dlu_dv = sx * (c[0] - a[0]);
+      const float dlv_du = sy * (b[1] - a[1]), dlv_dv = sy * (c[1] - a[1]);
+      for (int k = 0; k < 3; k++) {
+        tt[k] = (1.0f - lu) * (1.0f - lv) * p00[k] + lu * (1.0f - lv) * p01[k] +
+                (1.0f - lu) * lv * p10[k] + lu * lv * p11[k];
+        const float dt_dlu = (1.0f - lv) * (p01[k] - p00[k]) + lv * (p11[k] - p10[k]);
+        const float dt_dlv = (1.0f - lu) * (p10[k] - p00[k]) + lu * (p11[k] - p01[k]);
+        dtdu[k] = dt_dlu * dlu_du + dt_dlv * dlv_du;
+        dtdv[k] = dt_dlu * dlu_dv + dt_dlv * dlv_dv;
+      }
+    }
+    // dL/dc = G*t, dL/dt = G*c
+    float gu = 0.0f, gv = 0.0f;
+    float gt[3];
+    for (int k = 0; k < 3; k++) {
+      const float gc = G[k] * tt[k];
+      gt[k] = G[k] * cc[k];
+      for (int i = 0; i < 3; i++) gcol[i * 3 + k] = gc * beta[i];
+      gu += gc * dcdu[k] + gt[k] * dtdu[k];
+      gv += gc * dcdv[k] + gt[k] * dtdv[k];
+    }
+    if (mesh.ftex) {
+      const float wq[4] = {(1.0f - lu) * (1.0f - lv), lu * (1.0f - lv),
+                           (1.0f - lu) * lv, lu * lv};
+      for (int j = 0; j < 4; j++)
+        for (int k = 0; k < 3; k++) gtex[j * 3 + k] = gt[k] * wq[j];
+    }
+    if (g.verts) {
+      // b_i = q_i / s  =>  dL/dq_j = (dL/db_j - sum_i dL/db_i b_i) / s
+      const float S = gu * b1 + gv * b2;
+      const float gq[3] = {(0.0f - S) * zp, (gu - S) * zp, (gv - S) * zp};
+      // q_j = e_j / (A Z_j)
+      float ge[3], gZ[3], gA = 0.0f;
+      for (int j = 0; j < 3; j++) {
+        ge[j] = gq[j] / (A * Z[j]);
+        gZ[j] = -gq[j] * q[j] / Z[j];
+        gA -= gq[j] * q[j] / A;
+      }
+      float gX[3] = {0, 0, 0}, gY[3] = {0, 0, 0};
+      // E(a,b,c): dE/da = (c.y-b.y, b.x-c.x), dE/db = (a.y-c.y, c.x-a.x),
+      //           dE/dc = (b.y-a.y, a.x-b.x)
+      // e0 = E(v1, v2, p), e1 = E(v2, v0, p), e2 = E(v0, v1, p)
+      const int ea[3] = {1, 2, 0}, eb[3] = {2, 0, 1};
+      for (int j = 0; j < 3; j++) {
+        const int ia = ea[j], ib = eb[j];
+        gX[ia] += ge[j] * (y - Y[ib]);
+        gY[ia] += ge[j] * (X[ib] - x);
+        gX[ib] += ge[j] * (Y[ia] - y);
+        gY[ib] += ge[j] * (x - X[ia]);
+      }
+      // A = E(v0, v1, v2)
+      gX[0] += gA * (Y[2] - Y[1]);
+      gY[0] += gA * (X[1] - X[2]);
+      gX[1] += gA * (Y[0] - Y[2]);
+      gY[1] += gA * (X[2] - X[0]);
+      gX[2] += gA * (Y[1] - Y[0]);
+      gY[2] += gA * (X[0] - X[1]);
+      for (int i = 0; i < 3; i++) {
+        const float iz = 1.0f / Z[i];
+        const float gQx = gX[i] * v.fx * iz;
+        const float gQy = gY[i] * v.fy * iz;
+        const float gQz = gZ[i] - gX[i] * v.fx * Qx[i] * iz * iz -
+                          gY[i] * v.fy * Qy[i] * iz * iz;
+        for (int cdim = 0; cdim < 3; cdim++)
+          gpos[i * 3 + cdim] = v.w2c_R[0 * 3 + cdim] * gQx +
+                               v.w2c_R[1 * 3 + cdim] * gQy +
+                               v.w2c_R[2 * 3 + cdim] * gQz;
+      }
+    }
+  }
+
+  // ---- wavefront pre-reduction over runs of equal face id, then atomics
+  const int lane = __lane_id();
+  const int prev_fid = __shfl_up(fid, 1);
+  const bool head = valid && (lane == 0 || prev_fid != fid);
+  const uint64_t heads = __ballot(head);
+  const uint64_t vmask = __ballot(valid);
+  const bool reduce = heads != vmask;  // some run longer than one lane
+  const int run = valid ? __popcll(heads & ((2ull << lane) - 1ull)) : -1 - lane;
+  if (g.colors && mesh.vcolors) {
+    for (int k = 0; k < 9; k++) {
+      const float val = reduce ? run_sum(gcol[k], run) : gcol[k];
+      if (head) unsafeAtomicAdd(&g.colors[int64_t(vid[k / 3]) * 3 + (k % 3)], val);
+    }
+  }
+  if (g.verts) {
+    for (int k = 0; k < 9; k++) {
+      const float val = reduce ? run_sum(gpos[k], run) : gpos[k];
+      if (head) unsafeAtomicAdd(&g.verts[int64_t(vid[k / 3]) * 3 + (k % 3)], val);
+    }
+  }
+  if (g.texture && mesh.ftex) {
+    const int tkey = valid ? ty0 * mesh.ftex_w + tx0 : -1;
+    const int prev_t = __shfl_up(tkey, 1);
+    const bool thead = valid && (lane == 0 || prev_t != tkey);
+    const uint64_t theads = __ballot(thead);
+    const bool treduce = theads != vmask;
+    const int trun = valid ? __popcll(theads & ((2ull << lane) - 1ull)) : -1 - lane;
+    const int64_t tw = mesh.ftex_w;
+    const int64_t tidx[4] = {int64_t(ty0) * tw + tx0, int64_t(ty0) * tw + tx1,
+                             int64_t(ty1) * tw + tx0, int64_t(ty1) * tw + tx1};
+    for (int k = 0; k < 12; k++) {
+      const float val = treduce ? run_sum(gtex[k], trun) : gtex[k];
+      if (thead) unsafeAtomicAdd(&g.texture[tidx[k / 3] * 3 + (k % 3)], val);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// RenderW depth conversion: uint16(1.0 * (double)d), truncating as x86 does.
+__global__ void k_depth_to_u16(const float* __restrict__ d, int64_t n,
+                               uint16_t* __restrict__ out) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 1.0 * static_cast<double>(d[i]);
+  int32_t iv = (s >= -2147483648.0 && s < 2147483648.0) ? static_cast<int32_t>(s) : INT32_MIN;
+  out[i] = static_cast<uint16_t>(static_cast<uint32_t>(iv) & 0xffffu);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers (called from ugu_hip.cpp)
+
+int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status) {
+  int m = n > 4 ? n : 4;
+  hipLaunchKernelGGL(k_clear, dim3((m + 255) / 256), dim3(256), 0, st, counts, n, status);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_project(hipStream_t st, const DevView* views, int n_views,
+                   const float* verts, int32_t V, float4* proj) {
+  if (V <= 0) return 0;
+  hipLaunchKernelGGL(k_project, dim3((V + 255) / 256, n_views), dim3(256), 0, st,
+                     views, verts, V, proj);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_face_normals(hipStream_t st, const float* verts, const int32_t* faces,
+                        int32_t F, float* fn) {
+  if (F <= 0) return 0;
+  hipLaunchKernelGGL(k_face_normals, dim3((F + 255) / 256), dim3(256), 0, st,
+                     verts, faces, F, fn);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_setup_bin(hipStream_t st, int tw, int th, const DevView* views,
+                     int n_views, const DevMesh& mesh, const float4* proj,
+                     const DevBins& bins) {
+  if (mesh.F <= 0) return 0;
+  dim3 grid((mesh.F + 255) / 256, n_views), block(256);
+  if (tw == 8 && th == 8)
+    hipLaunchKernelGGL((k_setup_bin<8, 8>), grid, block, 0, st, views, mesh, proj, bins);
+  else if (tw == 16 && th == 16)
+    hipLaunchKernelGGL((k_setup_bin<16, 16>), grid, block, 0, st, views, mesh, proj, bins);
+  else if (tw == 32 && th == 8)
+    hipLaunchKernelGGL((k_setup_bin<32, 8>), grid, block, 0, st, views, mesh, proj, bins);
+  else
+    return -2;
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int TW, int TH, int MODE>
+static int launch_raster_t(hipStream_t st, const DevView* views,
+                           const DevMesh& mesh, const float4* proj,
+                           const DevBins& bins, const DevOptions& opt,
+                           int persistent_blocks) {
+  if (bins.total_tiles <= 0) return 0;
+  const int grid = persistent_blocks > 0 ? persistent_blocks : bins.total_tiles;
+  hipLaunchKernelGGL((k_raster<TW, TH, MODE>), dim3(grid), dim3(TW * TH), 0, st,
+                     views, mesh, proj, bins, opt, persistent_blocks > 0 ? 1 : 0);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_raster(hipStream_t st, int tw, int th, int mode, const DevView* views,
+                  const DevMesh& mesh, const float4* proj, const DevBins& bins,
+                  const DevOptions& opt, int persistent_blocks) {
+#define UGU_RASTER_CASE(TWV, THV)                                                  \
+  if (tw == TWV && th == THV)                                                      \
+    return mode == 0 ? launch_raster_t<TWV, THV, 0>(st, views, mesh, proj, bins,   \
+                                                    opt, persistent_blocks)        \
+                     : launch_raster_t<TWV, THV, 1>(st, views, mesh, proj, bins,   \
+                                                    opt, persistent_blocks);
+  UGU_RASTER_CASE(8, 8)
+  UGU_RASTER_CASE(16, 16)
+  UGU_RASTER_CASE(32, 8)
+#undef UGU_RASTER_CASE
+  return -2;
+}
+
+int raster_blocks_per_cu(int tw, int th, int mode) {
+  int nb = 0;
+  hipError_t e = hipErrorInvalidValue;
+#define UGU_OCC_CASE(TWV, THV)                                                       \
+  if (tw == TWV && th == THV) {                                                      \
+    if (mode == 0)                                                                   \
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_raster<TWV, THV, 0>,   \
+                                                       TWV * THV, 0);                \
+    else                                                                             \
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_raster<TWV, THV, 1>,   \
+                                                       TWV * THV, 0);                \
+  }
+  UGU_OCC_CASE(8, 8)
+  UGU_OCC_CASE(16, 16)
+  UGU_OCC_CASE(32, 8)
+#undef UGU_OCC_CASE
+  return e == hipSuccess ? nb : 0;
+}
+
+int launch_backward(hipStream_t st, const DevView* views, int n_views,
+                    int max_pixels, const DevMesh& mesh, const float4* proj,
+                    const DevGrads& g) {
+  if (max_pixels <= 0) return 0;
+  hipLaunchKernelGGL(k_backward, dim3((max_pixels + 255) / 256, n_views), dim3(256), 0,
+                     st, views, mesh, proj, g);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_depth_to_u16(hipStream_t st, const float* d, int64_t n, uint16_t* out) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_depth_to_u16, dim3((n + 255) / 256), dim3(256), 0, st, d, n, out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace ugu_hip

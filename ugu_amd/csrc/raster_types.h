@@ -1,0 +1,78 @@
+// Device-side descriptors shared by the HIP kernels and the host launcher.
+// Plain structs, passed by pointer (views) or by value (mesh, bins, options).
+#pragma once
+
+#include <stdint.h>
+
+namespace ugu_hip {
+
+// Per-view constants. The camera part is the UguCamera of ugu_hip.h; the rest
+// places the view's tiles in the global tile array and its outputs.
+struct DevView {
+  int32_t W, H;
+  float fx, fy, cx, cy;
+  float w2c_R[9];
+  float w2c_t[3];
+  float c2w_R[9];
+  int32_t tiles_x, tiles_y;
+  int32_t tile_base;  // first global tile of this view
+  int32_t pad0;
+  int64_t proj_offset;  // first projected vertex (float4 units) of this view
+  uint8_t* color;
+  float* depth;
+  float* normal;
+  uint8_t* mask;
+  int32_t* face_id;
+  float* image;        // differentiable forward: H x W x 3 f32
+  const float* grad;   // differentiable backward: H x W x 3 f32
+};
+
+struct TexInfo {
+  int64_t offset;  // byte offset (u8 textures) into the texture pool
+  int32_t w, h;
+};
+
+struct DevMesh {
+  int32_t V, F;
+  const float* verts;       // V x 3
+  const int32_t* faces;     // F x 3
+  const float* fnormals;    // F x 3 or null
+  const float* normals;     // N x 3 or null
+  const int32_t* nidx;      // F x 3 or null
+  const float* vcolors;     // V x 3 or null
+  const float* uv;          // U x 2 or null
+  const int32_t* uvidx;     // F x 3 or null
+  const int32_t* matid;     // F or null
+  const TexInfo* tex;       // per material
+  const uint8_t* texdata;   // u8 RGB texture pool
+  const float* ftex;        // differentiable path: f32 RGB texture or null
+  int32_t ftex_w, ftex_h;
+};
+
+struct DevOptions {
+  int32_t diffuse_color;    // 0 none, 1 texture, 2 vertex
+  int32_t interp;           // 0 nn, 1 bilinear
+  int32_t shading_normal;   // 0 face, 1 vertex
+  int32_t diffuse_shading;  // 0 none, 1 lambertian, 2 oren-nayar
+  int32_t culling;
+  float on_A, on_B;         // Oren-Nayar constants for sigma 0.3
+};
+
+struct DevBins {
+  int32_t* counts;   // one counter per global tile
+  int32_t* bins;     // global tile t owns bins[t*cap .. t*cap+cap)
+  int32_t cap;
+  int32_t total_tiles;
+  int32_t* status;   // [0] max bin count seen, [1] work-queue head
+  const int32_t* tile_base;  // n_views + 1 prefix of tiles per view
+  int32_t n_views;
+};
+
+// Gradient targets of the differentiable backward pass.
+struct DevGrads {
+  float* verts;    // V x 3 or null
+  float* colors;   // V x 3 or null
+  float* texture;  // Th x Tw x 3 or null
+};
+
+}  // namespace ugu_hip

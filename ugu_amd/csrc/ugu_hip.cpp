@@ -1,0 +1,868 @@
+// Host side of the C ABI declared in include/ugu_hip.h.
+//
+// One context owns one HIP stream, the device copy of the prepared mesh and
+// the work buffers of the binning pipeline (projected vertices, per-tile bin
+// counters and bins, view descriptors). Every call is synchronous.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ugu_hip.h"
+#include "raster_types.h"
+
+namespace ugu_hip {
+int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status);
+int launch_project(hipStream_t st, const DevView* views, int n_views,
+                   const float* verts, int32_t V, float4* proj);
+int launch_face_normals(hipStream_t st, const float* verts, const int32_t* faces,
+                        int32_t F, float* fn);
+int launch_setup_bin(hipStream_t st, int tw, int th, const DevView* views,
+                     int n_views, const DevMesh& mesh, const float4* proj,
+                     const DevBins& bins);
+int launch_raster(hipStream_t st, int tw, int th, int mode, const DevView* views,
+                  const DevMesh& mesh, const float4* proj, const DevBins& bins,
+                  const DevOptions& opt, int persistent_blocks);
+int raster_blocks_per_cu(int tw, int th, int mode);
+int launch_backward(hipStream_t st, const DevView* views, int n_views,
+                    int max_pixels, const DevMesh& mesh, const float4* proj,
+                    const DevGrads& g);
+int launch_depth_to_u16(hipStream_t st, const float* d, int64_t n, uint16_t* out);
+}  // namespace ugu_hip
+
+using namespace ugu_hip;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  // returns -1 on allocation failure, 1 when (re)allocated, 0 when unchanged
+  int ensure(size_t n) {
+    if (n == 0) n = 16;
+    if (n <= bytes) return 0;
+    release();
+    if (hipMalloc(&p, n) != hipSuccess) {
+      p = nullptr;
+      bytes = 0;
+      return -1;
+    }
+    bytes = n;
+    return 1;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct GraphKey {
+  int n, total_tiles, V, F, tw, th, persistent_blocks, mode;
+  int64_t buf_gen, mesh_gen;
+  DevOptions opt;
+  const void* verts;
+  const void* vcolors;
+  const void* ftex;
+  int ftex_w, ftex_h;
+  bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
+struct GraphEntry {
+  GraphKey key;
+  hipGraphExec_t exec;
+};
+
+constexpr size_t kMaxGraphs = 16;
+
+}  // namespace
+
+struct ugu_hip_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // prepared mesh
+  bool prepared = false;
+  int64_t mesh_gen = 0;
+  int32_t V = 0, F = 0, N = 0, U = 0, M = 0;
+  bool has_fn = false, has_normals = false, has_nidx = false, has_vc = false,
+       has_uv = false, has_uvidx = false, has_matid = false;
+  std::vector<int> tex_empty;
+  DevBuf verts, faces, fnormals, normals, nidx, vcolors, uv, uvidx, matid,
+      texinfo, texdata;
+
+  // work buffers
+  DevBuf proj, counts, bins, status, views, tile_base, stage_out, diff_fn;
+  int64_t buf_gen = 0;
+  void* h_views = nullptr;
+  size_t h_views_bytes = 0;
+  int32_t* h_status = nullptr;
+  std::vector<int32_t> h_tile_base;
+
+  UguTiling tiling{16, 16, 1024, 1};
+  int cu_count = 0;
+  std::vector<GraphEntry> graphs;
+
+  bool profile = false;
+  UguTimings tm{};
+  hipEvent_t ev[6] = {};
+};
+
+namespace {
+
+int fail(ugu_hip_ctx* c, int code, const char* msg) {
+  c->err = msg;
+  return code;
+}
+
+int hip_fail(ugu_hip_ctx* c, hipError_t e, const char* where) {
+  c->err = std::string(where) + ": " + hipGetErrorString(e);
+  return UGU_HIP_ERR_HIP;
+}
+
+#define UGU_HIP_CHECK(ctx, expr)                                   \
+  do {                                                             \
+    hipError_t e_ = (expr);                                        \
+    if (e_ != hipSuccess) return hip_fail((ctx), e_, #expr);       \
+  } while (0)
+
+int ensure(ugu_hip_ctx* c, DevBuf& b, size_t n) {
+  int r = b.ensure(n);
+  if (r < 0) return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, "device allocation failed");
+  if (r > 0) c->buf_gen++;
+  return 0;
+}
+
+void drop_graphs(ugu_hip_ctx* c) {
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+}
+
+int upload(ugu_hip_ctx* c, DevBuf& b, const void* src, size_t n) {
+  if (src == nullptr || n == 0) {
+    b.release();
+    return 0;
+  }
+  if (b.ensure(n) < 0) return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, "device allocation failed");
+  UGU_HIP_CHECK(c, hipMemcpy(b.p, src, n, hipMemcpyHostToDevice));
+  return 0;
+}
+
+bool check_idx(const int32_t* idx, int64_t n, int32_t limit) {
+  for (int64_t i = 0; i < n; i++)
+    if (idx[i] < 0 || idx[i] >= limit) return false;
+  return true;
+}
+
+// Validation of ValidateAndInitBeforeRender (util_private.cc:12-66), same
+// order and messages, plus the array presence the device path needs.
+int validate(ugu_hip_ctx* c, const UguCamera* cam, const UguOptions* o,
+             bool any_output) {
+  if (cam == nullptr) return fail(c, UGU_HIP_ERR_NO_CAMERA, "camera has not been set\n");
+  if (!c->prepared) return fail(c, UGU_HIP_ERR_MESH_NOT_PREPARED, "mesh has not been initialized\n");
+  if (o == nullptr) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "options are null");
+  if (o->backface_culling && !c->has_fn)
+    return fail(c, UGU_HIP_ERR_CULLING_NO_FACE_NORMAL,
+                "specified back-face culling but face normal is empty.\n");
+  if (o->diffuse_color == 1 && c->M == 0)
+    return fail(c, UGU_HIP_ERR_TEXTURE_EMPTY,
+                "specified texture as diffuse color but texture is empty.\n");
+  if (o->diffuse_color == 1) {
+    for (int i = 0; i < c->M; i++) {
+      if (c->tex_empty[i]) {
+        char buf[128];
+        std::snprintf(buf, sizeof(buf),
+                      "specified texture as diffuse color but %d th texture is empty.\n", i);
+        return fail(c, UGU_HIP_ERR_TEXTURE_NTH_EMPTY, buf);
+      }
+    }
+  }
+  if (o->diffuse_color == 2 && !c->has_vc)
+    return fail(c, UGU_HIP_ERR_VERTEX_COLOR_EMPTY,
+                "specified vertex color as diffuse color but vertex color is empty.\n");
+  if (o->shading_normal == 0 && !c->has_fn)
+    return fail(c, UGU_HIP_ERR_FACE_NORMAL_EMPTY,
+                "specified face normal as shading normal but face normal is empty.\n");
+  if (o->shading_normal == 1 && !c->has_normals)
+    return fail(c, UGU_HIP_ERR_VERTEX_NORMAL_EMPTY,
+                "specified vertex normal as shading normal but vertex normal is empty.\n");
+  if (!any_output)
+    return fail(c, UGU_HIP_ERR_ALL_NULL, "all arguments are nullptr. nothing to do\n");
+  // arrays the reference would read out of bounds without
+  if (o->diffuse_color == 1 && (!c->has_uv || !c->has_uvidx || !c->has_matid))
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT,
+                "texture colouring needs uv, uv_indices and material_ids");
+  if (o->shading_normal == 1 && !c->has_nidx)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT,
+                "vertex shading normals need normal_indices");
+  if (o->diffuse_color < 0 || o->diffuse_color > 2 || o->interp < 0 || o->interp > 1 ||
+      o->shading_normal < 0 || o->shading_normal > 1 || o->diffuse_shading < 0 ||
+      o->diffuse_shading > 2)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "option value out of range");
+  return UGU_HIP_OK;
+}
+
+int check_camera(ugu_hip_ctx* c, const UguCamera& cam) {
+  if (cam.width <= 0 || cam.height <= 0 || cam.width > 32768 || cam.height > 32768)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "camera size must be in [1, 32768]");
+  return 0;
+}
+
+bool valid_tile(int tw, int th) {
+  return (tw == 8 && th == 8) || (tw == 16 && th == 16) || (tw == 32 && th == 8);
+}
+
+DevOptions dev_options(const UguOptions& o) {
+  DevOptions d;
+  d.diffuse_color = o.diffuse_color;
+  d.interp = o.interp;
+  d.shading_normal = o.shading_normal;
+  d.diffuse_shading = o.diffuse_shading;
+  d.culling = o.backface_culling ? 1 : 0;
+  // OrenNayarParam(0.3f) (pixel_shader.h:143-148): RendererCpuOption::CopyTo
+  // never copies the sigma, so the renderer always sees the default 0.3.
+  const float sigma = 0.3f;
+  const float sigma_2 = sigma * sigma;
+  d.on_A = 1.0f - (0.5f * sigma_2 / (sigma_2 + 0.33f));
+  d.on_B = 0.45f * sigma_2 / (sigma_2 + 0.09f);
+  return d;
+}
+
+DevMesh dev_mesh(const ugu_hip_ctx* c) {
+  DevMesh m;
+  m.V = c->V;
+  m.F = c->F;
+  m.verts = c->verts.as<float>();
+  m.faces = c->faces.as<int32_t>();
+  m.fnormals = c->has_fn ? c->fnormals.as<float>() : nullptr;
+  m.normals = c->has_normals ? c->normals.as<float>() : nullptr;
+  m.nidx = c->has_nidx ? c->nidx.as<int32_t>() : nullptr;
+  m.vcolors = c->has_vc ? c->vcolors.as<float>() : nullptr;
+  m.uv = c->has_uv ? c->uv.as<float>() : nullptr;
+  m.uvidx = c->has_uvidx ? c->uvidx.as<int32_t>() : nullptr;
+  m.matid = c->has_matid ? c->matid.as<int32_t>() : nullptr;
+  m.tex = c->M > 0 ? c->texinfo.as<TexInfo>() : nullptr;
+  m.texdata = c->M > 0 ? c->texdata.as<uint8_t>() : nullptr;
+  m.ftex = nullptr;
+  m.ftex_w = m.ftex_h = 0;
+  return m;
+}
+
+struct OutReq {
+  bool color, depth, normal, mask, fid;
+};
+
+// Runs the pipeline for n views. mode 0: reference outputs; mode 1:
+// differentiable forward (outputs[i] unused, images/face_ids used).
+int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm,
+                 const DevOptions& dopt, int mode, const UguOutputs* outs,
+                 int memory, float* const* images, int32_t* const* fids,
+                 bool use_graph) {
+  const int tw = c->tiling.tile_w, th = c->tiling.tile_h;
+  // layout
+  c->h_tile_base.resize(n + 1);
+  int64_t total = 0;
+  int64_t stage_bytes = 0;
+  std::vector<int64_t> stage_off(size_t(n) * 5, -1);
+  for (int i = 0; i < n; i++) {
+    int err = check_camera(c, cams[i]);
+    if (err) return err;
+    const int tx = (cams[i].width + tw - 1) / tw, ty = (cams[i].height + th - 1) / th;
+    c->h_tile_base[i] = static_cast<int32_t>(total);
+    total += int64_t(tx) * ty;
+    if (mode == 0 && memory == UGU_HIP_MEMORY_HOST) {
+      const int64_t px = int64_t(cams[i].width) * cams[i].height;
+      const int64_t sz[5] = {3 * px, 4 * px, 12 * px, px, 4 * px};
+      const void* req[5] = {outs[i].color, outs[i].depth, outs[i].normal, outs[i].mask,
+                            outs[i].face_id};
+      for (int k = 0; k < 5; k++) {
+        if (req[k]) {
+          stage_off[size_t(i) * 5 + k] = stage_bytes;
+          stage_bytes += (sz[k] + 255) / 256 * 256;
+        }
+      }
+    }
+  }
+  if (total > INT32_MAX / 2) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "too many tiles");
+  c->h_tile_base[n] = static_cast<int32_t>(total);
+  const int total_tiles = static_cast<int>(total);
+
+  int err;
+  if ((err = ensure(c, c->proj, size_t(n) * std::max<int32_t>(dm.V, 1) * sizeof(float4)))) return err;
+  if ((err = ensure(c, c->counts, size_t(total_tiles) * 4))) return err;
+  if ((err = ensure(c, c->status, 64))) return err;
+  if ((err = ensure(c, c->views, size_t(n) * sizeof(DevView)))) return err;
+  if ((err = ensure(c, c->tile_base, size_t(n + 1) * 4))) return err;
+  if (stage_bytes > 0 && (err = ensure(c, c->stage_out, size_t(stage_bytes)))) return err;
+  const size_t view_bytes = size_t(n) * sizeof(DevView) + size_t(n + 1) * 4;
+  if (c->h_views_bytes < view_bytes) {
+    if (c->h_views) (void)hipHostFree(c->h_views);
+    c->h_views = nullptr;
+    c->h_views_bytes = 0;
+    UGU_HIP_CHECK(c, hipHostMalloc(&c->h_views, view_bytes, hipHostMallocDefault));
+    c->h_views_bytes = view_bytes;
+  }
+
+  // view descriptors
+  DevView* hv = static_cast<DevView*>(c->h_views);
+  for (int i = 0; i < n; i++) {
+    DevView& v = hv[i];
+    std::memset(&v, 0, sizeof(v));
+    v.W = cams[i].width;
+    v.H = cams[i].height;
+    v.fx = cams[i].fx;
+    v.fy = cams[i].fy;
+    v.cx = cams[i].cx;
+    v.cy = cams[i].cy;
+    std::memcpy(v.w2c_R, cams[i].w2c_R, sizeof(v.w2c_R));
+    std::memcpy(v.w2c_t, cams[i].w2c_t, sizeof(v.w2c_t));
+    std::memcpy(v.c2w_R, cams[i].c2w_R, sizeof(v.c2w_R));
+    v.tiles_x = (v.W + tw - 1) / tw;
+    v.tiles_y = (v.H + th - 1) / th;
+    v.tile_base = c->h_tile_base[i];
+    v.proj_offset = int64_t(i) * dm.V;
+    if (mode == 0) {
+      if (memory == UGU_HIP_MEMORY_DEVICE) {
+        v.color = outs[i].color;
+        v.depth = outs[i].depth;
+        v.normal = outs[i].normal;
+        v.mask = outs[i].mask;
+        v.face_id = outs[i].face_id;
+      } else {
+        uint8_t* sb = c->stage_out.as<uint8_t>();
+        const int64_t* so = &stage_off[size_t(i) * 5];
+        v.color = so[0] >= 0 ? sb + so[0] : nullptr;
+        v.depth = so[1] >= 0 ? reinterpret_cast<float*>(sb + so[1]) : nullptr;
+        v.normal = so[2] >= 0 ? reinterpret_cast<float*>(sb + so[2]) : nullptr;
+        v.mask = so[3] >= 0 ? sb + so[3] : nullptr;
+        v.face_id = so[4] >= 0 ? reinterpret_cast<int32_t*>(sb + so[4]) : nullptr;
+      }
+    } else {
+      v.image = images[i];
+      v.face_id = fids ? fids[i] : nullptr;
+    }
+  }
+  std::memcpy(reinterpret_cast<uint8_t*>(c->h_views) + size_t(n) * sizeof(DevView),
+              c->h_tile_base.data(), size_t(n + 1) * 4);
+
+  int persistent_blocks = 0;
+  if (c->tiling.persistent) {
+    const int per_cu = raster_blocks_per_cu(tw, th, mode);
+    persistent_blocks = std::max(1, per_cu) * std::max(1, c->cu_count);
+    if (persistent_blocks > total_tiles) persistent_blocks = 0;  // plain grid is enough
+  }
+
+  c->tm = UguTimings{};
+  for (int attempt = 0; attempt < 8; attempt++) {
+    if ((err = ensure(c, c->bins, size_t(total_tiles) * size_t(c->tiling.bin_capacity) * 4))) return err;
+    DevBins b;
+    b.counts = c->counts.as<int32_t>();
+    b.bins = c->bins.as<int32_t>();
+    b.cap = c->tiling.bin_capacity;
+    b.total_tiles = total_tiles;
+    b.status = c->status.as<int32_t>();
+    b.tile_base = c->tile_base.as<int32_t>();
+    b.n_views = n;
+    const DevView* dv = c->views.as<DevView>();
+    float4* proj = c->proj.as<float4>();
+
+    UGU_HIP_CHECK(c, hipMemcpyAsync(c->views.p, c->h_views, size_t(n) * sizeof(DevView),
+                                    hipMemcpyHostToDevice, c->stream));
+    UGU_HIP_CHECK(c, hipMemcpyAsync(c->tile_base.p,
+                                    reinterpret_cast<uint8_t*>(c->h_views) + size_t(n) * sizeof(DevView),
+                                    size_t(n + 1) * 4, hipMemcpyHostToDevice, c->stream));
+
+    auto launch_all = [&](bool timed) -> int {
+      if (timed) (void)hipEventRecord(c->ev[0], c->stream);
+      if (launch_clear(c->stream, b.counts, total_tiles, b.status)) return -1;
+      if (timed) (void)hipEventRecord(c->ev[1], c->stream);
+      if (launch_project(c->stream, dv, n, dm.verts, dm.V, proj)) return -1;
+      if (timed) (void)hipEventRecord(c->ev[2], c->stream);
+      if (launch_setup_bin(c->stream, tw, th, dv, n, dm, proj, b)) return -1;
+      if (timed) (void)hipEventRecord(c->ev[3], c->stream);
+      if (launch_raster(c->stream, tw, th, mode, dv, dm, proj, b, dopt, persistent_blocks)) return -1;
+      if (timed) (void)hipEventRecord(c->ev[4], c->stream);
+      return 0;
+    };
+
+    if (use_graph && !c->profile) {
+      GraphKey key;
+      std::memset(&key, 0, sizeof(key));
+      key.n = n;
+      key.total_tiles = total_tiles;
+      key.V = dm.V;
+      key.F = dm.F;
+      key.tw = tw;
+      key.th = th;
+      key.persistent_blocks = persistent_blocks;
+      key.mode = mode;
+      key.buf_gen = c->buf_gen;
+      key.mesh_gen = c->mesh_gen;
+      key.opt = dopt;
+      key.verts = dm.verts;
+      key.vcolors = dm.vcolors;
+      key.ftex = dm.ftex;
+      key.ftex_w = dm.ftex_w;
+      key.ftex_h = dm.ftex_h;
+      hipGraphExec_t exec = nullptr;
+      for (auto& g : c->graphs)
+        if (g.key == key) exec = g.exec;
+      if (!exec) {
+        hipGraph_t graph = nullptr;
+        UGU_HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int le = launch_all(false);
+        hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+        if (le || ce != hipSuccess) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return fail(c, UGU_HIP_ERR_HIP, "graph capture failed");
+        }
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return hip_fail(c, ie, "hipGraphInstantiate");
+        if (c->graphs.size() >= kMaxGraphs) {
+          (void)hipGraphExecDestroy(c->graphs.front().exec);
+          c->graphs.erase(c->graphs.begin());
+        }
+        c->graphs.push_back(GraphEntry{key, exec});
+      }
+      UGU_HIP_CHECK(c, hipGraphLaunch(exec, c->stream));
+    } else {
+      if (launch_all(c->profile)) return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
+    }
+    UGU_HIP_CHECK(c, hipMemcpyAsync(c->h_status, c->status.p, 16, hipMemcpyDeviceToHost, c->stream));
+    UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    const int max_count = c->h_status[0];
+    c->tm.max_bin_count = std::max(c->tm.max_bin_count, max_count);
+    if (c->profile) {
+      float ms[4] = {0, 0, 0, 0};
+      for (int k = 0; k < 4; k++) (void)hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]);
+      c->tm.clear_ms += ms[0];
+      c->tm.project_ms += ms[1];
+      c->tm.setup_bin_ms += ms[2];
+      c->tm.raster_ms += ms[3];
+      c->tm.total_ms += ms[0] + ms[1] + ms[2] + ms[3];
+    }
+    if (max_count <= c->tiling.bin_capacity) break;
+    // a bin overflowed: grow to the measured occupancy and render again
+    c->tm.retries++;
+    int64_t cap = int64_t(max_count) + max_count / 8 + 64;
+    cap = (cap + 63) / 64 * 64;
+    if (cap * int64_t(total_tiles) * 4 > (int64_t(64) << 30))
+      return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, "bins would exceed 64 GiB");
+    c->tiling.bin_capacity = static_cast<int32_t>(cap);
+    if (attempt == 7) return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, "bin capacity did not converge");
+  }
+
+  if (mode == 0 && memory == UGU_HIP_MEMORY_HOST) {
+    const uint8_t* sb = c->stage_out.as<uint8_t>();
+    for (int i = 0; i < n; i++) {
+      const int64_t px = int64_t(cams[i].width) * cams[i].height;
+      const int64_t sz[5] = {3 * px, 4 * px, 12 * px, px, 4 * px};
+      void* dst[5] = {outs[i].color, outs[i].depth, outs[i].normal, outs[i].mask, outs[i].face_id};
+      for (int k = 0; k < 5; k++) {
+        const int64_t off = stage_off[size_t(i) * 5 + k];
+        if (off >= 0)
+          UGU_HIP_CHECK(c, hipMemcpyAsync(dst[k], sb + off, size_t(sz[k]), hipMemcpyDeviceToHost, c->stream));
+      }
+    }
+    UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  }
+  return UGU_HIP_OK;
+}
+
+int render_common(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions* o,
+                  const UguOutputs* outs, int memory, bool use_graph) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  if (n < 1 || cams == nullptr || outs == nullptr)
+    return fail(c, n < 1 ? UGU_HIP_ERR_INVALID_ARGUMENT : UGU_HIP_ERR_NO_CAMERA,
+                cams == nullptr ? "camera has not been set\n" : "invalid view count or outputs");
+  if (memory != UGU_HIP_MEMORY_HOST && memory != UGU_HIP_MEMORY_DEVICE)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "unknown memory kind");
+  for (int i = 0; i < n; i++) {
+    const UguOutputs& q = outs[i];
+    int err = validate(c, &cams[i], o, q.color || q.depth || q.normal || q.mask || q.face_id);
+    if (err) return err;
+  }
+  UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  DevMesh dm = dev_mesh(c);
+  return run_pipeline(c, cams, n, dm, dev_options(*o), 0, outs, memory, nullptr, nullptr, use_graph);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ugu_hip_abi_version(void) { return UGU_HIP_ABI_VERSION; }
+
+ugu_hip_ctx* ugu_hip_create(int device) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return nullptr;
+  if (hipSetDevice(device) != hipSuccess) return nullptr;
+  ugu_hip_ctx* c = new ugu_hip_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return nullptr;
+  }
+  c->stream = c->own;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->h_status), 64, hipHostMallocDefault) != hipSuccess) {
+    (void)hipStreamDestroy(c->own);
+    delete c;
+    return nullptr;
+  }
+  for (auto& e : c->ev) (void)hipEventCreate(&e);
+  return c;
+}
+
+void ugu_hip_destroy(ugu_hip_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graphs(c);
+  DevBuf* bufs[] = {&c->verts, &c->faces, &c->fnormals, &c->normals, &c->nidx,
+                    &c->vcolors, &c->uv, &c->uvidx, &c->matid, &c->texinfo,
+                    &c->texdata, &c->proj, &c->counts, &c->bins, &c->status,
+                    &c->views, &c->tile_base, &c->stage_out, &c->diff_fn};
+  for (DevBuf* b : bufs) b->release();
+  if (c->h_views) (void)hipHostFree(c->h_views);
+  if (c->h_status) (void)hipHostFree(c->h_status);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+}
+
+const char* ugu_hip_last_error(const ugu_hip_ctx* c) {
+  return c ? c->err.c_str() : "null context";
+}
+
+int ugu_hip_set_stream(ugu_hip_ctx* c, void* s) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  hipStream_t ns = s ? static_cast<hipStream_t>(s) : c->own;
+  if (ns != c->stream) drop_graphs(c);
+  c->stream = ns;
+  return UGU_HIP_OK;
+}
+
+int ugu_hip_prepare_mesh(ugu_hip_ctx* c, const UguMeshDesc* m) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  c->prepared = false;
+  if (m == nullptr) return fail(c, UGU_HIP_ERR_NO_MESH, "mesh has not been set\n");
+  if (m->num_vertices < 0 || m->num_faces < 0 || (m->num_vertices > 0 && !m->vertices) ||
+      (m->num_faces > 0 && !m->faces))
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "vertices/faces missing");
+  const int64_t F = m->num_faces;
+  if (!check_idx(m->faces, F * 3, m->num_vertices))
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "face index out of range");
+  const bool has_normals = m->normals && m->num_normals > 0;
+  if (m->normal_indices && !check_idx(m->normal_indices, F * 3, has_normals ? m->num_normals : 0))
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "normal index out of range");
+  const bool has_vc = m->vertex_colors && m->num_vertex_colors > 0;
+  if (has_vc && m->num_vertex_colors < m->num_vertices)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "fewer vertex colours than vertices");
+  const bool has_uv = m->uv && m->num_uv > 0;
+  if (m->uv_indices && !check_idx(m->uv_indices, F * 3, has_uv ? m->num_uv : 0))
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "uv index out of range");
+  if (m->num_materials < 0 || (m->num_materials > 0 && !m->textures))
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "materials missing");
+  if (m->material_ids && !check_idx(m->material_ids, F, m->num_materials))
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "material id out of range");
+  UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+
+  int err;
+  if ((err = upload(c, c->verts, m->vertices, size_t(m->num_vertices) * 12))) return err;
+  if ((err = upload(c, c->faces, m->faces, size_t(F) * 12))) return err;
+  if ((err = upload(c, c->fnormals, F > 0 ? m->face_normals : nullptr, size_t(F) * 12))) return err;
+  if ((err = upload(c, c->normals, has_normals ? m->normals : nullptr, size_t(m->num_normals) * 12))) return err;
+  if ((err = upload(c, c->nidx, m->normal_indices, size_t(F) * 12))) return err;
+  if ((err = upload(c, c->vcolors, has_vc ? m->vertex_colors : nullptr, size_t(m->num_vertices) * 12))) return err;
+  if ((err = upload(c, c->uv, has_uv ? m->uv : nullptr, size_t(m->num_uv) * 8))) return err;
+  if ((err = upload(c, c->uvidx, m->uv_indices, size_t(F) * 12))) return err;
+  if ((err = upload(c, c->matid, m->material_ids, size_t(F) * 4))) return err;
+
+  c->M = m->num_materials;
+  c->tex_empty.assign(size_t(c->M), 0);
+  std::vector<TexInfo> ti(size_t(c->M));
+  int64_t pool = 0;
+  for (int i = 0; i < c->M; i++) {
+    const UguTexture& t = m->textures[i];
+    const bool empty = t.width < 1 || t.height < 1 || t.data == nullptr;
+    c->tex_empty[i] = empty ? 1 : 0;
+    ti[i].offset = pool;
+    ti[i].w = empty ? 1 : t.width;
+    ti[i].h = empty ? 1 : t.height;
+    pool += empty ? 4 : (int64_t(t.width) * t.height * 3 + 255) / 256 * 256;
+  }
+  if (c->M > 0) {
+    if ((err = upload(c, c->texinfo, ti.data(), ti.size() * sizeof(TexInfo)))) return err;
+    if (c->texdata.ensure(size_t(pool)) < 0) return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, "device allocation failed");
+    for (int i = 0; i < c->M; i++) {
+      const UguTexture& t = m->textures[i];
+      if (c->tex_empty[i]) continue;
+      UGU_HIP_CHECK(c, hipMemcpy(c->texdata.as<uint8_t>() + ti[i].offset, t.data,
+                                 size_t(t.width) * t.height * 3, hipMemcpyHostToDevice));
+    }
+  } else {
+    c->texinfo.release();
+    c->texdata.release();
+  }
+
+  c->V = m->num_vertices;
+  c->F = m->num_faces;
+  c->N = has_normals ? m->num_normals : 0;
+  c->U = has_uv ? m->num_uv : 0;
+  c->has_fn = m->face_normals != nullptr && F > 0;
+  c->has_normals = has_normals;
+  c->has_nidx = m->normal_indices != nullptr && F > 0;
+  c->has_vc = has_vc;
+  c->has_uv = has_uv;
+  c->has_uvidx = m->uv_indices != nullptr && F > 0;
+  c->has_matid = m->material_ids != nullptr && F > 0;
+  c->mesh_gen++;
+  c->prepared = true;
+  return UGU_HIP_OK;
+}
+
+int ugu_hip_render(ugu_hip_ctx* c, const UguCamera* cam, const UguOptions* o,
+                   const UguOutputs* out, int memory) {
+  if (c && cam == nullptr) return fail(c, UGU_HIP_ERR_NO_CAMERA, "camera has not been set\n");
+  return render_common(c, cam, 1, o, out, memory, true);
+}
+
+int ugu_hip_render_batch(ugu_hip_ctx* c, const UguCamera* cams, int n,
+                         const UguOptions* o, const UguOutputs* outs, int memory) {
+  return render_common(c, cams, n, o, outs, memory, true);
+}
+
+int ugu_hip_render_batch_eager(ugu_hip_ctx* c, const UguCamera* cams, int n,
+                               const UguOptions* o, const UguOutputs* outs, int memory) {
+  return render_common(c, cams, n, o, outs, memory, false);
+}
+
+int ugu_hip_set_tiling(ugu_hip_ctx* c, const UguTiling* t) {
+  if (!c || !t) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  if (!valid_tile(t->tile_w, t->tile_h))
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "tile must be 8x8, 16x16 or 32x8");
+  if (t->bin_capacity < 1) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "bin capacity must be >= 1");
+  c->tiling = *t;
+  drop_graphs(c);
+  return UGU_HIP_OK;
+}
+
+int ugu_hip_get_tiling(const ugu_hip_ctx* c, UguTiling* t) {
+  if (!c || !t) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  *t = c->tiling;
+  return UGU_HIP_OK;
+}
+
+int ugu_hip_enable_profiling(ugu_hip_ctx* c, int enable) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  c->profile = enable != 0;
+  return UGU_HIP_OK;
+}
+
+int ugu_hip_get_timings(const ugu_hip_ctx* c, UguTimings* t) {
+  if (!c || !t) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  *t = c->tm;
+  return UGU_HIP_OK;
+}
+
+int ugu_hip_autotune(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions* o,
+                     int repeats, UguTiling* chosen) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  if (n < 1 || !cams) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "no views to tune on");
+  if (repeats < 1) repeats = 1;
+  UguOptions opt = *o;
+  int err = validate(c, &cams[0], &opt, true);
+  if (err) return err;
+  // outputs: device buffers sized for the views (all five requested)
+  int64_t px = 0;
+  for (int i = 0; i < n; i++) px += int64_t(cams[i].width) * cams[i].height;
+  DevBuf outbuf;
+  if (outbuf.ensure(size_t(px) * 24 + size_t(n) * 5 * 256) < 0)
+    return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, "device allocation failed");
+  std::vector<UguOutputs> outs(static_cast<size_t>(n));
+  uint8_t* p = outbuf.as<uint8_t>();
+  for (int i = 0; i < n; i++) {
+    const int64_t q = int64_t(cams[i].width) * cams[i].height;
+    auto take = [&](int64_t bytes) {
+      uint8_t* r = p;
+      p += (bytes + 255) / 256 * 256;
+      return r;
+    };
+    outs[i].color = take(3 * q);
+    outs[i].depth = reinterpret_cast<float*>(take(4 * q));
+    outs[i].normal = reinterpret_cast<float*>(take(12 * q));
+    outs[i].mask = take(q);
+    outs[i].face_id = reinterpret_cast<int32_t*>(take(4 * q));
+  }
+  const int shapes[3][2] = {{8, 8}, {16, 16}, {32, 8}};
+  const UguTiling saved = c->tiling;
+  UguTiling best = saved;
+  double best_ms = 1e30;
+  for (auto& sh : shapes) {
+    UguTiling t = saved;
+    t.tile_w = sh[0];
+    t.tile_h = sh[1];
+    t.bin_capacity = 256;
+    c->tiling = t;
+    drop_graphs(c);
+    // first render sizes the bins from the measured occupancy
+    err = render_common(c, cams, n, &opt, outs.data(), UGU_HIP_MEMORY_DEVICE, true);
+    if (err) break;
+    std::vector<double> times;
+    for (int r = 0; r < repeats; r++) {
+      auto t0 = std::chrono::steady_clock::now();
+      err = render_common(c, cams, n, &opt, outs.data(), UGU_HIP_MEMORY_DEVICE, true);
+      auto t1 = std::chrono::steady_clock::now();
+      if (err) break;
+      times.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+    }
+    if (err) break;
+    std::sort(times.begin(), times.end());
+    const double med = times[times.size() / 2];
+    if (med < best_ms) {
+      best_ms = med;
+      best = c->tiling;
+    }
+  }
+  outbuf.release();
+  c->buf_gen++;
+  if (err) {
+    c->tiling = saved;
+    drop_graphs(c);
+    return err;
+  }
+  c->tiling = best;
+  drop_graphs(c);
+  if (chosen) *chosen = best;
+  return UGU_HIP_OK;
+}
+
+int ugu_hip_depth_to_u16(ugu_hip_ctx* c, const float* depth, int64_t n, uint16_t* out,
+                         int memory) {
+  if (!c || (n > 0 && (!depth || !out))) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  if (memory == UGU_HIP_MEMORY_HOST) {
+    for (int64_t i = 0; i < n; i++) {
+      const double s = static_cast<double>(1.0f) * static_cast<double>(depth[i]);
+      const int32_t iv = (s >= -2147483648.0 && s < 2147483648.0) ? static_cast<int32_t>(s) : INT32_MIN;
+      out[i] = static_cast<uint16_t>(static_cast<uint32_t>(iv) & 0xffffu);
+    }
+    return UGU_HIP_OK;
+  }
+  UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  if (launch_depth_to_u16(c->stream, depth, n, out)) return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
+  UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return UGU_HIP_OK;
+}
+
+static int diff_mesh(ugu_hip_ctx* c, const UguDiffParams* p, DevMesh* dm) {
+  if (!c->prepared) return fail(c, UGU_HIP_ERR_MESH_NOT_PREPARED, "mesh has not been initialized\n");
+  if (!p || !p->vertices) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "vertices are required");
+  if (p->texture && (!c->has_uv || !c->has_uvidx || p->tex_width < 1 || p->tex_height < 1))
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "texture needs uv, uv_indices and a size");
+  *dm = dev_mesh(c);
+  dm->verts = p->vertices;
+  dm->vcolors = p->vertex_colors;
+  dm->ftex = p->texture;
+  dm->ftex_w = p->texture ? p->tex_width : 0;
+  dm->ftex_h = p->texture ? p->tex_height : 0;
+  int err = ensure(c, c->diff_fn, size_t(std::max(c->F, 1)) * 12);
+  if (err) return err;
+  dm->fnormals = c->diff_fn.as<float>();
+  return 0;
+}
+
+int ugu_hip_diff_forward(ugu_hip_ctx* c, const UguCamera* cams, int n,
+                         const UguDiffParams* p, float* const* images,
+                         int32_t* const* face_ids) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  if (n < 1 || !cams || !images || !face_ids)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "views, images and face_ids are required");
+  for (int i = 0; i < n; i++)
+    if (!images[i] || !face_ids[i]) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "null image");
+  UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  DevMesh dm;
+  int err = diff_mesh(c, p, &dm);
+  if (err) return err;
+  if (launch_face_normals(c->stream, dm.verts, dm.faces, dm.F, c->diff_fn.as<float>()))
+    return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
+  UguOptions o{};
+  o.backface_culling = p->backface_culling;
+  DevOptions dopt = dev_options(o);
+  return run_pipeline(c, cams, n, dm, dopt, 1, nullptr, UGU_HIP_MEMORY_DEVICE, images,
+                      face_ids, true);
+}
+
+int ugu_hip_diff_backward(ugu_hip_ctx* c, const UguCamera* cams, int n,
+                          const UguDiffParams* p, const float* const* grad_images,
+                          const int32_t* const* face_ids, float* grad_vertices,
+                          float* grad_vertex_colors, float* grad_texture) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  if (n < 1 || !cams || !grad_images || !face_ids)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "views, gradients and face_ids are required");
+  UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  DevMesh dm;
+  int err = diff_mesh(c, p, &dm);
+  if (err) return err;
+  int max_px = 0;
+  if ((err = ensure(c, c->views, size_t(n) * sizeof(DevView)))) return err;
+  if ((err = ensure(c, c->proj, size_t(n) * std::max<int32_t>(dm.V, 1) * sizeof(float4)))) return err;
+  const size_t view_bytes = size_t(n) * sizeof(DevView) + size_t(n + 1) * 4;
+  if (c->h_views_bytes < view_bytes) {
+    if (c->h_views) (void)hipHostFree(c->h_views);
+    c->h_views = nullptr;
+    c->h_views_bytes = 0;
+    UGU_HIP_CHECK(c, hipHostMalloc(&c->h_views, view_bytes, hipHostMallocDefault));
+    c->h_views_bytes = view_bytes;
+  }
+  DevView* hv = static_cast<DevView*>(c->h_views);
+  for (int i = 0; i < n; i++) {
+    if ((err = check_camera(c, cams[i]))) return err;
+    if (!grad_images[i] || !face_ids[i]) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "null gradient");
+    DevView& v = hv[i];
+    std::memset(&v, 0, sizeof(v));
+    v.W = cams[i].width;
+    v.H = cams[i].height;
+    v.fx = cams[i].fx;
+    v.fy = cams[i].fy;
+    v.cx = cams[i].cx;
+    v.cy = cams[i].cy;
+    std::memcpy(v.w2c_R, cams[i].w2c_R, sizeof(v.w2c_R));
+    std::memcpy(v.w2c_t, cams[i].w2c_t, sizeof(v.w2c_t));
+    std::memcpy(v.c2w_R, cams[i].c2w_R, sizeof(v.c2w_R));
+    v.proj_offset = int64_t(i) * dm.V;
+    v.face_id = const_cast<int32_t*>(face_ids[i]);
+    v.grad = grad_images[i];
+    max_px = std::max(max_px, v.W * v.H);
+  }
+  UGU_HIP_CHECK(c, hipMemcpyAsync(c->views.p, c->h_views, size_t(n) * sizeof(DevView),
+                                  hipMemcpyHostToDevice, c->stream));
+  const DevView* dv = c->views.as<DevView>();
+  float4* proj = c->proj.as<float4>();
+  if (launch_project(c->stream, dv, n, dm.verts, dm.V, proj))
+    return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
+  DevGrads g;
+  g.verts = grad_vertices;
+  g.colors = p->vertex_colors ? grad_vertex_colors : nullptr;
+  g.texture = p->texture ? grad_texture : nullptr;
+  if (launch_backward(c->stream, dv, n, max_px, dm, proj, g))
+    return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
+  UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return UGU_HIP_OK;
+}
+
+}  // extern "C"
