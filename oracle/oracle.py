@@ -42,14 +42,23 @@ class OracleOptions(ctypes.Structure):
 
 
 _lib = None
+_lib_path = LIB_PATH
+
+
+def use_library(path=None):
+    """Switch to another build of the oracle (the planted-mistake builds of the
+    mutation tests); None restores oracle/liboracle.so."""
+    global _lib, _lib_path
+    _lib = None
+    _lib_path = path or LIB_PATH
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
-        L = ctypes.CDLL(LIB_PATH)
+        if not os.path.exists(_lib_path):
+            raise RuntimeError(f"{_lib_path} missing: run `make -C oracle`")
+        L = ctypes.CDLL(_lib_path)
         L.oracle_render.restype = ctypes.c_int
         L.oracle_render.argtypes = [ctypes.POINTER(OracleMesh), ctypes.POINTER(OracleCamera), _P,
                                     ctypes.POINTER(OracleOptions), _P, _P, _P, _P, _P,

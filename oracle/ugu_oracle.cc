@@ -42,6 +42,14 @@
 #include <omp.h>
 #endif
 
+// Planted-mistake switch for the oracle's own pin tests
+// (tests/test_oracle_mutations.py builds the oracle with -DORACLE_MUTATION=k
+// and checks that the pins fail); 0 = the faithful restatement.
+#ifndef ORACLE_MUTATION
+#define ORACLE_MUTATION 0
+#endif
+#define MUT(k) (ORACLE_MUTATION == (k))
+
 #if defined(__FP_FAST_FMAF) || defined(__FAST_MATH__)
 #error "oracle must be compiled without FMA contraction or fast-math"
 #endif
@@ -290,8 +298,8 @@ void ShadePixel(const ShadeCtx& s, uint8_t* c, float u, float v, int fid,
     float fy = (1.0f - iuv[1]) * static_cast<float>(t.h - 1);
     if (s.o->interp == 0) {
       // DiffuseTextureNnColorizer (:260-298)
-      int tx = f2i_x86(std::round(fx));
-      int ty = f2i_x86(std::round(fy));
+      int tx = f2i_x86(MUT(7) ? std::floor(fx) : std::round(fx));
+      int ty = f2i_x86(MUT(7) ? std::floor(fy) : std::round(fy));
       const uint8_t* dt = t.at(ty, tx);
       for (int k = 0; k < 3; k++) {
         float ic = static_cast<float>(dt[k]);
@@ -305,6 +313,7 @@ void ShadePixel(const ShadeCtx& s, uint8_t* c, float u, float v, int fid,
       int y1 = y0 + 1;
       float lu = fx - static_cast<float>(x0);
       float lv = fy - static_cast<float>(y0);
+      if (MUT(8)) std::swap(lu, lv);
       const uint8_t* a = t.at(y0, x0);
       const uint8_t* b = t.at(y0, x1);
       const uint8_t* cc = t.at(y1, x0);
@@ -324,7 +333,7 @@ void ShadePixel(const ShadeCtx& s, uint8_t* c, float u, float v, int fid,
   // shader
   if (s.o->diffuse_shading == 1) {
     // DiffuseLambertianShader (:370-388)
-    float coeff = -dot3(light_dir, n);
+    float coeff = MUT(9) ? dot3(light_dir, n) : -dot3(light_dir, n);
     if (coeff < 0.0f) coeff = 0.0f;
     for (int k = 0; k < 3; k++) c[k] = f2u8_x86(coeff * static_cast<float>(c[k]));
   } else if (s.o->diffuse_shading == 2) {
@@ -419,7 +428,7 @@ int oracle_render(const OracleMesh* mesh, const OracleCamera* cam,
     // PinholeCamera::Project (camera.h:380-387): divide first, then multiply
     const V3& p = camera_vertices[i];
     image_vertices[i][0] = cam->fx / p[2] * p[0] + cam->cx;
-    image_vertices[i][1] = cam->fy / p[2] * p[1] + cam->cy;
+    image_vertices[i][1] = cam->fy / p[2] * p[1] + (MUT(10) ? cam->cx : cam->cy);
     image_vertices[i][2] = p[2];
   }
 
@@ -463,9 +472,10 @@ int oracle_render(const OracleMesh* mesh, const OracleCamera* cam,
     for (int32_t y = y0; y <= y1; ++y) {
       for (int32_t x = x0; x <= x1; ++x) {
         const V3& ray_w = rt.ray_w[static_cast<size_t>(y) * W + x];
-        bool backface = m.face_normals ? dot3(fn, ray_w) > 0 : false;
+        bool backface = m.face_normals ? (MUT(2) ? dot3(fn, ray_w) < 0 : dot3(fn, ray_w) > 0) : false;
         V3 pixel_sample{{static_cast<float>(x), static_cast<float>(y), 0.0f}};
-        float w0 = EdgeFunction(v1_i, v2_i, pixel_sample);
+        float w0 = MUT(1) ? EdgeFunction(v2_i, v1_i, pixel_sample)
+                          : EdgeFunction(v1_i, v2_i, pixel_sample);
         float w1 = EdgeFunction(v2_i, v0_i, pixel_sample);
         float w2 = EdgeFunction(v0_i, v1_i, pixel_sample);
         if ((!backface && (w0 >= 0 && w1 >= 0 && w2 >= 0)) ||
@@ -473,16 +483,21 @@ int oracle_render(const OracleMesh* mesh, const OracleCamera* cam,
           w0 /= area;
           w1 /= area;
           w2 /= area;
-          w0 /= v0_i[2];
-          w1 /= v1_i[2];
-          w2 /= v2_i[2];
-          pixel_sample[2] = 1.0f / (w0 + w1 + w2);
-          w0 = w0 * pixel_sample[2];
-          w1 = w1 * pixel_sample[2];
-          w2 = w2 * pixel_sample[2];
+          if (MUT(4)) {
+            // affine (not perspective-correct) interpolation
+            pixel_sample[2] = w0 * v0_i[2] + w1 * v1_i[2] + w2 * v2_i[2];
+          } else {
+            w0 /= v0_i[2];
+            w1 /= v1_i[2];
+            w2 /= v2_i[2];
+            pixel_sample[2] = 1.0f / (w0 + w1 + w2);
+            w0 = w0 * pixel_sample[2];
+            w1 = w1 * pixel_sample[2];
+            w2 = w2 * pixel_sample[2];
+          }
           size_t pi = static_cast<size_t>(y) * W + x;
           float& d = depth_[pi];
-          if (d < flt_min || pixel_sample[2] < d) {
+          if (d < flt_min || (MUT(3) ? pixel_sample[2] <= d : pixel_sample[2] < d)) {
             d = pixel_sample[2];
             face_id_[pi] = i;
             weight_image[pi * 3 + 0] = w0;
@@ -501,12 +516,12 @@ int oracle_render(const OracleMesh* mesh, const OracleCamera* cam,
       size_t pi = static_cast<size_t>(y) * W + x;
       const uint8_t bf = backface_image[pi];
       int32_t& fid = face_id_[pi];
-      if (o.backface_culling && bf == 255) {
+      if (o.backface_culling && bf == 255 && !MUT(13)) {
         depth_[pi] = 0.0f;
         fid = -1;
         continue;
       }
-      if (fid > 0) {
+      if (MUT(5) ? fid >= 0 : fid > 0) {
         const V3& ray_w = rt.ray_w[pi];
         float w0 = weight_image[pi * 3 + 0];
         float w1 = weight_image[pi * 3 + 1];
@@ -524,11 +539,20 @@ int oracle_render(const OracleMesh* mesh, const OracleCamera* cam,
         }
         if (normal) {
           V3 snc = matvec(cam->w2c_R, sn);
+          if (MUT(12)) {
+            float Rt[9];
+            for (int a = 0; a < 3; a++)
+              for (int b = 0; b < 3; b++) Rt[a * 3 + b] = cam->w2c_R[b * 3 + a];
+            snc = matvec(Rt, sn);
+          }
           for (int k = 0; k < 3; k++) normal[pi * 3 + k] = snc[k];
         }
         if (color) {
           V3 light_dir = ray_w;
-          ShadePixel(sc, color + pi * 3, w1, w2, fid, ray_w, light_dir, sn);
+          if (MUT(6))
+            ShadePixel(sc, color + pi * 3, w2, w1, fid, ray_w, light_dir, sn);
+          else
+            ShadePixel(sc, color + pi * 3, w1, w2, fid, ray_w, light_dir, sn);
         }
       }
     }

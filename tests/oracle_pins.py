@@ -1,0 +1,378 @@
+"""Pins of the CPU oracle against facts that do not come from the oracle.
+
+Each pin returns (ok, message). test_oracle_pins.py requires every pin to
+pass on the faithful oracle; test_oracle_mutations.py rebuilds the oracle
+with planted mistakes and requires at least one pin to fail on each.
+
+Sources of truth used here:
+* an independent float64 brute-force rasterizer (per pixel, per face,
+  exact-sign coverage, the reference's sequential depth rule);
+* closed forms of a sphere seen by a pinhole camera (silhouette radius
+  f R / sqrt(d^2 - R^2), near-point depth d - R, analytic normals);
+* ray-plane intersections for perspective-correct attribute interpolation;
+* hand-computed values of the truncating conversions.
+"""
+import math
+
+import numpy as np
+
+import oracle as O
+import tiny_scenes as TS
+from ugu_amd import (ColorInterpolation, DiffuseColor, DiffuseShading, Mesh, PinholeCamera,
+                     RendererCpuOption, ShadingNormal, look_at_c2w, scenes)
+
+FLT_MIN = float(np.finfo(np.float32).tiny)
+
+
+def render(mesh, cam, opt):
+    rc, out, _ = O.render(mesh, O.camera_from_abi(cam.to_abi()), opt, c2w_t=cam.c2w[:3, 3])
+    assert rc == 0, rc
+    return out
+
+
+# ---------------------------------------------------------------------------
+# float64 brute force
+
+
+def brute_force(mesh, cam, culling):
+    """Independent float64 rasterizer: returns (face_id, depth, ambiguous)."""
+    a = cam.to_abi()
+    R = np.array(a.w2c_R[:], np.float64).reshape(3, 3)
+    t = np.array(a.w2c_t[:], np.float64)
+    C = np.array(a.c2w_R[:], np.float64).reshape(3, 3)
+    fx, fy, cx, cy = float(a.fx), float(a.fy), float(a.cx), float(a.cy)
+    W, H = cam.width, cam.height
+    q = mesh.vertices.astype(np.float64) @ R.T + t
+    X = fx * q[:, 0] / q[:, 2] + cx
+    Y = fy * q[:, 1] / q[:, 2] + cy
+    Z = q[:, 2]
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
+    d = np.stack([(xs - cx) / fx, (ys - cy) / fy, np.ones_like(xs)], -1)
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    ray = d @ C.T
+    depth = np.zeros((H, W))
+    fid = -np.ones((H, W), np.int64)
+    bfimg = np.zeros((H, W), bool)
+    amb = np.zeros((H, W), bool)
+    fn = mesh.face_normals.astype(np.float64)
+    for i, (i0, i1, i2) in enumerate(mesh.vertex_indices):
+        if Z[i0] < 0 or Z[i1] < 0 or Z[i2] < 0:
+            continue
+        x0, y0, x1, y1, x2, y2 = X[i0], Y[i0], X[i1], Y[i1], X[i2], Y[i2]
+        A = (x2 - x0) * (y1 - y0) - (y2 - y0) * (x1 - x0)
+        if abs(A) < FLT_MIN:
+            continue
+        e0 = (xs - x1) * (y2 - y1) - (ys - y1) * (x2 - x1)
+        e1 = (xs - x2) * (y0 - y2) - (ys - y2) * (x0 - x2)
+        e2 = (xs - x0) * (y1 - y0) - (ys - y0) * (x1 - x0)
+        bf = (ray @ fn[i]) > 0
+        cov = np.where(bf, (e0 <= 0) & (e1 <= 0) & (e2 <= 0), (e0 >= 0) & (e1 >= 0) & (e2 >= 0))
+        # the reference only visits its integer bbox
+        bx0, bx1 = max(0, math.ceil(min(x0, x1, x2))), min(W - 1, math.floor(max(x0, x1, x2)))
+        by0, by1 = max(0, math.ceil(min(y0, y1, y2))), min(H - 1, math.floor(max(y0, y1, y2)))
+        inbox = (xs >= bx0) & (xs <= bx1) & (ys >= by0) & (ys <= by1)
+        cov &= inbox
+        near_edge = inbox & (np.minimum(np.minimum(abs(e0), abs(e1)), abs(e2)) < 1e-6 * abs(A))
+        amb |= near_edge & ~cov
+        with np.errstate(divide="ignore", invalid="ignore"):
+            z = 1.0 / (e0 / A / Z[i0] + e1 / A / Z[i1] + e2 / A / Z[i2])
+        for yy, xx in zip(*np.nonzero(cov)):
+            zz, dd = z[yy, xx], depth[yy, xx]
+            if dd > 0 and abs(zz - dd) <= 1e-5 * dd:
+                amb[yy, xx] = True
+            if dd < FLT_MIN or zz < dd:
+                depth[yy, xx] = zz
+                fid[yy, xx] = i
+                bfimg[yy, xx] = bf[yy, xx]
+    if culling:
+        fid[bfimg] = -1
+        depth[bfimg] = 0.0
+    return fid, depth, amb
+
+
+def pin_brute_force(seeds=range(8)):
+    for seed in seeds:
+        w, h = [(8, 8), (16, 12), (21, 17), (32, 32)][seed % 4]
+        tris = TS.random_triangles(8 + 2 * seed, w, h, 100 + seed)
+        mesh = TS.mesh_from_pixels(tris, seed=seed)
+        cam = TS.exact_camera(w, h)
+        for cull in (False, True):
+            opt = RendererCpuOption(diffuse_color=DiffuseColor.kVertex, backface_culling=cull)
+            out = render(mesh, cam, opt)
+            fid, depth, amb = brute_force(mesh, cam, cull)
+            ok = (out["face_id"] == fid) | amb
+            if not ok.all():
+                return False, f"brute force seed {seed} cull {cull}: face_id differs at " \
+                              f"{np.argwhere(~ok)[:3].tolist()}"
+            sel = (fid >= 0) & ~amb
+            if not np.allclose(out["depth"][sel], depth[sel], rtol=1e-5):
+                return False, f"brute force seed {seed}: depth differs"
+    return True, ""
+
+
+def pin_ties_lowest_id():
+    """Duplicated faces produce bitwise-equal depths: the lower id must win
+    (strict less in face order, rasterizer.cc:229)."""
+    tris = TS.random_triangles(10, 16, 16, 9, zs=(1.0,))
+    tris = np.concatenate([tris, tris], 0)
+    mesh = TS.mesh_from_pixels(tris, seed=2)
+    out = render(mesh, TS.exact_camera(16, 16),
+                 RendererCpuOption(diffuse_color=DiffuseColor.kVertex, backface_culling=False))
+    f = out["face_id"]
+    if (f >= 10).any():
+        return False, "a duplicated face with the higher id won a depth tie"
+    if not (f >= 0).any():
+        return False, "nothing covered"
+    return True, ""
+
+
+def pin_face_zero_not_shaded():
+    """Face 0 sets depth and face_id but is never shaded (`if (fid > 0)`,
+    rasterizer.cc:254)."""
+    tris = np.array([[(1, 1, 1), (14, 2, 1), (2, 14, 1)],
+                     [(8, 8, 2), (15, 9, 2), (9, 15, 2)]], np.float64)[:, ::-1, :]
+    mesh = TS.mesh_from_pixels(tris, seed=4)
+    out = render(mesh, TS.exact_camera(16, 16),
+                 RendererCpuOption(diffuse_color=DiffuseColor.kVertex))
+    z = out["face_id"] == 0
+    o = out["face_id"] == 1
+    if not z.any() or not o.any():
+        return False, "scene does not cover both faces"
+    if out["mask"][z].any() or out["color"][z].any() or out["normal"][z].any():
+        return False, "face 0 was shaded"
+    if not (out["depth"][z] > 0).all():
+        return False, "face 0 lost its depth"
+    if not (out["mask"][o] == 255).all():
+        return False, "face 1 not shaded"
+    return True, ""
+
+
+def _tilted_quad(texture=None, uvs=None, colors=None):
+    """A quad on the plane z = 2 + 0.8 x (strongly tilted), seen by a 64x48
+    camera at the origin; vertex colours linear in world x and y."""
+    P = np.array([[-1.0, -0.8], [1.0, -0.8], [1.0, 0.8], [-1.0, 0.8]])
+    verts = np.array([(x, y, 2.0 + 0.8 * x) for x, y in P], np.float32)
+    faces = np.array([[0, 2, 1], [0, 3, 2]], np.int32)
+    m = Mesh(verts, faces)
+    m.CalcFaceNormal()
+    m.set_normals(np.repeat(m.face_normals[:1], 4, axis=0))
+    m.set_normal_indices(faces)
+    if colors is None:
+        colors = np.stack([(verts[:, 0] + 1) * 100 + 20, (verts[:, 1] + 0.8) * 120 + 10,
+                           np.full(4, 200.0)], 1)
+    m.set_vertex_colors(colors.astype(np.float32))
+    if uvs is not None:
+        m.set_uv(uvs)
+        m.set_uv_indices(faces)
+    if texture is not None:
+        m.set_single_material(texture)
+    cam = PinholeCamera(64, 48, np.eye(4), (31.5, 23.5), (40.0, 40.0))
+    return m, cam
+
+
+def _plane_hits(cam):
+    """World points where each pixel's ray meets z = 2 + 0.8 x (float64)."""
+    fx, cx, cy = 40.0, 31.5, 23.5
+    ys, xs = np.mgrid[0:cam.height, 0:cam.width].astype(np.float64)
+    dx, dy = (xs - cx) / fx, (ys - cy) / fx
+    tz = 2.0 / (1.0 - 0.8 * dx)          # z = t, x = dx t
+    return dx * tz, dy * tz, tz
+
+
+def pin_perspective_correct():
+    m, cam = _tilted_quad()
+    out = render(m, cam, RendererCpuOption(diffuse_color=DiffuseColor.kVertex))
+    X, Y, Z = _plane_hits(cam)
+    sel = (out["face_id"] > 0) & (np.abs(X) < 0.97) & (np.abs(Y) < 0.77)
+    sel |= (out["face_id"] == 0) & False
+    cov = out["face_id"] >= 0
+    if cov.sum() < 500:
+        return False, "quad not covered"
+    if not np.allclose(out["depth"][cov & (np.abs(X) < 0.97) & (np.abs(Y) < 0.77)],
+                       Z[cov & (np.abs(X) < 0.97) & (np.abs(Y) < 0.77)], rtol=2e-5):
+        return False, "depth is not the ray-plane intersection depth"
+    exp_r = (X + 1) * 100 + 20
+    exp_g = (Y + 0.8) * 120 + 10
+    dr = np.abs(out["color"][..., 0][sel] - np.floor(exp_r[sel]))
+    dg = np.abs(out["color"][..., 1][sel] - np.floor(exp_g[sel]))
+    if dr.max() > 1 or dg.max() > 1:
+        return False, f"vertex colour not perspective-correct (max err {dr.max()}, {dg.max()})"
+    return True, ""
+
+
+def pin_texture_sampling():
+    """Texture with tex[r, c] = (c*2, r*3, 77): bilinear gives the linear
+    function of the texel position, NN the value at the rounded position."""
+    tw, th = 101, 71
+    cc, rr = np.meshgrid(np.arange(tw), np.arange(th))
+    tex = np.stack([cc * 2, rr * 3, np.full_like(cc, 77)], -1).astype(np.uint8)
+    uvs = np.array([[0.05, 0.1], [0.95, 0.1], [0.95, 0.9], [0.05, 0.9]], np.float32)
+    m, cam = _tilted_quad(texture=tex, uvs=uvs)
+    X, Y, _ = _plane_hits(cam)
+    # uv is linear in world x, y on the quad
+    u = 0.05 + (X + 1) / 2 * 0.9
+    v = 0.1 + (Y + 0.8) / 1.6 * 0.8
+    fx_, fy_ = u * (tw - 1), (1 - v) * (th - 1)
+    for interp in (ColorInterpolation.kBilinear, ColorInterpolation.kNn):
+        out = render(m, cam, RendererCpuOption(diffuse_color=DiffuseColor.kTexture, interp=interp))
+        sel = (out["face_id"] > 0) & (np.abs(X) < 0.97) & (np.abs(Y) < 0.77)
+        if sel.sum() < 300:
+            return False, "texture quad not covered"
+        if interp == ColorInterpolation.kBilinear:
+            er = np.abs(out["color"][..., 0][sel] - np.floor(fx_[sel] * 2))
+            eg = np.abs(out["color"][..., 1][sel] - np.floor(fy_[sel] * 3))
+            if er.max() > 1 or eg.max() > 1:
+                return False, f"bilinear texture wrong ({er.max()}, {eg.max()})"
+        else:
+            # skip samples within 1e-3 texel of a .5 boundary
+            safe = sel & (np.abs(fx_ - np.floor(fx_) - 0.5) > 1e-3) & \
+                (np.abs(fy_ - np.floor(fy_) - 0.5) > 1e-3)
+            er = out["color"][..., 0][safe] != np.round(fx_[safe]).astype(int) * 2
+            eg = out["color"][..., 1][safe] != np.round(fy_[safe]).astype(int) * 3
+            if er.any() or eg.any():
+                return False, "nearest-neighbour texture is not rounded"
+    return True, ""
+
+
+def _sphere_scene(w=160, h=120, f=125.0, d=3.0, rotated=False, nseg=48):
+    m = scenes.make_uv_sphere(nseg, nseg)
+    n = m.vertices / np.linalg.norm(m.vertices, axis=1, keepdims=True)
+    m.set_normals(n.astype(np.float32))
+    m.set_normal_indices(m.vertex_indices)
+    m.CalcFaceNormal()
+    m.set_vertex_colors(np.full((len(m.vertices), 3), 200.0, np.float32))
+    if rotated:
+        pos = np.array([1.3, -0.9, -2.5])
+        pos = pos / np.linalg.norm(pos) * d
+        c2w = look_at_c2w(pos, (0, 0, 0), (0.2, 1.0, 0.1))
+    else:
+        c2w = np.eye(4)
+        c2w[2, 3] = -d
+    cam = PinholeCamera(w, h, c2w, (w * 0.5 - 0.5, h * 0.5 - 0.5), (f, f))
+    return m, cam
+
+
+def pin_sphere_closed_forms():
+    m, cam = _sphere_scene()
+    out = render(m, cam, RendererCpuOption(diffuse_color=DiffuseColor.kVertex))
+    cov = out["face_id"] >= 0
+    rad = 125.0 / math.sqrt(9.0 - 1.0)
+    area = cov.sum()
+    if abs(area / (math.pi * rad * rad) - 1) > 0.02:
+        return False, f"silhouette area {area} vs {math.pi * rad * rad:.1f}"
+    ys, xs = np.nonzero(cov)
+    if abs(xs.mean() - 79.5) > 0.5 or abs(ys.mean() - 59.5) > 0.5:
+        return False, f"silhouette centroid ({xs.mean():.2f},{ys.mean():.2f}) not at the principal point"
+    dc = out["depth"][59:61, 79:81]
+    if not (np.abs(dc - 2.0) < 2e-3).all():
+        return False, f"near-point depth {dc.ravel()} != d - R"
+    # (1-u-v)*200 + u*200 + v*200 truncates to 200 or, after rounding, 199
+    col = out["color"][cov & (out["face_id"] > 0)]
+    if not np.isin(col, (199, 200)).all() or (col == 200).mean() < 0.5:
+        return False, "constant vertex colour is not reproduced"
+    return True, ""
+
+
+def _sphere_hits(cam):
+    """Camera-space hit points and normals of each pixel's ray with the unit
+    sphere (float64)."""
+    a = cam.to_abi()
+    R = np.array(a.w2c_R[:], np.float64).reshape(3, 3)
+    t = np.array(a.w2c_t[:], np.float64)
+    ccam = t  # sphere centre (world origin) in camera coordinates
+    ys, xs = np.mgrid[0:cam.height, 0:cam.width].astype(np.float64)
+    d = np.stack([(xs - a.cx) / a.fx, (ys - a.cy) / a.fy, np.ones_like(xs)], -1)
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    b = d @ ccam
+    disc = b * b - (ccam @ ccam - 1.0)
+    hit = disc > 0
+    s = b - np.sqrt(np.maximum(disc, 0))
+    P = d * s[..., None]
+    N = P - ccam
+    return hit, P, N, d, R
+
+
+def pin_normals_rotated_camera():
+    m, cam = _sphere_scene(rotated=True, nseg=96)
+    out = render(m, cam, RendererCpuOption(diffuse_color=DiffuseColor.kVertex,
+                                           shading_normal=ShadingNormal.kVertex))
+    hit, P, N, d, R = _sphere_hits(cam)
+    sel = (out["face_id"] > 0) & hit
+    # stay away from the silhouette
+    sel &= (-(d * N).sum(-1)) > 0.5
+    if sel.sum() < 1000:
+        return False, "too few pixels"
+    n_out = out["normal"][sel].astype(np.float64)
+    n_out /= np.linalg.norm(n_out, axis=-1, keepdims=True)
+    cosang = (n_out * N[sel]).sum(-1)
+    if cosang.min() < 0.995:
+        return False, f"camera-space normal deviates from the analytic normal (min cos {cosang.min():.4f})"
+    return True, ""
+
+
+def pin_lambertian():
+    m, cam = _sphere_scene(nseg=96)
+    for sn in (ShadingNormal.kFace, ShadingNormal.kVertex):
+        out = render(m, cam, RendererCpuOption(diffuse_color=DiffuseColor.kVertex,
+                                               shading_normal=sn,
+                                               diffuse_shading=DiffuseShading.kLambertian))
+        hit, P, N, d, R = _sphere_hits(cam)
+        sel = (out["face_id"] > 0) & hit & ((-(d * N).sum(-1)) > 0.3)
+        if sn == ShadingNormal.kFace:
+            # the winning face's normal, rotated to camera space (float64)
+            Nf = m.face_normals[out["face_id"].clip(0)].astype(np.float64) @ R.T
+            k = -(d * Nf).sum(-1)
+        else:
+            k = -(d * N).sum(-1)  # analytic sphere normal (camera space)
+        exp = np.floor(200 * k)
+        err = np.abs(out["color"][..., 0][sel].astype(float) - exp[sel])
+        tol = 1 if sn == ShadingNormal.kFace else 2
+        if err.max() > tol:
+            return False, f"Lambertian shading off by {err.max()} ({sn.name})"
+    return True, ""
+
+
+def pin_oren_nayar():
+    """Oren-Nayar with light = view ray (pixel_shader.h:392-422), sigma 0.3,
+    against the formula evaluated in float64 on the same normals."""
+    m, cam = _sphere_scene(nseg=64)
+    out = render(m, cam, RendererCpuOption(diffuse_color=DiffuseColor.kVertex,
+                                           shading_normal=ShadingNormal.kFace,
+                                           diffuse_shading=DiffuseShading.kOrenNayar))
+    s2 = 0.3 * 0.3
+    A = 1 - 0.5 * s2 / (s2 + 0.33)
+    B = 0.45 * s2 / (s2 + 0.09)
+    a = cam.to_abi()
+    C = np.array(a.c2w_R[:], np.float64).reshape(3, 3)
+    ys, xs = np.mgrid[0:cam.height, 0:cam.width].astype(np.float64)
+    d = np.stack([(xs - a.cx) / a.fx, (ys - a.cy) / a.fy, np.ones_like(xs)], -1)
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    r = d @ C.T
+    f = out["face_id"]
+    sel = f > 0
+    n = m.face_normals[f[sel]].astype(np.float64)
+    rr = r[sel]
+    dl = -(rr * n).sum(-1)
+    th = np.arccos(np.clip(dl, -1, 1))
+    bl = -n * dl[:, None] - rr
+    phi = np.maximum(0, (bl * bl).sum(-1))
+    coeff = np.maximum(0, dl) * (A + B * phi * np.sin(th) * np.tan(th))
+    exp = np.floor(coeff * 200)
+    err = np.abs(out["color"][..., 0][sel] - exp)
+    if err.max() > 1:
+        return False, f"Oren-Nayar off by {err.max()}"
+    return True, ""
+
+
+def pin_depth_u16():
+    d = np.array([2.9, 0.0, 65535.9, 65536.5, -1.5, 1234.999], np.float32)
+    exp = np.array([2, 0, 65535, 0, 65535, 1234], np.uint16)
+    got = O.depth_to_u16(d)
+    if not np.array_equal(got, exp):
+        return False, f"ConvertTo to uint16: {got} != {exp}"
+    return True, ""
+
+
+ALL_PINS = [pin_brute_force, pin_ties_lowest_id, pin_face_zero_not_shaded,
+            pin_perspective_correct, pin_texture_sampling, pin_sphere_closed_forms,
+            pin_normals_rotated_camera, pin_lambertian, pin_oren_nayar, pin_depth_u16]

@@ -1,0 +1,94 @@
+"""Small hand-built and seeded scenes for exact-arithmetic tests.
+
+With the camera at the origin looking down +z (identity pose), f = 1 and
+principal point (0, 0), a vertex (u*z, v*z, z) with z a power of two projects
+exactly to pixel coordinates (u, v): fx / z * X + cx = (1/z) * (u z) = u.
+Integer and half-integer (u, v) therefore put edges exactly through pixel
+centres (edge-function ties) and equal z gives exact depth ties.
+"""
+import numpy as np
+
+from ugu_amd import Mesh, PinholeCamera
+
+
+def exact_camera(w, h):
+    return PinholeCamera(w, h, np.eye(4), (0.0, 0.0), (1.0, 1.0))
+
+
+def mesh_from_pixels(tris_uvz, colors=None, seed=0):
+    """tris_uvz: (T, 3, 3) of (u, v, z) per corner -> Mesh with face normals,
+    per-vertex normals (= face normal), vertex colours and uvs."""
+    t = np.asarray(tris_uvz, np.float64)
+    T = len(t)
+    verts = np.empty((T * 3, 3), np.float32)
+    verts[:, 0] = (t[..., 0] * t[..., 2]).reshape(-1)
+    verts[:, 1] = (t[..., 1] * t[..., 2]).reshape(-1)
+    verts[:, 2] = t[..., 2].reshape(-1)
+    faces = np.arange(T * 3, dtype=np.int32).reshape(T, 3)
+    m = Mesh(verts, faces)
+    m.CalcFaceNormal()
+    m.set_normals(np.repeat(m.face_normals, 3, axis=0))
+    m.set_normal_indices(faces)
+    rng = np.random.default_rng(seed)
+    if colors is None:
+        colors = rng.random((T * 3, 3), dtype=np.float32) * 255.0
+    m.set_vertex_colors(colors)
+    m.set_uv(rng.random((T * 3, 2), dtype=np.float32))
+    m.set_uv_indices(faces)
+    return m
+
+
+def random_triangles(n, w, h, seed, grid=0.5, zs=(1.0, 2.0, 4.0)):
+    """n triangles with corners on a `grid`-pixel lattice inside [-2, w+1] x
+    [-2, h+1] and one power-of-two depth per triangle."""
+    rng = np.random.default_rng(seed)
+    tris = np.empty((n, 3, 3))
+    for i in range(n):
+        cx, cy = rng.uniform(-2, w + 1), rng.uniform(-2, h + 1)
+        size = rng.choice([0.7, 2.0, 5.0, max(w, h) * 1.5])
+        for k in range(3):
+            tris[i, k, 0] = np.round((cx + rng.uniform(-size, size)) / grid) * grid
+            tris[i, k, 1] = np.round((cy + rng.uniform(-size, size)) / grid) * grid
+        tris[i, :, 2] = rng.choice(zs)
+    return tris
+
+
+def add_texture(mesh, size, seed):
+    rng = np.random.default_rng(seed)
+    tex = (rng.random((size, size, 3)) * 255).astype(np.uint8)
+    mesh.set_single_material(tex)
+    return mesh
+
+
+def subnormal_depth_scene(w=16, h=16, flip=False):
+    """Camera f = 2^-10 and one vertex at the subnormal depth 2^-135: it still
+    projects to an exact finite pixel, but its perspective weight overflows
+    (q = a / z = inf), so the pixel depth 1/(q0+q1+q2) is 0 < FLT_MIN -- the
+    z-buffer's empty sentinel in the reference. Later faces always overwrite
+    such a pixel; the packed atomicMin alone cannot express that order."""
+    fx = 2.0 ** -10
+    zsub = 2.0 ** -135
+
+    def vert(u, v, z):
+        return (u * z / fx, v * z / fx, z)
+
+    tris = [
+        [vert(0, 0, 1.0), vert(15, 0, 1.0), vert(0, 15, 1.0)],        # 0 big, z=1
+        [vert(2, 2, zsub), vert(12, 3, 2.0), vert(5, 12, 2.0)],      # 1 subnormal corner
+        [vert(1, 1, 4.0), vert(14, 2, 4.0), vert(3, 14, 4.0)],        # 2 behind 0, after 1
+        [vert(3, 3, zsub), vert(9, 4, 0.5), vert(4, 10, 0.5)],       # 3 subnormal again
+        [vert(6, 0, 0.25), vert(15, 8, 0.25), vert(8, 15, 0.25)],    # 4 nearest
+    ]
+    t = np.asarray(tris, np.float64)
+    if flip:  # reversed winding: normals face the camera (front faces)
+        t = t[:, ::-1, :]
+    verts = t.reshape(-1, 3).astype(np.float32)
+    faces = np.arange(len(verts), dtype=np.int32).reshape(-1, 3)
+    m = Mesh(verts, faces)
+    m.CalcFaceNormal()
+    m.set_normals(np.repeat(m.face_normals, 3, axis=0))
+    m.set_normal_indices(faces)
+    rng = np.random.default_rng(3)
+    m.set_vertex_colors(rng.random((len(verts), 3), dtype=np.float32) * 255.0)
+    cam = PinholeCamera(w, h, np.eye(4), (0.0, 0.0), (fx, fx))
+    return m, cam
