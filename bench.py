@@ -17,6 +17,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -90,6 +91,100 @@ def cpu_baseline(mesh, cams, opt, budget_s):
     return done, t_total, O.num_threads()
 
 
+def diff_benchmarks(torch, np, scenes, mesh, cams, args):
+    """Forward+backward legs of the differentiable renderer (float colours and
+    float texture as leaf tensors, L2 image loss)."""
+    from ugu_amd.diff import DiffRenderer
+    out = {}
+    dev = torch.device("cuda")
+
+    def fit_step(dr, cam_list, v, c, t, targets, opt=None):
+        imgs = dr.render(cam_list, v, c, t)
+        loss = sum(((im - tg) ** 2).sum() for im, tg in zip(imgs, targets))
+        if opt is not None:
+            opt.zero_grad(set_to_none=True)
+        loss.backward()
+        if opt is not None:
+            opt.step()
+        return loss
+
+    # -- 100k-face perturbed icosphere (geodesic frequency 71), one 512x512 view
+    log("diff: 100k faces, 512x512, fwd+bwd")
+    gv, gf = scenes.geodesic_unit(71)
+    m4 = scenes.perturbed_sphere(gv, gf, texture=None)
+    rng = np.random.default_rng(4)
+    cols = torch.tensor(rng.random((len(m4.vertices), 3), dtype=np.float32), device=dev)
+    tex = torch.tensor(scenes.checker_noise_texture(1024, 5, dtype=np.float32), device=dev)
+    cam4 = scenes.ring_cameras(1, 3.0, 11, 12, res_choices=[(512, 512)])[0].to_abi()
+    dr4 = DiffRenderer(m4)
+    v4 = torch.tensor(m4.vertices, device=dev)
+    with torch.no_grad():
+        noise = torch.tensor(rng.normal(0, 0.1, cols.shape).astype(np.float32), device=dev)
+        tgt4 = dr4.render([cam4], v4, cols + noise, tex)
+    pv, pc, pt = (x.clone().requires_grad_(True) for x in (v4, cols, tex))
+    t4 = time_calls(torch, lambda: fit_step(dr4, [cam4], pv, pc, pt, tgt4), args.steps, 3)
+    out["diff_100k_512"] = {"iters_per_s": 1.0 / t4, "ms": t4 * 1e3, "faces": len(m4.vertex_indices)}
+    del dr4
+
+    # -- 1.31M-face mesh, the 64 ring views: batched vs per-view loop
+    log("diff: 64 views fwd+bwd, batched vs per-view")
+    dr = DiffRenderer(mesh)
+    vv = torch.tensor(mesh.vertices, device=dev)
+    cc = torch.tensor(np.random.default_rng(6).random((len(mesh.vertices), 3), dtype=np.float32),
+                      device=dev)
+    tt = torch.tensor(scenes.checker_noise_texture(2048, 7, dtype=np.float32), device=dev)
+    abi = [c.to_abi() for c in cams]
+    with torch.no_grad():
+        tg = dr.render(abi, vv, cc * 0.9, tt)
+    pv, pc, pt = (x.clone().requires_grad_(True) for x in (vv, cc, tt))
+    tb = time_calls(torch, lambda: fit_step(dr, abi, pv, pc, pt, tg), max(3, args.steps // 2), 2)
+
+    def per_view():
+        for a, g in zip(abi, tg):
+            fit_step(dr, [a], pv, pc, pt, [g])
+    tl = time_calls(torch, per_view, 3, 1)
+    out["diff_64view"] = {"batched_iters_per_s": 1.0 / tb, "per_view_iters_per_s": 1.0 / tl,
+                          "batched_vs_loop": tl / tb, "batched_ms": tb * 1e3, "loop_ms": tl * 1e3}
+
+    # -- inverse texturing: 32 views (16 at 512x512, 16 at 1024x768), GT
+    #    2048x2048 float texture, learned texture from 0.5 grey, Adam lr 1e-2
+    log("diff: inverse texturing, 32 views, 200 Adam iterations")
+    vc = scenes.ring_cameras(32, 3.0, 41, 42, res_choices=[(512, 512)])
+    for k in range(16, 32):
+        vc[k].set_size(1024, 768)
+        vc[k].principal_point[:] = (511.5, 383.5)
+        vc[k].set_fov_y(45.0)
+    a5 = [c.to_abi() for c in vc]
+    gt_tex = torch.tensor(scenes.checker_noise_texture(2048, 8, dtype=np.float32), device=dev)
+    with torch.no_grad():
+        tg5 = dr.render(a5, vv, None, gt_tex)
+    learn = torch.full_like(gt_tex, 0.5).requires_grad_(True)
+    opt = torch.optim.Adam([learn], lr=1e-2)
+    seen = None
+    n_iter = 200
+    for it in range(n_iter + 3):
+        if it == 3:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+        fit_step(dr, a5, vv, None, learn, tg5, opt)
+        if seen is None:
+            # texels that receive gradient from at least one view
+            seen = (learn.grad.abs().sum(-1) > 0) if learn.grad is not None else None
+    torch.cuda.synchronize()
+    t5 = (time.perf_counter() - t0) / n_iter
+    with torch.no_grad():
+        if seen is None:
+            imgs = dr.render(a5, vv, None, learn)
+        m = seen if seen is not None else torch.ones(gt_tex.shape[:2], dtype=torch.bool, device=dev)
+        mse = ((learn.clamp(0, 1) - gt_tex) ** 2).mean(-1)[m].mean().item()
+    out["inverse_texturing_32view"] = {"iters_per_s": 1.0 / t5, "ms_per_iter": t5 * 1e3,
+                                       "iterations": n_iter,
+                                       "psnr_db": 10.0 * math.log10(1.0 / max(mse, 1e-12)),
+                                       "texels_seen": int(m.sum().item())}
+    del dr
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,7 +193,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--tile", default="auto", help="auto | 8x8 | 16x16 | 32x8")
+    ap.add_argument("--tile", default="auto",
+                    help="auto | 8x8 | 16x16 | 32x8, optionally with :p (persistent) or :g (grid)")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable legs")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run a 1 GiB device-to-device copy first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
@@ -112,6 +211,13 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.init()
+    if args.calib_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     log("building scenes")
     mesh, cam2 = scenes.scene_icosphere8()
     cams = scenes.multiview_cameras(64)
@@ -128,8 +234,9 @@ def main():
         log("autotuning tile size / bin capacity")
         tiling = r.autotune(cams, repeats=3)
     else:
-        tw, th = (int(x) for x in args.tile.split("x"))
-        r.set_tiling(tw, th, 1024, True)
+        shape, _, mode = args.tile.partition(":")
+        tw, th = (int(x) for x in shape.split("x"))
+        r.set_tiling(tw, th, 1024, mode != "g")
         r.render_device(abi_cams, bufs)  # sizes the bins
         tiling = r.get_tiling()
     log(f"tiling {tiling}")
@@ -191,6 +298,9 @@ def main():
                                       "mtri_per_s": len(m1.vertex_indices) / t1 / 1e6,
                                       "mpix_per_s": 640 * 480 / t1 / 1e6}
         del r1
+
+    if not args.quick and not args.no_diff:
+        extra.update(diff_benchmarks(torch, np, scenes, mesh, cams, args))
 
     cpu = None
     if not args.no_cpu:

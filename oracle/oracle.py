@@ -211,3 +211,41 @@ def ray_w(cam, x, y):
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+def _diff_args(cam, mesh, verts, colors, tex):
+    L = lib()
+    for fn in (L.oracle_diff_forward_f64, L.oracle_diff_backward_f64):
+        fn.restype = ctypes.c_int
+    f = np.ascontiguousarray(mesh.vertex_indices, np.int32)
+    uv = np.ascontiguousarray(mesh.uv, np.float32) if len(mesh.uv) else None
+    ui = np.ascontiguousarray(mesh.uv_indices, np.int32) if len(mesh.uv_indices) else None
+    v = np.ascontiguousarray(verts, np.float64)
+    c = None if colors is None else np.ascontiguousarray(colors, np.float64)
+    t = None if tex is None else np.ascontiguousarray(tex, np.float64)
+    th, tw = (0, 0) if t is None else t.shape[:2]
+    keep = (f, uv, ui, v, c, t)
+    args = [ctypes.byref(cam), _p(f), _p(uv), _p(ui), _p(v), _p(c), _p(t),
+            ctypes.c_int(tw), ctypes.c_int(th)]
+    return keep, args
+
+
+def diff_forward_f64(cam, mesh, verts, colors, tex, face_id):
+    """Float64 image (H, W, 3) of the differentiable model at fixed visibility."""
+    keep, args = _diff_args(cam, mesh, verts, colors, tex)
+    fid = np.ascontiguousarray(face_id, np.int32)
+    img = np.zeros((cam.height, cam.width, 3), np.float64)
+    lib().oracle_diff_forward_f64(*args, _p(fid), _p(img))
+    return img
+
+
+def diff_backward_f64(cam, mesh, verts, colors, tex, face_id, grad_image):
+    """Float64 gradients (verts, colors, texture) of sum(grad_image * image)."""
+    keep, args = _diff_args(cam, mesh, verts, colors, tex)
+    fid = np.ascontiguousarray(face_id, np.int32)
+    g = np.ascontiguousarray(grad_image, np.float64)
+    gv = np.zeros((len(verts), 3), np.float64)
+    gc = None if colors is None else np.zeros_like(np.asarray(colors, np.float64))
+    gt = None if tex is None else np.zeros_like(np.asarray(tex, np.float64))
+    lib().oracle_diff_backward_f64(*args, _p(fid), _p(g), _p(gv), _p(gc), _p(gt))
+    return gv, gc, gt

@@ -596,3 +596,235 @@ int oracle_num_threads() {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Differentiable rendering reference in float64 (no reference counterpart:
+// the image model and its gradients are those documented in
+// include/ugu_hip.h for ugu_hip_diff_forward / ugu_hip_diff_backward).
+// Visibility is fixed: face_id[p] comes from the forward render; a pixel is
+// shaded when face_id > 0.
+
+namespace {
+
+struct DiffCtx {
+  int W, H;
+  double fx, fy, cx, cy, R[9], t[3];
+  const int32_t* faces;
+  const float* uv;
+  const int32_t* uvidx;
+  const double* verts;
+  const double* colors;
+  const double* tex;
+  int tw, th;
+};
+
+struct PixelEval {
+  double X[3], Y[3], Z[3], Qx[3], Qy[3];
+  double e[3], A, q[3], s, b0, b1, b2;
+  double c[3], tt[3], dcdu[3], dcdv[3], dtdu[3], dtdv[3];
+  int tx0, ty0, tx1, ty1;
+  double lu, lv;
+  double uv0[2], uv1[2], uv2[2];
+};
+
+void eval_pixel(const DiffCtx& d, int fid, int px, int py, PixelEval& e) {
+  const int32_t* f = d.faces + int64_t(fid) * 3;
+  const double x = px, y = py;
+  for (int i = 0; i < 3; i++) {
+    const double* P = d.verts + int64_t(f[i]) * 3;
+    double Q[3];
+    for (int r = 0; r < 3; r++)
+      Q[r] = d.R[r * 3 + 0] * P[0] + d.R[r * 3 + 1] * P[1] + d.R[r * 3 + 2] * P[2] + d.t[r];
+    e.Qx[i] = Q[0];
+    e.Qy[i] = Q[1];
+    e.Z[i] = Q[2];
+    e.X[i] = d.fx * Q[0] / Q[2] + d.cx;
+    e.Y[i] = d.fy * Q[1] / Q[2] + d.cy;
+  }
+  auto E = [](double ax, double ay, double bx, double by, double cx_, double cy_) {
+    return (cx_ - ax) * (by - ay) - (cy_ - ay) * (bx - ax);
+  };
+  e.A = E(e.X[0], e.Y[0], e.X[1], e.Y[1], e.X[2], e.Y[2]);
+  e.e[0] = E(e.X[1], e.Y[1], e.X[2], e.Y[2], x, y);
+  e.e[1] = E(e.X[2], e.Y[2], e.X[0], e.Y[0], x, y);
+  e.e[2] = E(e.X[0], e.Y[0], e.X[1], e.Y[1], x, y);
+  for (int i = 0; i < 3; i++) e.q[i] = e.e[i] / e.A / e.Z[i];
+  e.s = e.q[0] + e.q[1] + e.q[2];
+  e.b1 = e.q[1] / e.s;
+  e.b2 = e.q[2] / e.s;
+  e.b0 = 1.0 - e.b1 - e.b2;
+  for (int k = 0; k < 3; k++) {
+    e.c[k] = 1.0; e.tt[k] = 1.0;
+    e.dcdu[k] = e.dcdv[k] = e.dtdu[k] = e.dtdv[k] = 0.0;
+  }
+  if (d.colors) {
+    const double* C0 = d.colors + int64_t(f[0]) * 3;
+    const double* C1 = d.colors + int64_t(f[1]) * 3;
+    const double* C2 = d.colors + int64_t(f[2]) * 3;
+    for (int k = 0; k < 3; k++) {
+      e.c[k] = e.b0 * C0[k] + e.b1 * C1[k] + e.b2 * C2[k];
+      e.dcdu[k] = C1[k] - C0[k];
+      e.dcdv[k] = C2[k] - C0[k];
+    }
+  }
+  if (d.tex) {
+    const int32_t* ui = d.uvidx + int64_t(fid) * 3;
+    for (int k = 0; k < 2; k++) {
+      e.uv0[k] = d.uv[int64_t(ui[0]) * 2 + k];
+      e.uv1[k] = d.uv[int64_t(ui[1]) * 2 + k];
+      e.uv2[k] = d.uv[int64_t(ui[2]) * 2 + k];
+    }
+    const double iu = e.b0 * e.uv0[0] + e.b1 * e.uv1[0] + e.b2 * e.uv2[0];
+    const double iv = e.b0 * e.uv0[1] + e.b1 * e.uv1[1] + e.b2 * e.uv2[1];
+    const double fxp = iu * (d.tw - 1), fyp = (1.0 - iv) * (d.th - 1);
+    const double flx = std::floor(fxp), fly = std::floor(fyp);
+    e.lu = fxp - flx;
+    e.lv = fyp - fly;
+    auto clampi = [](double v, int hi) {
+      if (!(v >= 0.0)) return 0;
+      if (v > hi) return hi;
+      return static_cast<int>(v);
+    };
+    e.tx0 = clampi(flx, d.tw - 1);
+    e.ty0 = clampi(fly, d.th - 1);
+    e.tx1 = clampi(flx + 1.0, d.tw - 1);
+    e.ty1 = clampi(fly + 1.0, d.th - 1);
+    const double* p00 = d.tex + (int64_t(e.ty0) * d.tw + e.tx0) * 3;
+    const double* p01 = d.tex + (int64_t(e.ty0) * d.tw + e.tx1) * 3;
+    const double* p10 = d.tex + (int64_t(e.ty1) * d.tw + e.tx0) * 3;
+    const double* p11 = d.tex + (int64_t(e.ty1) * d.tw + e.tx1) * 3;
+    const double sx = d.tw - 1, sy = -(d.th - 1.0);
+    const double dlu_du = sx * (e.uv1[0] - e.uv0[0]), dlu_dv = sx * (e.uv2[0] - e.uv0[0]);
+    const double dlv_du = sy * (e.uv1[1] - e.uv0[1]), dlv_dv = sy * (e.uv2[1] - e.uv0[1]);
+    const double lu = e.lu, lv = e.lv;
+    for (int k = 0; k < 3; k++) {
+      e.tt[k] = (1 - lu) * (1 - lv) * p00[k] + lu * (1 - lv) * p01[k] +
+                (1 - lu) * lv * p10[k] + lu * lv * p11[k];
+      const double a = (1 - lv) * (p01[k] - p00[k]) + lv * (p11[k] - p10[k]);
+      const double b = (1 - lu) * (p10[k] - p00[k]) + lu * (p11[k] - p01[k]);
+      e.dtdu[k] = a * dlu_du + b * dlv_du;
+      e.dtdv[k] = a * dlu_dv + b * dlv_dv;
+    }
+  }
+}
+
+DiffCtx make_ctx(const OracleCamera* cam, const int32_t* faces, const float* uv,
+                 const int32_t* uvidx, const double* verts, const double* colors,
+                 const double* tex, int tw, int th) {
+  DiffCtx d;
+  d.W = cam->width;
+  d.H = cam->height;
+  d.fx = cam->fx;
+  d.fy = cam->fy;
+  d.cx = cam->cx;
+  d.cy = cam->cy;
+  for (int i = 0; i < 9; i++) d.R[i] = cam->w2c_R[i];
+  for (int i = 0; i < 3; i++) d.t[i] = cam->w2c_t[i];
+  d.faces = faces;
+  d.uv = uv;
+  d.uvidx = uvidx;
+  d.verts = verts;
+  d.colors = colors;
+  d.tex = tex;
+  d.tw = tw;
+  d.th = th;
+  return d;
+}
+
+}  // namespace
+
+extern "C" {
+
+// image[p] (H x W x 3) at fixed visibility face_id[p].
+int oracle_diff_forward_f64(const OracleCamera* cam, const int32_t* faces, const float* uv,
+                            const int32_t* uvidx, const double* verts, const double* colors,
+                            const double* tex, int tw, int th, const int32_t* face_id,
+                            double* image) {
+  DiffCtx d = make_ctx(cam, faces, uv, uvidx, verts, colors, tex, tw, th);
+  for (int py = 0; py < d.H; py++)
+    for (int px = 0; px < d.W; px++) {
+      const int64_t p = int64_t(py) * d.W + px;
+      double* o = image + p * 3;
+      o[0] = o[1] = o[2] = 0.0;
+      const int fid = face_id[p];
+      if (fid <= 0) continue;
+      PixelEval e;
+      eval_pixel(d, fid, px, py, e);
+      for (int k = 0; k < 3; k++) o[k] = e.c[k] * e.tt[k];
+    }
+  return 0;
+}
+
+// Accumulates (+=) the gradients of sum(grad_image * image) at fixed visibility.
+int oracle_diff_backward_f64(const OracleCamera* cam, const int32_t* faces, const float* uv,
+                             const int32_t* uvidx, const double* verts, const double* colors,
+                             const double* tex, int tw, int th, const int32_t* face_id,
+                             const double* grad_image, double* g_verts, double* g_colors,
+                             double* g_tex) {
+  DiffCtx d = make_ctx(cam, faces, uv, uvidx, verts, colors, tex, tw, th);
+  for (int py = 0; py < d.H; py++)
+    for (int px = 0; px < d.W; px++) {
+      const int64_t p = int64_t(py) * d.W + px;
+      const int fid = face_id[p];
+      if (fid <= 0) continue;
+      PixelEval e;
+      eval_pixel(d, fid, px, py, e);
+      const double* G = grad_image + p * 3;
+      const int32_t* f = faces + int64_t(fid) * 3;
+      const double beta[3] = {e.b0, e.b1, e.b2};
+      double gu = 0, gv = 0;
+      for (int k = 0; k < 3; k++) {
+        const double gc = G[k] * e.tt[k];
+        const double gt = G[k] * e.c[k];
+        if (g_colors && colors)
+          for (int i = 0; i < 3; i++) g_colors[int64_t(f[i]) * 3 + k] += gc * beta[i];
+        if (g_tex && tex) {
+          const double w[4] = {(1 - e.lu) * (1 - e.lv), e.lu * (1 - e.lv), (1 - e.lu) * e.lv,
+                               e.lu * e.lv};
+          const int64_t idx[4] = {int64_t(e.ty0) * tw + e.tx0, int64_t(e.ty0) * tw + e.tx1,
+                                  int64_t(e.ty1) * tw + e.tx0, int64_t(e.ty1) * tw + e.tx1};
+          for (int j = 0; j < 4; j++) g_tex[idx[j] * 3 + k] += gt * w[j];
+        }
+        gu += gc * e.dcdu[k] + gt * e.dtdu[k];
+        gv += gc * e.dcdv[k] + gt * e.dtdv[k];
+      }
+      if (!g_verts) continue;
+      // b1 = q1/s, b2 = q2/s; dL/dq_j = (dL/db_j - (gu b1 + gv b2)) / s
+      const double S = gu * e.b1 + gv * e.b2;
+      const double gq[3] = {-S / e.s, (gu - S) / e.s, (gv - S) / e.s};
+      double gX[3] = {0, 0, 0}, gY[3] = {0, 0, 0}, gZ[3], gA = 0;
+      double ge[3];
+      for (int j = 0; j < 3; j++) {
+        ge[j] = gq[j] / (e.A * e.Z[j]);
+        gZ[j] = -gq[j] * e.q[j] / e.Z[j];
+        gA -= gq[j] * e.q[j] / e.A;
+      }
+      const double x = px, y = py;
+      const int ea[3] = {1, 2, 0}, eb[3] = {2, 0, 1};
+      for (int j = 0; j < 3; j++) {
+        const int ia = ea[j], ib = eb[j];
+        gX[ia] += ge[j] * (y - e.Y[ib]);
+        gY[ia] += ge[j] * (e.X[ib] - x);
+        gX[ib] += ge[j] * (e.Y[ia] - y);
+        gY[ib] += ge[j] * (x - e.X[ia]);
+      }
+      gX[0] += gA * (e.Y[2] - e.Y[1]);
+      gY[0] += gA * (e.X[1] - e.X[2]);
+      gX[1] += gA * (e.Y[0] - e.Y[2]);
+      gY[1] += gA * (e.X[2] - e.X[0]);
+      gX[2] += gA * (e.Y[1] - e.Y[0]);
+      gY[2] += gA * (e.X[0] - e.X[1]);
+      for (int i = 0; i < 3; i++) {
+        const double iz = 1.0 / e.Z[i];
+        const double gQ[3] = {gX[i] * d.fx * iz, gY[i] * d.fy * iz,
+                              gZ[i] - gX[i] * d.fx * e.Qx[i] * iz * iz -
+                                  gY[i] * d.fy * e.Qy[i] * iz * iz};
+        for (int c = 0; c < 3; c++)
+          g_verts[int64_t(f[i]) * 3 + c] +=
+              d.R[0 * 3 + c] * gQ[0] + d.R[1 * 3 + c] * gQ[1] + d.R[2 * 3 + c] * gQ[2];
+      }
+    }
+  return 0;
+}
+
+}  // extern "C"

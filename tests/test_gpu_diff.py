@@ -1,0 +1,201 @@
+"""GPU: differentiable rendering against the float64 CPU reference and
+central finite differences.
+
+Scene: perturbed geodesic icosphere (frequency 71, 100,820 faces), 512x512
+camera (fov 45 deg, distance 3), vertex colours and a 1024x1024 float texture
+as leaf tensors; loss = squared error against a target rendered with seeded
+N(0, 0.1) colour noise, over interior pixels (the pixel and its 4 neighbours
+are covered). Tolerances: forward within 1e-5 relative of float64; gradients
+within 1e-4 relative of float64 and of central differences (h = 1e-3) on 256
+seeded parameters."""
+import numpy as np
+import pytest
+
+import oracle as O
+from ugu_amd import RendererCpuOption, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c4():
+    import torch
+    from ugu_amd.diff import DiffRenderer
+    v, f = scenes.geodesic_unit(71)
+    mesh = scenes.perturbed_sphere(v, f, texture=None)
+    rng = np.random.default_rng(4)
+    colors = rng.random((len(mesh.vertices), 3), dtype=np.float32)
+    tex = scenes.checker_noise_texture(1024, 5, dtype=np.float32, block=64)
+    cam = scenes.ring_cameras(1, 3.0, 11, 12, res_choices=[(512, 512)])[0]
+    dr = DiffRenderer(mesh)
+    dev = torch.device("cuda")
+    P = dict(v=torch.tensor(mesh.vertices, device=dev), c=torch.tensor(colors, device=dev),
+             t=torch.tensor(tex, device=dev))
+    return mesh, cam, dr, P, colors, tex
+
+
+def _interior(fid):
+    ok = fid > 0
+    inner = ok.copy()
+    inner[1:, :] &= ok[:-1, :]
+    inner[:-1, :] &= ok[1:, :]
+    inner[:, 1:] &= ok[:, :-1]
+    inner[:, :-1] &= ok[:, 1:]
+    inner[0, :] = inner[-1, :] = False
+    inner[:, 0] = inner[:, -1] = False
+    return inner
+
+
+def test_visibility_matches_reference_face_id(gpu_available, c4):
+    mesh, cam, dr, P, colors, tex = c4
+    imgs, fids = dr.render([cam], P["v"], P["c"], P["t"], return_face_ids=True)
+    rc, ref, _ = O.render(mesh, O.camera_from_abi(cam.to_abi()), RendererCpuOption(),
+                          c2w_t=cam.c2w[:3, 3])
+    assert rc == 0
+    fid = fids[0].cpu().numpy()
+    print("[diff] covered", int((fid >= 0).sum()), flush=True)
+    assert np.array_equal(fid, ref["face_id"])
+
+
+def test_forward_matches_float64(gpu_available, c4):
+    mesh, cam, dr, P, colors, tex = c4
+    oc = O.camera_from_abi(cam.to_abi())
+    for use_c, use_t in ((True, True), (True, False), (False, True)):
+        imgs, fids = dr.render([cam], P["v"], P["c"] if use_c else None,
+                               P["t"] if use_t else None, return_face_ids=True)
+        fid = fids[0].cpu().numpy()
+        ref = O.diff_forward_f64(oc, mesh, mesh.vertices, colors if use_c else None,
+                                 tex if use_t else None, fid)
+        got = imgs[0].cpu().numpy().astype(np.float64)
+        err = np.abs(got - ref).max()
+        print(f"[diff] forward colours={use_c} texture={use_t}: max abs err {err:.3g}", flush=True)
+        assert err <= 1e-5 * max(1.0, np.abs(ref).max()), f"forward err {err}"
+
+
+def _loss_setup(c4, use_tex=True):
+    import torch
+    mesh, cam, dr, P, colors, tex = c4
+    rng = np.random.default_rng(7)
+    target_c = P["c"] + torch.tensor(rng.normal(0, 0.1, colors.shape).astype(np.float32),
+                                     device=P["c"].device)
+    with torch.no_grad():
+        target, fids = dr.render([cam], P["v"], target_c, P["t"] if use_tex else None,
+                                 return_face_ids=True)
+    fid = fids[0].cpu().numpy()
+    mask = torch.tensor(_interior(fid), device=P["v"].device)[..., None].float()
+    return target[0], mask, fid
+
+
+def test_backward_matches_float64_and_finite_differences(gpu_available, c4):
+    import torch
+    mesh, cam, dr, P, colors, tex = c4
+    oc = O.camera_from_abi(cam.to_abi())
+    for use_tex in (True, False):
+        target, mask, fid = _loss_setup(c4, use_tex)
+        v = P["v"].clone().requires_grad_(True)
+        c = P["c"].clone().requires_grad_(True)
+        t = P["t"].clone().requires_grad_(True) if use_tex else None
+        img = dr.render([cam], v, c, t)[0]
+        loss = (((img - target) ** 2) * mask).sum()
+        loss.backward()
+        # float64 reference gradient with the same dL/dimage
+        G = (2.0 * (img.detach() - target) * mask).cpu().numpy().astype(np.float64)
+        texd = tex.astype(np.float64) if use_tex else None
+        gv, gc, gt = O.diff_backward_f64(oc, mesh, mesh.vertices, colors, texd, fid, G)
+        pairs = [("verts", v.grad, gv), ("colors", c.grad, gc)]
+        if use_tex:
+            pairs.append(("texture", t.grad, gt))
+        for name, g_gpu, g_ref in pairs:
+            g_gpu = g_gpu.cpu().numpy().astype(np.float64)
+            rel = np.linalg.norm(g_gpu - g_ref) / np.linalg.norm(g_ref)
+            print(f"[diff] tex={use_tex} {name}: |g|={np.linalg.norm(g_ref):.4g} "
+                  f"rel err vs float64 {rel:.3g}", flush=True)
+            assert rel < 1e-4, f"{name} gradient rel err {rel}"
+
+        # central differences (h = 1e-3) of the float64 fixed-visibility loss
+        Tn = target.cpu().numpy().astype(np.float64)
+        Mn = mask.cpu().numpy().astype(np.float64)
+
+        def loss64(vv, cc, tt):
+            im = O.diff_forward_f64(oc, mesh, vv, cc, tt, fid)
+            return (((im - Tn) ** 2) * Mn).sum()
+
+        rng = np.random.default_rng(256)
+        h = 1e-3
+        checks = []
+
+        def central(name, j, base, step):
+            plus = base.copy().reshape(-1)
+            minus = plus.copy()
+            plus[j] += step
+            minus[j] -= step
+            args_p = {"verts": mesh.vertices.astype(np.float64),
+                      "colors": colors.astype(np.float64), "texture": texd}
+            args_m = dict(args_p)
+            args_p[name] = plus.reshape(base.shape)
+            args_m[name] = minus.reshape(base.shape)
+            return (loss64(args_p["verts"], args_p["colors"], args_p["texture"]) -
+                    loss64(args_m["verts"], args_m["colors"], args_m["texture"])) / (2 * step)
+        # position parameters: only the smooth (colour-only) model; bilinear
+        # texture sampling has kinks at texel boundaries
+        groups = [("colors", gc, colors.astype(np.float64))]
+        if not use_tex:
+            groups.append(("verts", gv, mesh.vertices.astype(np.float64)))
+        else:
+            groups.append(("texture", gt, tex.astype(np.float64)))
+        g_gpu_all = {"verts": v.grad.cpu().numpy(), "colors": c.grad.cpu().numpy()}
+        if use_tex:
+            g_gpu_all["texture"] = t.grad.cpu().numpy()
+        n_per = 128
+        fine = []
+        for name, g_ref, base in groups:
+            flat_ref = g_ref.reshape(-1)
+            nz = np.nonzero(np.abs(flat_ref) > 1e-3 * np.abs(flat_ref).max())[0]
+            idx = rng.choice(nz, size=min(n_per, len(nz)), replace=False)
+            for j in idx:
+                if name == "verts":
+                    # the loss is rational in the positions: central differences
+                    # at h and h/2 combined by Richardson extrapolation (error
+                    # O(h^4)); h is 6% of an edge of these 100k-face triangles
+                    d1 = central(name, j, base, h)
+                    d2 = central(name, j, base, h / 2)
+                    fd = (4.0 * d2 - d1) / 3.0
+                    small = central(name, j, base, 1e-5)
+                    g = g_gpu_all[name].reshape(-1)[j]
+                    fine.append(abs(g - small) / max(abs(small), 1e-12))
+                else:
+                    # linear in colours and texels: central differences are exact
+                    fd = central(name, j, base, h)
+                checks.append((name, fd, g_gpu_all[name].reshape(-1)[j]))
+        fd = np.array([x[1] for x in checks])
+        gg = np.array([x[2] for x in checks], np.float64)
+        rel = np.abs(gg - fd) / np.maximum(np.abs(fd), 1e-12)
+        normwise = np.linalg.norm(gg - fd) / np.linalg.norm(fd)
+        print(f"[diff] tex={use_tex} FD on {len(checks)} params: norm-wise rel {normwise:.3g}, "
+              f"max rel {rel.max():.3g}, median {np.median(rel):.3g}", flush=True)
+        if fine:
+            fine = np.array(fine)
+            print(f"[diff] positions, central h=1e-5: max rel {fine.max():.3g} "
+                  f"median {np.median(fine):.3g}", flush=True)
+            assert np.median(fine) < 1e-4
+        assert len(checks) >= 200
+        assert normwise < 1e-4
+        assert np.median(rel) < 1e-4
+
+
+def test_batched_views_gradients_add_up(gpu_available, c4):
+    """Gradients of a multi-view batch equal the sum of per-view gradients."""
+    import torch
+    mesh, cam, dr, P, colors, tex = c4
+    cams = scenes.ring_cameras(4, 3.0, 21, 22, res_choices=[(256, 256), (320, 240)])
+    c = P["c"].clone().requires_grad_(True)
+    imgs = dr.render(cams, P["v"], c, P["t"])
+    sum(im.square().sum() for im in imgs).backward()
+    g_batch = c.grad.clone()
+    g_sum = torch.zeros_like(g_batch)
+    for cm in cams:
+        c2 = P["c"].clone().requires_grad_(True)
+        dr.render([cm], P["v"], c2, P["t"])[0].square().sum().backward()
+        g_sum += c2.grad
+    rel = (g_batch - g_sum).norm() / g_sum.norm()
+    assert rel < 1e-5

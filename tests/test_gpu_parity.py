@@ -218,3 +218,21 @@ def test_offscreen_and_empty(gpu_available, uv_scene):
     out = _render(mesh, c2, opt)
     assert (out["face_id"] == -1).all() and (out["depth"] == 0).all()
     assert (out["color"] == 0).all() and (out["mask"] == 0).all()
+
+
+def test_capacity_growth_without_reallocation(gpu_available, uv_scene):
+    """A capacity grown after an overflow must not replay a graph captured
+    with the old capacity (the bins buffer is already large enough, so no
+    reallocation marks the graph stale)."""
+    mesh, cam = uv_scene
+    opt = RendererCpuOption(diffuse_color=DiffuseColor.kVertex)
+    ref = _oracle(mesh, cam, opt)
+    r = Rasterizer(opt)
+    r.set_mesh(mesh)
+    assert r.PrepareMesh()
+    r.set_tiling(16, 16, 8192)
+    _compare(r.render_batch([cam])[0], ref, "cap 8192")
+    for cap in (8, 64, 8):
+        r.set_tiling(16, 16, cap)
+        for _ in range(2):
+            _compare(r.render_batch([cam])[0], ref, f"cap {cap} -> {r.get_tiling()['bin_capacity']}")

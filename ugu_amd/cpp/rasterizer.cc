@@ -1,0 +1,340 @@
+// ugu::Rasterizer on the gfx950 pipeline.
+//
+// Same interface and error behaviour as UGU's CPU rasterizer
+// (src/renderer/cpu/rasterizer.cc:35-404): set_* never fail, PrepareMesh and
+// Render* return false with the reference's LOGE text. PrepareMesh uploads
+// the mesh once (allowed by the contract "don't modify mesh after
+// PrepareMesh", renderer.h:52-53); Render reads the camera again on every call
+// (callers move the camera between renders, examples/ex02_renderer.cc:142-143),
+// reduces it to float on the host exactly as rasterizer.cc:118-119 and
+// camera.h:322 do, and renders through ugu_hip_render.
+
+#include "ugu/renderer/cpu/rasterizer.h"
+
+#include <cstdlib>
+#include <map>
+#include <utility>
+
+#include "ugu/log.h"
+#include "../../include/ugu_hip.h"
+
+namespace ugu {
+
+namespace {
+
+int device_ordinal() {
+  const char* s = std::getenv("UGU_HIP_DEVICE");
+  return s ? std::atoi(s) : 0;
+}
+
+UguOptions to_abi(const RendererCpuOption& o) {
+  UguOptions r;
+  r.diffuse_color = static_cast<int32_t>(o.diffuse_color);
+  r.interp = static_cast<int32_t>(o.interp);
+  r.shading_normal = static_cast<int32_t>(o.shading_normal);
+  r.diffuse_shading = static_cast<int32_t>(o.diffuse_shading);
+  r.depth_scale = o.depth_scale;
+  r.backface_culling = o.backface_culling ? 1 : 0;
+  r.oren_nayar_sigma = o.oren_nayar_sigma;
+  return r;
+}
+
+// rasterizer.cc:118-119 and camera.h:322: float reductions of the pose
+bool to_abi(const Camera& camera, UguCamera* c) {
+  const PinholeCamera* p = dynamic_cast<const PinholeCamera*>(&camera);
+  if (p == nullptr) return false;
+  c->width = p->width();
+  c->height = p->height();
+  c->fx = p->focal_length()[0];
+  c->fy = p->focal_length()[1];
+  c->cx = p->principal_point()[0];
+  c->cy = p->principal_point()[1];
+  const Eigen::Matrix3f w2c_R = p->w2c().rotation().cast<float>();
+  const Eigen::Vector3f w2c_t = p->w2c().translation().cast<float>();
+  const Eigen::Matrix3f c2w_R = p->c2w().matrix().block<3, 3>(0, 0).cast<float>();
+  for (int r = 0; r < 3; r++) {
+    for (int k = 0; k < 3; k++) {
+      c->w2c_R[r * 3 + k] = w2c_R(r, k);
+      c->c2w_R[r * 3 + k] = c2w_R(r, k);
+    }
+    c->w2c_t[r] = w2c_t[r];
+  }
+  return true;
+}
+
+// ValidateAndInitBeforeRender checks (src/renderer/cpu/util_private.cc:19-66)
+bool Validate(bool mesh_initialized, const std::shared_ptr<const Camera>& camera,
+              const std::shared_ptr<const Mesh>& mesh, const RendererCpuOption& option,
+              bool any_output) {
+  if (camera == nullptr) {
+    LOGE("camera has not been set\n");
+    return false;
+  }
+  if (!mesh_initialized) {
+    LOGE("mesh has not been initialized\n");
+    return false;
+  }
+  if (option.backface_culling && mesh->face_normals().empty()) {
+    LOGE("specified back-face culling but face normal is empty.\n");
+    return false;
+  }
+  if (option.diffuse_color == DiffuseColor::kTexture && mesh->materials().empty()) {
+    LOGE("specified texture as diffuse color but texture is empty.\n");
+    return false;
+  }
+  if (option.diffuse_color == DiffuseColor::kTexture) {
+    for (int i = 0; i < static_cast<int>(mesh->materials().size()); i++) {
+      if (mesh->materials()[i].diffuse_tex.empty()) {
+        LOGE("specified texture as diffuse color but %d th texture is empty.\n", i);
+        return false;
+      }
+    }
+  }
+  if (option.diffuse_color == DiffuseColor::kVertex && mesh->vertex_colors().empty()) {
+    LOGE("specified vertex color as diffuse color but vertex color is empty.\n");
+    return false;
+  }
+  if (option.shading_normal == ShadingNormal::kFace && mesh->face_normals().empty()) {
+    LOGE("specified face normal as shading normal but face normal is empty.\n");
+    return false;
+  }
+  if (option.shading_normal == ShadingNormal::kVertex && mesh->normals().empty()) {
+    LOGE("specified vertex normal as shading normal but vertex normal is empty.\n");
+    return false;
+  }
+  if (!any_output) {
+    LOGE("all arguments are nullptr. nothing to do\n");
+    return false;
+  }
+  return true;
+}
+
+template <typename V>
+const void* data_or_null(const std::vector<V>& v) {
+  return v.empty() ? nullptr : static_cast<const void*>(v.data());
+}
+
+}  // namespace
+
+class Rasterizer::Impl {
+  bool mesh_initialized_{false};
+  std::shared_ptr<const Camera> camera_{nullptr};
+  std::shared_ptr<const Mesh> mesh_{nullptr};
+  RendererCpuOption option_;
+  ugu_hip_ctx* ctx_{nullptr};
+  mutable std::map<std::pair<int, int>, bool> tuned_;
+
+ public:
+  Impl() { ctx_ = ugu_hip_create(device_ordinal()); }
+  explicit Impl(const RendererCpuOption& option) : Impl() { set_option(option); }
+  ~Impl() { ugu_hip_destroy(ctx_); }
+
+  void set_option(const RendererCpuOption& option) { option.CopyTo(&option_); }
+
+  void set_mesh(std::shared_ptr<const Mesh> mesh) {
+    mesh_initialized_ = false;
+    mesh_ = mesh;
+    if (mesh_->face_normals().empty())
+      LOGW("face normal is empty. culling and shading may not work\n");
+    if (mesh_->normals().empty()) LOGW("vertex normal is empty. shading may not work\n");
+  }
+
+  bool PrepareMesh() {
+    if (mesh_ == nullptr) {
+      LOGE("mesh has not been set\n");
+      return false;
+    }
+    if (ctx_ == nullptr) {
+      LOGE("no HIP device available\n");
+      return false;
+    }
+    const Mesh& m = *mesh_;
+    std::vector<UguTexture> tex(m.materials().size());
+    for (size_t i = 0; i < tex.size(); i++) {
+      const Image3b& t = m.materials()[i].diffuse_tex;
+      tex[i].width = t.empty() ? 0 : t.cols;
+      tex[i].height = t.empty() ? 0 : t.rows;
+      tex[i].data = t.empty() ? nullptr : t.data;
+    }
+    UguMeshDesc d;
+    d.num_vertices = static_cast<int32_t>(m.vertices().size());
+    d.vertices = static_cast<const float*>(data_or_null(m.vertices()));
+    d.num_faces = static_cast<int32_t>(m.vertex_indices().size());
+    d.faces = static_cast<const int32_t*>(data_or_null(m.vertex_indices()));
+    d.face_normals = m.face_normals().size() >= m.vertex_indices().size()
+                         ? static_cast<const float*>(data_or_null(m.face_normals()))
+                         : nullptr;
+    d.num_normals = static_cast<int32_t>(m.normals().size());
+    d.normals = static_cast<const float*>(data_or_null(m.normals()));
+    d.normal_indices = m.normal_indices().size() >= m.vertex_indices().size()
+                           ? static_cast<const int32_t*>(data_or_null(m.normal_indices()))
+                           : nullptr;
+    d.num_vertex_colors = static_cast<int32_t>(m.vertex_colors().size());
+    d.vertex_colors = static_cast<const float*>(data_or_null(m.vertex_colors()));
+    d.num_uv = static_cast<int32_t>(m.uv().size());
+    d.uv = static_cast<const float*>(data_or_null(m.uv()));
+    d.uv_indices = m.uv_indices().size() >= m.vertex_indices().size()
+                       ? static_cast<const int32_t*>(data_or_null(m.uv_indices()))
+                       : nullptr;
+    d.material_ids = m.material_ids().size() >= m.vertex_indices().size()
+                         ? static_cast<const int32_t*>(data_or_null(m.material_ids()))
+                         : nullptr;
+    d.num_materials = static_cast<int32_t>(tex.size());
+    d.textures = tex.empty() ? nullptr : tex.data();
+    if (ugu_hip_prepare_mesh(ctx_, &d) != UGU_HIP_OK) {
+      LOGE("%s", ugu_hip_last_error(ctx_));
+      return false;
+    }
+    tuned_.clear();
+    mesh_initialized_ = true;
+    return true;
+  }
+
+  void set_camera(std::shared_ptr<const Camera> camera) { camera_ = camera; }
+
+  bool RenderViews(const std::vector<const Camera*>& cams, Image3b* const* color,
+                   Image1f* const* depth, Image3f* const* normal, Image1b* const* mask,
+                   Image1i* const* face_id) const {
+    const size_t n = cams.size();
+    std::vector<UguCamera> abi(n);
+    std::vector<UguOutputs> outs(n);
+    for (size_t i = 0; i < n; i++) {
+      if (!to_abi(*cams[i], &abi[i])) {
+        LOGE("only PinholeCamera is supported by the GPU rasterizer\n");
+        return false;
+      }
+    }
+    // ValidateAndInitBeforeRender's output initialisation (util_private.cc:68-87)
+    for (size_t i = 0; i < n; i++) {
+      const int w = cams[i]->width(), h = cams[i]->height();
+      if (color[i]) Init(color[i], w, h, static_cast<unsigned char>(0));
+      if (depth[i]) Init(depth[i], w, h, 0.0f);
+      if (normal[i]) Init(normal[i], w, h, 0.0f);
+      if (mask[i]) Init(mask[i], w, h, static_cast<unsigned char>(0));
+      if (face_id[i]) Init(face_id[i], w, h, -1);
+      outs[i].color = color[i] ? color[i]->data : nullptr;
+      outs[i].depth = depth[i] ? reinterpret_cast<float*>(depth[i]->data) : nullptr;
+      outs[i].normal = normal[i] ? reinterpret_cast<float*>(normal[i]->data) : nullptr;
+      outs[i].mask = mask[i] ? mask[i]->data : nullptr;
+      outs[i].face_id = face_id[i] ? reinterpret_cast<int32_t*>(face_id[i]->data) : nullptr;
+    }
+    const UguOptions opt = to_abi(option_);
+    // tile size and bin capacity are tuned once per view set (results do
+    // not depend on them)
+    const auto key = std::make_pair(abi[0].width * 65536 + abi[0].height, static_cast<int>(n));
+    if (!tuned_.count(key)) {
+      UguTiling t;
+      if (ugu_hip_autotune(ctx_, abi.data(), static_cast<int>(n), &opt, 2, &t) == UGU_HIP_OK)
+        tuned_[key] = true;
+    }
+    const int rc = n == 1 ? ugu_hip_render(ctx_, abi.data(), &opt, outs.data(), UGU_HIP_MEMORY_HOST)
+                          : ugu_hip_render_batch(ctx_, abi.data(), static_cast<int>(n), &opt,
+                                                 outs.data(), UGU_HIP_MEMORY_HOST);
+    if (rc != UGU_HIP_OK) {
+      LOGE("%s", ugu_hip_last_error(ctx_));
+      return false;
+    }
+    return true;
+  }
+
+  bool Render(Image3b* color, Image1f* depth, Image3f* normal, Image1b* mask,
+              Image1i* face_id) const {
+    if (!Validate(mesh_initialized_, camera_, mesh_, option_,
+                  color || depth || normal || mask || face_id))
+      return false;
+    std::vector<const Camera*> cams{camera_.get()};
+    return RenderViews(cams, &color, &depth, &normal, &mask, &face_id);
+  }
+
+  bool RenderBatch(const std::vector<std::shared_ptr<const Camera>>& cameras,
+                   std::vector<Image3b>* color, std::vector<Image1f>* depth,
+                   std::vector<Image3f>* normal, std::vector<Image1b>* mask,
+                   std::vector<Image1i>* face_id) const {
+    if (cameras.empty()) {
+      LOGE("camera has not been set\n");
+      return false;
+    }
+    for (const auto& c : cameras)
+      if (!Validate(mesh_initialized_, c, mesh_, option_,
+                    color || depth || normal || mask || face_id))
+        return false;
+    const size_t n = cameras.size();
+    if (color) color->resize(n);
+    if (depth) depth->resize(n);
+    if (normal) normal->resize(n);
+    if (mask) mask->resize(n);
+    if (face_id) face_id->resize(n);
+    std::vector<const Camera*> cams(n);
+    std::vector<Image3b*> pc(n, nullptr);
+    std::vector<Image1f*> pd(n, nullptr);
+    std::vector<Image3f*> pn(n, nullptr);
+    std::vector<Image1b*> pm(n, nullptr);
+    std::vector<Image1i*> pf(n, nullptr);
+    for (size_t i = 0; i < n; i++) {
+      cams[i] = cameras[i].get();
+      if (color) pc[i] = &(*color)[i];
+      if (depth) pd[i] = &(*depth)[i];
+      if (normal) pn[i] = &(*normal)[i];
+      if (mask) pm[i] = &(*mask)[i];
+      if (face_id) pf[i] = &(*face_id)[i];
+    }
+    return RenderViews(cams, pc.data(), pd.data(), pn.data(), pm.data(), pf.data());
+  }
+
+  bool RenderW(Image3b* color, Image1w* depth, Image3f* normal, Image1b* mask,
+               Image1i* face_id) const {
+    if (depth == nullptr) {
+      LOGE("depth is nullptr");
+      return false;
+    }
+    Image1f f_depth;
+    bool org_ret = Render(color, &f_depth, normal, mask, face_id);
+    if (org_ret) ConvertTo(f_depth, depth);
+    return org_ret;
+  }
+};
+
+Rasterizer::Rasterizer() : pimpl_(std::unique_ptr<Impl>(new Impl)) {}
+Rasterizer::~Rasterizer() {}
+Rasterizer::Rasterizer(const RendererCpuOption& option)
+    : pimpl_(std::unique_ptr<Impl>(new Impl(option))) {}
+void Rasterizer::set_option(const RendererCpuOption& option) { pimpl_->set_option(option); }
+void Rasterizer::set_mesh(std::shared_ptr<const Mesh> mesh) { pimpl_->set_mesh(mesh); }
+bool Rasterizer::PrepareMesh() { return pimpl_->PrepareMesh(); }
+void Rasterizer::set_camera(std::shared_ptr<const Camera> camera) { pimpl_->set_camera(camera); }
+bool Rasterizer::Render(Image3b* color, Image1f* depth, Image3f* normal, Image1b* mask,
+                        Image1i* face_id) const {
+  return pimpl_->Render(color, depth, normal, mask, face_id);
+}
+bool Rasterizer::RenderColor(Image3b* color) const {
+  return pimpl_->Render(color, nullptr, nullptr, nullptr, nullptr);
+}
+bool Rasterizer::RenderDepth(Image1f* depth) const {
+  return pimpl_->Render(nullptr, depth, nullptr, nullptr, nullptr);
+}
+bool Rasterizer::RenderNormal(Image3f* normal) const {
+  return pimpl_->Render(nullptr, nullptr, normal, nullptr, nullptr);
+}
+bool Rasterizer::RenderMask(Image1b* mask) const {
+  return pimpl_->Render(nullptr, nullptr, nullptr, mask, nullptr);
+}
+bool Rasterizer::RenderFaceId(Image1i* face_id) const {
+  return pimpl_->Render(nullptr, nullptr, nullptr, nullptr, face_id);
+}
+bool Rasterizer::RenderW(Image3b* color, Image1w* depth, Image3f* normal, Image1b* mask,
+                         Image1i* face_id) const {
+  return pimpl_->RenderW(color, depth, normal, mask, face_id);
+}
+bool Rasterizer::RenderDepthW(Image1w* depth) const {
+  return pimpl_->RenderW(nullptr, depth, nullptr, nullptr, nullptr);
+}
+#ifdef UGU_RASTERIZER_HAS_RENDER_BATCH
+bool Rasterizer::RenderBatch(const std::vector<std::shared_ptr<const Camera>>& cameras,
+                             std::vector<Image3b>* color, std::vector<Image1f>* depth,
+                             std::vector<Image3f>* normal, std::vector<Image1b>* mask,
+                             std::vector<Image1i>* face_id) const {
+  return pimpl_->RenderBatch(cameras, color, depth, normal, mask, face_id);
+}
+#endif
+
+}  // namespace ugu

@@ -1,0 +1,195 @@
+// Test driver of the C++ drop-in API (built by ugu_amd/cpp/Makefile, run by
+// tests/test_gpu_cpp_api.py). It is used like UGU's own example
+// (examples/ex02_renderer.cc:212-281): build a ugu::Mesh, a PinholeCamera
+// from an Affine3d pose, Rasterizer(option) -> set_mesh -> PrepareMesh ->
+// set_camera -> Render, then writes the outputs and the float camera it
+// rendered with as raw files for the Python side to compare with the oracle.
+//
+// usage: test_rasterizer_api <scene_dir> <out_dir>
+// scene_dir: meta.txt (V F N U tw th ncam diffuse_color interp shading_normal
+//            diffuse_shading culling), verts.f32, faces.i32, fnormals.f32,
+//            normals.f32, colors.f32, uv.f32, tex.u8, cams.f64 (ncam x
+//            (W H fx fy cx cy c2w[16 row-major]))
+
+#include <cstdio>
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ugu/log.h"
+#include "ugu/renderer/cpu/rasterizer.h"
+
+namespace {
+
+template <typename T>
+std::vector<T> read_all(const std::string& path, size_t n) {
+  std::vector<T> v(n);
+  std::ifstream f(path, std::ios::binary);
+  f.read(reinterpret_cast<char*>(v.data()), std::streamsize(n * sizeof(T)));
+  return v;
+}
+
+template <typename T>
+void write_img(const std::string& path, const ugu::Image<T>& img) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(img.data), std::streamsize(img.total() * sizeof(T)));
+}
+
+int g_fail = 0;
+void check(bool ok, const char* name) {
+  std::printf("CHECK %s %s\n", name, ok ? "ok" : "FAILED");
+  if (!ok) g_fail++;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 3) return 2;
+  const std::string in = argv[1], out = argv[2];
+  int V, F, N, U, tw, th, ncam, dc, interp, sn, ds, cull;
+  {
+    std::ifstream m(in + "/meta.txt");
+    m >> V >> F >> N >> U >> tw >> th >> ncam >> dc >> interp >> sn >> ds >> cull;
+    if (!m) return 3;
+  }
+  auto vf = read_all<float>(in + "/verts.f32", size_t(V) * 3);
+  auto fi = read_all<int32_t>(in + "/faces.i32", size_t(F) * 3);
+  auto fn = read_all<float>(in + "/fnormals.f32", size_t(F) * 3);
+  auto nf = read_all<float>(in + "/normals.f32", size_t(N) * 3);
+  auto cf = read_all<float>(in + "/colors.f32", size_t(V) * 3);
+  auto uf = read_all<float>(in + "/uv.f32", size_t(U) * 2);
+  auto tex = read_all<uint8_t>(in + "/tex.u8", size_t(tw) * th * 3);
+  auto cams = read_all<double>(in + "/cams.f64", size_t(ncam) * 22);
+
+  std::vector<Eigen::Vector3f> verts(V), fnormals(F), normals(N), colors(V);
+  std::vector<Eigen::Vector3i> faces(F);
+  std::vector<Eigen::Vector2f> uv(U);
+  for (int i = 0; i < V; i++) {
+    verts[i] = Eigen::Vector3f(vf[i * 3], vf[i * 3 + 1], vf[i * 3 + 2]);
+    colors[i] = Eigen::Vector3f(cf[i * 3], cf[i * 3 + 1], cf[i * 3 + 2]);
+  }
+  for (int i = 0; i < F; i++) {
+    faces[i] = Eigen::Vector3i(fi[i * 3], fi[i * 3 + 1], fi[i * 3 + 2]);
+    fnormals[i] = Eigen::Vector3f(fn[i * 3], fn[i * 3 + 1], fn[i * 3 + 2]);
+  }
+  for (int i = 0; i < N; i++) normals[i] = Eigen::Vector3f(nf[i * 3], nf[i * 3 + 1], nf[i * 3 + 2]);
+  for (int i = 0; i < U; i++) uv[i] = Eigen::Vector2f(uf[i * 2], uf[i * 2 + 1]);
+
+  auto mesh = ugu::Mesh::Create();
+  mesh->set_vertices(verts);
+  mesh->set_vertex_indices(faces);
+  mesh->set_face_normals(fnormals);
+  mesh->set_normals(normals);
+  mesh->set_normal_indices(faces);
+  mesh->set_vertex_colors(colors);
+  mesh->set_uv(uv);
+  mesh->set_uv_indices(faces);
+  ugu::ObjMaterial mat;
+  mat.diffuse_tex = ugu::Image3b::zeros(th, tw);
+  std::copy(tex.begin(), tex.end(), mat.diffuse_tex.data);
+  mesh->set_single_material(mat);
+
+  ugu::RendererCpuOption option;
+  option.diffuse_color = static_cast<ugu::DiffuseColor>(dc);
+  option.interp = static_cast<ugu::ColorInterpolation>(interp);
+  option.shading_normal = static_cast<ugu::ShadingNormal>(sn);
+  option.diffuse_shading = static_cast<ugu::DiffuseShading>(ds);
+  option.backface_culling = cull != 0;
+
+  std::vector<std::shared_ptr<ugu::PinholeCamera>> cameras;
+  for (int c = 0; c < ncam; c++) {
+    const double* p = &cams[size_t(c) * 22];
+    Eigen::Affine3d c2w = Eigen::Affine3d::Identity();
+    for (int r = 0; r < 4; r++)
+      for (int k = 0; k < 4; k++) c2w.matrix()(r, k) = p[6 + r * 4 + k];
+    cameras.push_back(std::make_shared<ugu::PinholeCamera>(
+        int(p[0]), int(p[1]), c2w, Eigen::Vector2f(float(p[4]), float(p[5])),
+        Eigen::Vector2f(float(p[2]), float(p[3]))));
+  }
+
+  // ---- error conventions (util_private.cc:19-66, rasterizer.cc:91-94,331-334)
+  ugu::set_log_level(ugu::LogLevel::kNone);
+  {
+    ugu::Rasterizer r(option);
+    ugu::Image1i fid;
+    check(!r.PrepareMesh(), "prepare_without_mesh_fails");
+    r.set_mesh(mesh);
+    r.set_camera(cameras[0]);
+    check(!r.Render(nullptr, nullptr, nullptr, nullptr, &fid), "render_before_prepare_fails");
+    check(r.PrepareMesh(), "prepare_ok");
+    check(!r.Render(nullptr, nullptr, nullptr, nullptr, nullptr), "all_null_fails");
+    check(!r.RenderW(nullptr, nullptr, nullptr, nullptr, nullptr), "renderw_null_depth_fails");
+    ugu::Rasterizer r2(option);
+    r2.set_mesh(mesh);
+    r2.PrepareMesh();
+    check(!r2.Render(nullptr, nullptr, nullptr, nullptr, &fid), "render_without_camera_fails");
+    auto bare = ugu::Mesh::Create();
+    bare->set_vertices(verts);
+    bare->set_vertex_indices(faces);
+    ugu::RendererCpuOption cull_opt;
+    ugu::Rasterizer r3(cull_opt);
+    r3.set_mesh(bare);
+    r3.PrepareMesh();
+    r3.set_camera(cameras[0]);
+    check(!r3.Render(nullptr, nullptr, nullptr, nullptr, &fid), "culling_without_face_normals_fails");
+    ugu::RendererCpuOption vc_opt;
+    vc_opt.diffuse_color = ugu::DiffuseColor::kVertex;
+    vc_opt.backface_culling = false;
+    vc_opt.shading_normal = ugu::ShadingNormal::kFace;
+    auto nocol = ugu::Mesh::Create();
+    nocol->set_vertices(verts);
+    nocol->set_vertex_indices(faces);
+    nocol->set_face_normals(fnormals);
+    ugu::Rasterizer r4(vc_opt);
+    r4.set_mesh(nocol);
+    r4.PrepareMesh();
+    r4.set_camera(cameras[0]);
+    check(!r4.Render(nullptr, nullptr, nullptr, nullptr, &fid), "vertex_color_without_colors_fails");
+  }
+  ugu::set_log_level(ugu::LogLevel::kWarning);
+
+  // ---- renders: per-view loop (ex02 pattern), then the batched extension
+  ugu::Rasterizer renderer(option);
+  renderer.set_mesh(mesh);
+  if (!renderer.PrepareMesh()) return 4;
+  for (int c = 0; c < ncam; c++) {
+    renderer.set_camera(cameras[c]);
+    ugu::Image3b color;
+    ugu::Image1f depth;
+    ugu::Image3f normal;
+    ugu::Image1b mask;
+    ugu::Image1i face_id;
+    ugu::Image1w depthw;
+    check(renderer.Render(&color, &depth, &normal, &mask, &face_id), "render");
+    check(renderer.RenderDepthW(&depthw), "render_depth_w");
+    const std::string pfx = out + "/view" + std::to_string(c) + "_";
+    write_img(pfx + "color.u8", color);
+    write_img(pfx + "depth.f32", depth);
+    write_img(pfx + "normal.f32", normal);
+    write_img(pfx + "mask.u8", mask);
+    write_img(pfx + "face_id.i32", face_id);
+    write_img(pfx + "depthw.u16", depthw);
+    // float camera the renderer used (rasterizer.cc:118-119, camera.h:322)
+    const Eigen::Matrix3f w2c_R = cameras[c]->w2c().rotation().cast<float>();
+    const Eigen::Vector3f w2c_t = cameras[c]->w2c().translation().cast<float>();
+    std::ofstream cf_out(pfx + "camera.f32", std::ios::binary);
+    float cam_f[12];
+    for (int r = 0; r < 3; r++) {
+      for (int k = 0; k < 3; k++) cam_f[r * 3 + k] = w2c_R(r, k);
+      cam_f[9 + r] = w2c_t[r];
+    }
+    cf_out.write(reinterpret_cast<const char*>(cam_f), sizeof(cam_f));
+  }
+  std::vector<std::shared_ptr<const ugu::Camera>> all(cameras.begin(), cameras.end());
+  std::vector<ugu::Image1i> bfid;
+  std::vector<ugu::Image3b> bcol;
+  check(renderer.RenderBatch(all, &bcol, nullptr, nullptr, nullptr, &bfid), "render_batch");
+  for (int c = 0; c < ncam; c++) {
+    const std::string pfx = out + "/batch" + std::to_string(c) + "_";
+    write_img(pfx + "face_id.i32", bfid[c]);
+    write_img(pfx + "color.u8", bcol[c]);
+  }
+  std::printf("DONE failures=%d\n", g_fail);
+  return g_fail == 0 ? 0 : 1;
+}

@@ -163,12 +163,13 @@ UGU_DEV bool backface_of(const float* fn, int64_t fid, float r0, float r1, float
 }
 
 // ImageBase::at on an RGB8 texture (image.h:416-427) with the linear index
-// clamped to the image.
-UGU_DEV const uint8_t* tex_at(const uint8_t* base, int w, int h, int row, int col) {
+// clamped to the image. Device textures are stored RGBA8 (alpha unused) so a
+// texel is one aligned 32-bit load.
+UGU_DEV uchar4 tex_at(const uint8_t* base, int w, int h, int row, int col) {
   int64_t lin = static_cast<int64_t>(row) * w + col;
   int64_t n = static_cast<int64_t>(w) * h;
   lin = lin < 0 ? 0 : (lin >= n ? n - 1 : lin);
-  return base + lin * 3;
+  return reinterpret_cast<const uchar4*>(base)[lin];
 }
 
 UGU_DEV int xcd_remap(int b, int nb) {
@@ -183,9 +184,21 @@ UGU_DEV int xcd_remap(int b, int nb) {
 // ---------------------------------------------------------------------------
 // k_clear
 
-__global__ void k_clear(int32_t* counts, int32_t n, int32_t* status) {
+// Also fills the tile -> view table (largest v with tile_base[v] <= tile),
+// so the rasterizer finds its view with one load instead of a search.
+__global__ void k_clear(int32_t* counts, int32_t n, int32_t* status,
+                        const int32_t* __restrict__ tile_base, int32_t n_views,
+                        int32_t* __restrict__ tile_view) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) counts[i] = 0;
+  if (i < n) {
+    counts[i] = 0;
+    int lo = 0, hi = n_views - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_base[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    tile_view[i] = lo;
+  }
   if (i < 4) status[i] = 0;
 }
 
@@ -264,36 +277,50 @@ UGU_DEV void wave_append(const DevBins& bins, int tile, int fid, bool has) {
   }
 }
 
+// Views per thread: the face's indices are loaded once and reused for
+// kSetupViews consecutive views (grid.y = ceil(n_views / kSetupViews)).
+constexpr int kSetupViews = 8;
+
 template <int TW, int TH>
 __global__ __launch_bounds__(256) void k_setup_bin(
-    const DevView* __restrict__ views, DevMesh mesh,
+    const DevView* __restrict__ views, int n_views, DevMesh mesh,
     const float4* __restrict__ proj, DevBins bins) {
-  const DevView& v = views[blockIdx.y];
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  bool has = false;
-  int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+  int i0 = 0, i1 = 0, i2 = 0;
   if (f < mesh.F) {
     const int32_t* fi = mesh.faces + int64_t(f) * 3;
-    const float4* pv = proj + v.proj_offset;
-    float4 a = pv[fi[0]], b = pv[fi[1]], c = pv[fi[2]];
-    int x0, x1, y0, y1;
-    float area;
-    if (face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) {
-      has = true;
-      tx0 = x0 / TW;
-      tx1 = x1 / TW;
-      ty0 = y0 / TH;
-      ty1 = y1 / TH;
-    }
+    i0 = fi[0];
+    i1 = fi[1];
+    i2 = fi[2];
   }
-  int tx = tx0, ty = ty0;
-  while (__any(has)) {
-    const int tile = has ? v.tile_base + ty * v.tiles_x + tx : -1;
-    wave_append(bins, tile, f, has);
-    if (has) {
-      if (++tx > tx1) {
-        tx = tx0;
-        if (++ty > ty1) has = false;
+  const int vb = blockIdx.y * kSetupViews;
+  const int ve = min(vb + kSetupViews, n_views);
+  for (int vi = vb; vi < ve; vi++) {
+    const DevView& v = views[vi];
+    bool has = false;
+    int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+    if (f < mesh.F) {
+      const float4* pv = proj + v.proj_offset;
+      const float4 a = pv[i0], b = pv[i1], c = pv[i2];
+      int x0, x1, y0, y1;
+      float area;
+      if (face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) {
+        has = true;
+        tx0 = x0 / TW;
+        tx1 = x1 / TW;
+        ty0 = y0 / TH;
+        ty1 = y1 / TH;
+      }
+    }
+    int tx = tx0, ty = ty0;
+    while (__any(has)) {
+      const int tile = has ? v.tile_base + ty * v.tiles_x + tx : -1;
+      wave_append(bins, tile, f, has);
+      if (has) {
+        if (++tx > tx1) {
+          tx = tx0;
+          if (++ty > ty1) has = false;
+        }
       }
     }
   }
@@ -308,6 +335,7 @@ struct RasterLds {
   float ray[3][NT];            // ray_w of each tile pixel
   float fv[13][NT];            // staged faces: 3 projected verts, area, face normal
   int32_t fid[NT];
+  int32_t vid[3][NT];          // staged face vertex indices
   int32_t box[NT];             // lx0 | ly0 << 8 | nx << 16 (tile-local pixel bbox)
   int32_t pref[NT + 1];        // exclusive prefix of pair counts
   int32_t wsum[NT / 64 + 1];
@@ -361,15 +389,14 @@ UGU_DEV void sequential_pixel(const DevView& v, const DevMesh& mesh,
 
 // Colour of one shaded pixel of the reference path: colourizer then shader
 // (pixel_shader.h:215-218 with the classes at :222-422).
-UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, float u,
-                      float vv, float r0, float r1, float r2, float n0,
+UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, int v0, int v1,
+                      int v2, float u, float vv, float r0, float r1, float r2, float n0,
                       float n1, float n2, uint8_t c[3]) {
   if (o.diffuse_color == 2) {
     // DiffuseVertexColorColorizer (pixel_shader.h:234-256)
-    const int32_t* f = m.faces + int64_t(fid) * 3;
-    const float* c0 = m.vcolors + int64_t(f[0]) * 3;
-    const float* c1 = m.vcolors + int64_t(f[1]) * 3;
-    const float* c2 = m.vcolors + int64_t(f[2]) * 3;
+    const float* c0 = m.vcolors + int64_t(v0) * 3;
+    const float* c1 = m.vcolors + int64_t(v1) * 3;
+    const float* c2 = m.vcolors + int64_t(v2) * 3;
     const float w = 1.0f - u - vv;
     for (int k = 0; k < 3; k++) c[k] = f2u8_x86(w * c0[k] + u * c1[k] + vv * c2[k]);
   } else if (o.diffuse_color == 1) {
@@ -386,10 +413,10 @@ UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, float u,
     const float fy = (1.0f - iv) * static_cast<float>(t.h - 1);
     if (o.interp == 0) {
       // DiffuseTextureNnColorizer (pixel_shader.h:260-298)
-      const uint8_t* dt = tex_at(base, t.w, t.h, f2i_x86(roundf(fy)), f2i_x86(roundf(fx)));
-      c[0] = dt[0];
-      c[1] = dt[1];
-      c[2] = dt[2];
+      const uchar4 dt = tex_at(base, t.w, t.h, f2i_x86(roundf(fy)), f2i_x86(roundf(fx)));
+      c[0] = dt.x;
+      c[1] = dt.y;
+      c[2] = dt.z;
     } else {
       // DiffuseTextureBilinearColorizer (pixel_shader.h:302-359)
       const int x0 = f2i_x86(floorf(fx));
@@ -397,10 +424,12 @@ UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, float u,
       const int x1 = x0 + 1, y1 = y0 + 1;
       const float lu = fx - static_cast<float>(x0);
       const float lv = fy - static_cast<float>(y0);
-      const uint8_t* p00 = tex_at(base, t.w, t.h, y0, x0);
-      const uint8_t* p01 = tex_at(base, t.w, t.h, y0, x1);
-      const uint8_t* p10 = tex_at(base, t.w, t.h, y1, x0);
-      const uint8_t* p11 = tex_at(base, t.w, t.h, y1, x1);
+      const uchar4 q00 = tex_at(base, t.w, t.h, y0, x0);
+      const uchar4 q01 = tex_at(base, t.w, t.h, y0, x1);
+      const uchar4 q10 = tex_at(base, t.w, t.h, y1, x0);
+      const uchar4 q11 = tex_at(base, t.w, t.h, y1, x1);
+      const uint8_t p00[3] = {q00.x, q00.y, q00.z}, p01[3] = {q01.x, q01.y, q01.z};
+      const uint8_t p10[3] = {q10.x, q10.y, q10.z}, p11[3] = {q11.x, q11.y, q11.z};
       for (int k = 0; k < 3; k++) {
         float ic = (1.0f - lu) * (1.0f - lv) * static_cast<float>(p00[k]) +
                    lu * (1.0f - lv) * static_cast<float>(p01[k]) +
@@ -451,6 +480,105 @@ UGU_DEV void ftex_coords(int tw, int th, float iu, float iv, int& x0, int& y0,
   y1 = min(max(iy + 1, 0), th - 1);
 }
 
+// ---------------------------------------------------------------------------
+// Differentiable image model, evaluated in float64 for one pixel of a given
+// face (visibility comes from the float32 rasterizer; the shading and its
+// derivatives are computed in double so the gradients match the exact model,
+// not the reference's float32 rounding of thin triangles).
+
+struct DiffEval {
+  double X[3], Y[3], Z[3], Qx[3], Qy[3];
+  double e[3], A, q[3], s, b0, b1, b2;
+  double c[3], t[3], dcdu[3], dcdv[3], dtdu[3], dtdv[3];
+  double lu, lv;
+  int tx0, ty0, tx1, ty1;
+};
+
+UGU_DEV int clamp_texel(double v, int hi) {
+  if (!(v >= 0.0)) return 0;
+  if (v > static_cast<double>(hi)) return hi;
+  return static_cast<int>(v);
+}
+
+UGU_DEV void diff_eval(const DevView& v, const DevMesh& m, int fid, int px, int py,
+                       const int32_t* vid, DiffEval& e) {
+  const double x = px, y = py;
+  for (int i = 0; i < 3; i++) {
+    const float* P = m.verts + int64_t(vid[i]) * 3;
+    double Q[3];
+    for (int r = 0; r < 3; r++)
+      Q[r] = double(v.w2c_R[r * 3 + 0]) * P[0] + double(v.w2c_R[r * 3 + 1]) * P[1] +
+             double(v.w2c_R[r * 3 + 2]) * P[2] + double(v.w2c_t[r]);
+    e.Qx[i] = Q[0];
+    e.Qy[i] = Q[1];
+    e.Z[i] = Q[2];
+    e.X[i] = double(v.fx) * Q[0] / Q[2] + double(v.cx);
+    e.Y[i] = double(v.fy) * Q[1] / Q[2] + double(v.cy);
+  }
+  auto E = [](double ax, double ay, double bx, double by, double cx, double cy) {
+    return (cx - ax) * (by - ay) - (cy - ay) * (bx - ax);
+  };
+  e.A = E(e.X[0], e.Y[0], e.X[1], e.Y[1], e.X[2], e.Y[2]);
+  e.e[0] = E(e.X[1], e.Y[1], e.X[2], e.Y[2], x, y);
+  e.e[1] = E(e.X[2], e.Y[2], e.X[0], e.Y[0], x, y);
+  e.e[2] = E(e.X[0], e.Y[0], e.X[1], e.Y[1], x, y);
+  for (int i = 0; i < 3; i++) e.q[i] = e.e[i] / e.A / e.Z[i];
+  e.s = e.q[0] + e.q[1] + e.q[2];
+  e.b1 = e.q[1] / e.s;
+  e.b2 = e.q[2] / e.s;
+  e.b0 = 1.0 - e.b1 - e.b2;
+  for (int k = 0; k < 3; k++) {
+    e.c[k] = 1.0;
+    e.t[k] = 1.0;
+    e.dcdu[k] = e.dcdv[k] = e.dtdu[k] = e.dtdv[k] = 0.0;
+  }
+  e.lu = e.lv = 0.0;
+  e.tx0 = e.ty0 = e.tx1 = e.ty1 = 0;
+  if (m.vcolors) {
+    const float* C0 = m.vcolors + int64_t(vid[0]) * 3;
+    const float* C1 = m.vcolors + int64_t(vid[1]) * 3;
+    const float* C2 = m.vcolors + int64_t(vid[2]) * 3;
+    for (int k = 0; k < 3; k++) {
+      e.c[k] = e.b0 * C0[k] + e.b1 * C1[k] + e.b2 * C2[k];
+      e.dcdu[k] = double(C1[k]) - C0[k];
+      e.dcdv[k] = double(C2[k]) - C0[k];
+    }
+  }
+  if (m.ftex) {
+    const int32_t* ui = m.uvidx + int64_t(fid) * 3;
+    const float* a = m.uv + int64_t(ui[0]) * 2;
+    const float* b = m.uv + int64_t(ui[1]) * 2;
+    const float* c = m.uv + int64_t(ui[2]) * 2;
+    const double iu = e.b0 * a[0] + e.b1 * b[0] + e.b2 * c[0];
+    const double iv = e.b0 * a[1] + e.b1 * b[1] + e.b2 * c[1];
+    const int tw = m.ftex_w, th = m.ftex_h;
+    const double fxp = iu * (tw - 1), fyp = (1.0 - iv) * (th - 1);
+    const double flx = floor(fxp), fly = floor(fyp);
+    e.lu = fxp - flx;
+    e.lv = fyp - fly;
+    e.tx0 = clamp_texel(flx, tw - 1);
+    e.ty0 = clamp_texel(fly, th - 1);
+    e.tx1 = clamp_texel(flx + 1.0, tw - 1);
+    e.ty1 = clamp_texel(fly + 1.0, th - 1);
+    const float* p00 = m.ftex + (int64_t(e.ty0) * tw + e.tx0) * 3;
+    const float* p01 = m.ftex + (int64_t(e.ty0) * tw + e.tx1) * 3;
+    const float* p10 = m.ftex + (int64_t(e.ty1) * tw + e.tx0) * 3;
+    const float* p11 = m.ftex + (int64_t(e.ty1) * tw + e.tx1) * 3;
+    const double sx = tw - 1, sy = -(th - 1.0);
+    const double dlu_du = sx * (double(b[0]) - a[0]), dlu_dv = sx * (double(c[0]) - a[0]);
+    const double dlv_du = sy * (double(b[1]) - a[1]), dlv_dv = sy * (double(c[1]) - a[1]);
+    const double lu = e.lu, lv = e.lv;
+    for (int k = 0; k < 3; k++) {
+      e.t[k] = (1 - lu) * (1 - lv) * p00[k] + lu * (1 - lv) * p01[k] +
+               (1 - lu) * lv * p10[k] + lu * lv * p11[k];
+      const double da = (1 - lv) * (double(p01[k]) - p00[k]) + lv * (double(p11[k]) - p10[k]);
+      const double db = (1 - lu) * (double(p10[k]) - p00[k]) + lu * (double(p11[k]) - p01[k]);
+      e.dtdu[k] = da * dlu_du + db * dlv_du;
+      e.dtdv[k] = da * dlu_dv + db * dlv_dv;
+    }
+  }
+}
+
 template <int TW, int TH, int MODE>
 UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
                          const DevMesh& mesh, const float4* proj,
@@ -458,13 +586,8 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   constexpr int NT = TW * TH;
   const int t = threadIdx.x;
 
-  // view of this tile: largest v with tile_base[v] <= gtile
-  int lo = 0, hi = bins.n_views - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (bins.tile_base[mid] <= gtile) lo = mid; else hi = mid - 1;
-  }
-  const DevView& v = views[lo];
+  const int count = bins.counts[gtile];
+  const DevView& v = views[bins.tile_view[gtile]];
   const int lt = gtile - v.tile_base;
   const int tyi = lt / v.tiles_x;
   const int txi = lt - tyi * v.tiles_x;
@@ -483,9 +606,15 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   s.ray[2][t] = r2;
   if (t == 0) s.any_flag = 0;
 
-  const int count = bins.counts[gtile];
   const int n = count < bins.cap ? count : bins.cap;
-  if (t == 0 && count > 0) atomicMax(&bins.status[0], count);
+  // When the whole face list fits one LDS chunk and face ids fit 24 bits,
+  // the depth key also carries the LDS slot of the face (below the id, so
+  // ties still go to the lowest id) and the resolve reads the winner's
+  // staged record instead of gathering it again.
+  const bool slot_key = n <= NT && mesh.F <= (1 << 24);
+  // overflowing tiles report their true occupancy; the host re-renders with
+  // bins that large (one same-address atomic per overflowing tile only)
+  if (t == 0 && count > bins.cap) atomicMax(&bins.status[0], count);
   const int32_t* bin = bins.bins + int64_t(gtile) * bins.cap;
   __syncthreads();
 
@@ -517,6 +646,9 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
         s.fv[10][t] = fn[0]; s.fv[11][t] = fn[1]; s.fv[12][t] = fn[2];
       }
       s.fid[t] = f;
+      s.vid[0][t] = fi[0];
+      s.vid[1][t] = fi[1];
+      s.vid[2][t] = fi[2];
     }
     // block exclusive scan of the pair counts
     {
@@ -558,9 +690,11 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
                      bf, static_cast<float>(px0 + pxl), static_cast<float>(py0 + pyl),
                      z, w0, w1, w2)) {
         if (z >= FLT_MIN) {
+          const unsigned int lowbits =
+              slot_key ? (static_cast<unsigned int>(s.fid[j]) << 8) | static_cast<unsigned int>(j)
+                       : static_cast<unsigned int>(s.fid[j]);
           const unsigned long long key =
-              (static_cast<unsigned long long>(__float_as_uint(z)) << 32) |
-              static_cast<unsigned int>(s.fid[j]);
+              (static_cast<unsigned long long>(__float_as_uint(z)) << 32) | lowbits;
           atomicMin(&s.key[pix], key);
         } else {
           s.flag[pix] = 1;
@@ -572,12 +706,36 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   }
 
   // ---- resolve (rasterizer.cc:243-301) for this thread's pixel
-  int fid = -1;
+  int fid = -1, slot = -1;
   if (s.any_flag && s.flag[t]) {
     sequential_pixel(v, mesh, pv, bin, n, gx, gy, r0, r1, r2, fid);
   } else {
     const unsigned long long key = s.key[t];
-    if (key != ~0ull) fid = static_cast<int>(key & 0xffffffffull);
+    if (key != ~0ull) {
+      const unsigned int low = static_cast<unsigned int>(key & 0xffffffffull);
+      if (slot_key) {
+        fid = static_cast<int>(low >> 8);
+        slot = static_cast<int>(low & 0xffu);
+      } else {
+        fid = static_cast<int>(low);
+      }
+    }
+  }
+  // the winner's projected vertices, area, face normal and vertex ids
+  float4 wa = make_float4(0, 0, 0, 0), wb = wa, wc = wa;
+  float warea = 0.0f, wn0 = 0.0f, wn1 = 0.0f, wn2 = 0.0f;
+  int wv0 = 0, wv1 = 0, wv2 = 0;
+  if (slot >= 0) {
+    wa = make_float4(s.fv[0][slot], s.fv[1][slot], s.fv[2][slot], 0.0f);
+    wb = make_float4(s.fv[3][slot], s.fv[4][slot], s.fv[5][slot], 0.0f);
+    wc = make_float4(s.fv[6][slot], s.fv[7][slot], s.fv[8][slot], 0.0f);
+    warea = s.fv[9][slot];
+    wn0 = s.fv[10][slot];
+    wn1 = s.fv[11][slot];
+    wn2 = s.fv[12][slot];
+    wv0 = s.vid[0][slot];
+    wv1 = s.vid[1][slot];
+    wv2 = s.vid[2][slot];
   }
   __syncthreads();  // LDS is reused by the next tile of a persistent block
   if (gx >= v.W || gy >= v.H) return;
@@ -585,11 +743,24 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   float z = 0.0f, w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
   bool bf = false;
   if (fid >= 0) {
-    const int32_t* fi = mesh.faces + int64_t(fid) * 3;
-    const float4 a = pv[fi[0]], b = pv[fi[1]], c = pv[fi[2]];
-    const float area = edge_fn(a.x, a.y, b.x, b.y, c.x, c.y);
-    bf = backface_of(mesh.fnormals, fid, r0, r1, r2);
-    cover_test(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, area, bf,
+    if (slot < 0) {
+      const int32_t* fi = mesh.faces + int64_t(fid) * 3;
+      wv0 = fi[0];
+      wv1 = fi[1];
+      wv2 = fi[2];
+      wa = pv[wv0];
+      wb = pv[wv1];
+      wc = pv[wv2];
+      warea = edge_fn(wa.x, wa.y, wb.x, wb.y, wc.x, wc.y);
+      if (mesh.fnormals) {
+        const float* fn = mesh.fnormals + int64_t(fid) * 3;
+        wn0 = fn[0];
+        wn1 = fn[1];
+        wn2 = fn[2];
+      }
+    }
+    bf = mesh.fnormals != nullptr && sum3(wn0 * r0, wn1 * r1, wn2 * r2) > 0.0f;
+    cover_test(wa.x, wa.y, wa.z, wb.x, wb.y, wb.z, wc.x, wc.y, wc.z, warea, bf,
                static_cast<float>(gx), static_cast<float>(gy), z, w0, w1, w2);
     if (opt.culling && bf) {
       fid = -1;
@@ -622,7 +793,7 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
         nrm[1] = sum3(v.w2c_R[3] * n0, v.w2c_R[4] * n1, v.w2c_R[5] * n2);
         nrm[2] = sum3(v.w2c_R[6] * n0, v.w2c_R[7] * n1, v.w2c_R[8] * n2);
       }
-      if (v.color) shade_u8(mesh, opt, fid, w1, w2, r0, r1, r2, n0, n1, n2, col);
+      if (v.color) shade_u8(mesh, opt, fid, wv0, wv1, wv2, w1, w2, r0, r1, r2, n0, n1, n2, col);
     }
     if (v.color) {
       uint8_t* cp = v.color + pidx * 3;
@@ -640,39 +811,12 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
     if (v.mask) v.mask[pidx] = msk;
     if (v.face_id) v.face_id[pidx] = fid;
   } else {
-    // differentiable forward: float image = colour term * texture term
+    // differentiable forward: image = colour term * texture term (float64)
     float img[3] = {0.0f, 0.0f, 0.0f};
     if (fid > 0) {
-      const float u = w1, vv = w2, w = 1.0f - u - vv;
-      float cc[3] = {1.0f, 1.0f, 1.0f};
-      if (mesh.vcolors) {
-        const int32_t* f = mesh.faces + int64_t(fid) * 3;
-        const float* c0 = mesh.vcolors + int64_t(f[0]) * 3;
-        const float* c1 = mesh.vcolors + int64_t(f[1]) * 3;
-        const float* c2 = mesh.vcolors + int64_t(f[2]) * 3;
-        for (int k = 0; k < 3; k++) cc[k] = w * c0[k] + u * c1[k] + vv * c2[k];
-      }
-      float tt[3] = {1.0f, 1.0f, 1.0f};
-      if (mesh.ftex) {
-        const int32_t* ui = mesh.uvidx + int64_t(fid) * 3;
-        const float* a = mesh.uv + int64_t(ui[0]) * 2;
-        const float* b = mesh.uv + int64_t(ui[1]) * 2;
-        const float* c = mesh.uv + int64_t(ui[2]) * 2;
-        const float iu = w * a[0] + u * b[0] + vv * c[0];
-        const float iv = w * a[1] + u * b[1] + vv * c[1];
-        int x0, y0, x1, y1;
-        float lu, lv;
-        ftex_coords(mesh.ftex_w, mesh.ftex_h, iu, iv, x0, y0, x1, y1, lu, lv);
-        const int64_t tw = mesh.ftex_w;
-        const float* p00 = mesh.ftex + (int64_t(y0) * tw + x0) * 3;
-        const float* p01 = mesh.ftex + (int64_t(y0) * tw + x1) * 3;
-        const float* p10 = mesh.ftex + (int64_t(y1) * tw + x0) * 3;
-        const float* p11 = mesh.ftex + (int64_t(y1) * tw + x1) * 3;
-        for (int k = 0; k < 3; k++)
-          tt[k] = (1.0f - lu) * (1.0f - lv) * p00[k] + lu * (1.0f - lv) * p01[k] +
-                  (1.0f - lu) * lv * p10[k] + lu * lv * p11[k];
-      }
-      for (int k = 0; k < 3; k++) img[k] = cc[k] * tt[k];
+      DiffEval e;
+      diff_eval(v, mesh, fid, gx, gy, mesh.faces + int64_t(fid) * 3, e);
+      for (int k = 0; k < 3; k++) img[k] = static_cast<float>(e.c[k] * e.t[k]);
     }
     float* ip = v.image + pidx * 3;
     ip[0] = img[0];
@@ -693,15 +837,18 @@ __global__ __launch_bounds__(TW * TH) void k_raster(
                               proj, bins, opt);
     return;
   }
-  // persistent: pull (view, tile) items until the queue is drained; every
-  // block exits once the head passes total_tiles.
+  // persistent: pull runs of kQueueRun consecutive (view, tile) items until
+  // the queue is drained; every block exits once the head passes total_tiles.
+  constexpr int kQueueRun = 4;
   for (;;) {
-    if (threadIdx.x == 0) s.item = atomicAdd(&bins.status[1], 1);
+    if (threadIdx.x == 0) s.item = atomicAdd(&bins.status[1], kQueueRun);
     __syncthreads();
     const int item = s.item;
     __syncthreads();
     if (item >= bins.total_tiles) break;
-    raster_tile<TW, TH, MODE>(s, item, views, mesh, proj, bins, opt);
+    const int end = min(item + kQueueRun, bins.total_tiles);
+    for (int it = item; it < end; it++)
+      raster_tile<TW, TH, MODE>(s, it, views, mesh, proj, bins, opt);
   }
 }
 
@@ -726,6 +873,7 @@ UGU_DEV float run_sum(float val, int run) {
 __global__ __launch_bounds__(256) void k_backward(
     const DevView* __restrict__ views, DevMesh mesh,
     const float4* __restrict__ proj, DevGrads g) {
+  (void)proj;
   const DevView& v = views[blockIdx.y];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int npx = v.W * v.H;
@@ -740,129 +888,71 @@ __global__ __launch_bounds__(256) void k_backward(
   int vid[3] = {0, 0, 0};
 
   if (valid) {
-    const float x = static_cast<float>(p % v.W), y = static_cast<float>(p / v.W);
-    const float G[3] = {v.grad[int64_t(p) * 3 + 0], v.grad[int64_t(p) * 3 + 1],
-                        v.grad[int64_t(p) * 3 + 2]};
+    const int px = p % v.W, py = p / v.W;
+    const double G[3] = {v.grad[int64_t(p) * 3 + 0], v.grad[int64_t(p) * 3 + 1],
+                         v.grad[int64_t(p) * 3 + 2]};
     const int32_t* fi = mesh.faces + int64_t(fid) * 3;
     vid[0] = fi[0]; vid[1] = fi[1]; vid[2] = fi[2];
-    const float4* pv = proj + v.proj_offset;
-    float X[3], Y[3], Z[3], Qx[3], Qy[3];
-    for (int i = 0; i < 3; i++) {
-      const float4 q = pv[vid[i]];
-      X[i] = q.x; Y[i] = q.y; Z[i] = q.z;
-      const float* P = mesh.verts + int64_t(vid[i]) * 3;
-      Qx[i] = sum3(v.w2c_R[0] * P[0], v.w2c_R[1] * P[1], v.w2c_R[2] * P[2]) + v.w2c_t[0];
-      Qy[i] = sum3(v.w2c_R[3] * P[0], v.w2c_R[4] * P[1], v.w2c_R[5] * P[2]) + v.w2c_t[1];
-    }
-    // forward quantities (same expressions as the rasterizer)
-    const float A = edge_fn(X[0], Y[0], X[1], Y[1], X[2], Y[2]);
-    const float e[3] = {edge_fn(X[1], Y[1], X[2], Y[2], x, y),
-                        edge_fn(X[2], Y[2], X[0], Y[0], x, y),
-                        edge_fn(X[0], Y[0], X[1], Y[1], x, y)};
-    float q[3];
-    for (int i = 0; i < 3; i++) q[i] = e[i] / A / Z[i];
-    const float ssum = q[0] + q[1] + q[2];
-    const float zp = 1.0f / ssum;
-    const float b1 = q[1] * zp, b2 = q[2] * zp;
-    const float b0 = 1.0f - b1 - b2;
-    const float beta[3] = {b0, b1, b2};
-
-    float cc[3] = {1.0f, 1.0f, 1.0f}, dcdu[3] = {0, 0, 0}, dcdv[3] = {0, 0, 0};
-    if (mesh.vcolors) {
-      const float* c0 = mesh.vcolors + int64_t(vid[0]) * 3;
-      const float* c1 = mesh.vcolors + int64_t(vid[1]) * 3;
-      const float* c2 = mesh.vcolors + int64_t(vid[2]) * 3;
-      for (int k = 0; k < 3; k++) {
-        cc[k] = b0 * c0[k] + b1 * c1[k] + b2 * c2[k];
-        dcdu[k] = c1[k] - c0[k];
-        dcdv[k] = c2[k] - c0[k];
-      }
-    }
-    float tt[3] = {1.0f, 1.0f, 1.0f}, dtdu[3] = {0, 0, 0}, dtdv[3] = {0, 0, 0};
-    float lu = 0.0f, lv = 0.0f;
-    if (mesh.ftex) {
-      const int32_t* ui = mesh.uvidx + int64_t(fid) * 3;
-      const float* a = mesh.uv + int64_t(ui[0]) * 2;
-      const float* b = mesh.uv + int64_t(ui[1]) * 2;
-      const float* c = mesh.uv + int64_t(ui[2]) * 2;
-      const float iu = b0 * a[0] + b1 * b[0] + b2 * c[0];
-      const float iv = b0 * a[1] + b1 * b[1] + b2 * c[1];
-      ftex_coords(mesh.ftex_w, mesh.ftex_h, iu, iv, tx0, ty0, tx1, ty1, lu, lv);
-      const int64_t tw = mesh.ftex_w;
-      const float* p00 = mesh.ftex + (int64_t(ty0) * tw + tx0) * 3;
-      const float* p01 = mesh.ftex + (int64_t(ty0) * tw + tx1) * 3;
-      const float* p10 = mesh.ftex + (int64_t(ty1) * tw + tx0) * 3;
-      const float* p11 = mesh.ftex + (int64_t(ty1) * tw + tx1) * 3;
-      // d(lu)/du etc. through uv = b0*uv0 + u*uv1 + v*uv2
-      const float sx = static_cast<float>(mesh.ftex_w - 1);
-      const float sy = -static_cast<float>(mesh.ftex_h - 1);
-      const float dlu_du = sx * (b[0] - a[0]), dlu_dv = sx * (c[0] - a[0]);
-      const float dlv_du = sy * (b[1] - a[1]), dlv_dv = sy * (c[1] - a[1]);
-      for (int k = 0; k < 3; k++) {
-        tt[k] = (1.0f - lu) * (1.0f - lv) * p00[k] + lu * (1.0f - lv) * p01[k] +
-                (1.0f - lu) * lv * p10[k] + lu * lv * p11[k];
-        const float dt_dlu = (1.0f - lv) * (p01[k] - p00[k]) + lv * (p11[k] - p10[k]);
-        const float dt_dlv = (1.0f - lu) * (p10[k] - p00[k]) + lu * (p11[k] - p01[k]);
-        dtdu[k] = dt_dlu * dlu_du + dt_dlv * dlv_du;
-        dtdv[k] = dt_dlu * dlu_dv + dt_dlv * dlv_dv;
-      }
-    }
-    // dL/dc = G*t, dL/dt = G*c
-    float gu = 0.0f, gv = 0.0f;
-    float gt[3];
+    DiffEval e;
+    diff_eval(v, mesh, fid, px, py, vid, e);
+    tx0 = e.tx0; ty0 = e.ty0; tx1 = e.tx1; ty1 = e.ty1;
+    const double beta[3] = {e.b0, e.b1, e.b2};
+    // dL/dc = G t, dL/dt = G c; u = b1, v = b2 (b0 = 1 - u - v)
+    double gu = 0.0, gv = 0.0, gt[3];
     for (int k = 0; k < 3; k++) {
-      const float gc = G[k] * tt[k];
-      gt[k] = G[k] * cc[k];
-      for (int i = 0; i < 3; i++) gcol[i * 3 + k] = gc * beta[i];
-      gu += gc * dcdu[k] + gt[k] * dtdu[k];
-      gv += gc * dcdv[k] + gt[k] * dtdv[k];
+      const double gc = G[k] * e.t[k];
+      gt[k] = G[k] * e.c[k];
+      for (int i = 0; i < 3; i++) gcol[i * 3 + k] = static_cast<float>(gc * beta[i]);
+      gu += gc * e.dcdu[k] + gt[k] * e.dtdu[k];
+      gv += gc * e.dcdv[k] + gt[k] * e.dtdv[k];
     }
     if (mesh.ftex) {
-      const float wq[4] = {(1.0f - lu) * (1.0f - lv), lu * (1.0f - lv),
-                           (1.0f - lu) * lv, lu * lv};
+      const double lu = e.lu, lv = e.lv;
+      const double wq[4] = {(1 - lu) * (1 - lv), lu * (1 - lv), (1 - lu) * lv, lu * lv};
       for (int j = 0; j < 4; j++)
-        for (int k = 0; k < 3; k++) gtex[j * 3 + k] = gt[k] * wq[j];
+        for (int k = 0; k < 3; k++) gtex[j * 3 + k] = static_cast<float>(gt[k] * wq[j]);
     }
     if (g.verts) {
-      // b_i = q_i / s  =>  dL/dq_j = (dL/db_j - sum_i dL/db_i b_i) / s
-      const float S = gu * b1 + gv * b2;
-      const float gq[3] = {(0.0f - S) * zp, (gu - S) * zp, (gv - S) * zp};
+      // b_i = q_i / s  =>  dL/dq_j = (dL/db_j - (gu b1 + gv b2)) / s
+      const double S = gu * e.b1 + gv * e.b2;
+      const double gq[3] = {-S / e.s, (gu - S) / e.s, (gv - S) / e.s};
       // q_j = e_j / (A Z_j)
-      float ge[3], gZ[3], gA = 0.0f;
+      double ge[3], gZ[3], gA = 0.0;
       for (int j = 0; j < 3; j++) {
-        ge[j] = gq[j] / (A * Z[j]);
-        gZ[j] = -gq[j] * q[j] / Z[j];
-        gA -= gq[j] * q[j] / A;
+        ge[j] = gq[j] / (e.A * e.Z[j]);
+        gZ[j] = -gq[j] * e.q[j] / e.Z[j];
+        gA -= gq[j] * e.q[j] / e.A;
       }
-      float gX[3] = {0, 0, 0}, gY[3] = {0, 0, 0};
       // E(a,b,c): dE/da = (c.y-b.y, b.x-c.x), dE/db = (a.y-c.y, c.x-a.x),
       //           dE/dc = (b.y-a.y, a.x-b.x)
-      // e0 = E(v1, v2, p), e1 = E(v2, v0, p), e2 = E(v0, v1, p)
+      // e0 = E(v1, v2, p), e1 = E(v2, v0, p), e2 = E(v0, v1, p), A = E(v0, v1, v2)
+      double gX[3] = {0, 0, 0}, gY[3] = {0, 0, 0};
+      const double x = px, y = py;
       const int ea[3] = {1, 2, 0}, eb[3] = {2, 0, 1};
       for (int j = 0; j < 3; j++) {
         const int ia = ea[j], ib = eb[j];
-        gX[ia] += ge[j] * (y - Y[ib]);
-        gY[ia] += ge[j] * (X[ib] - x);
-        gX[ib] += ge[j] * (Y[ia] - y);
-        gY[ib] += ge[j] * (x - X[ia]);
+        gX[ia] += ge[j] * (y - e.Y[ib]);
+        gY[ia] += ge[j] * (e.X[ib] - x);
+        gX[ib] += ge[j] * (e.Y[ia] - y);
+        gY[ib] += ge[j] * (x - e.X[ia]);
       }
-      // A = E(v0, v1, v2)
-      gX[0] += gA * (Y[2] - Y[1]);
-      gY[0] += gA * (X[1] - X[2]);
-      gX[1] += gA * (Y[0] - Y[2]);
-      gY[1] += gA * (X[2] - X[0]);
-      gX[2] += gA * (Y[1] - Y[0]);
-      gY[2] += gA * (X[0] - X[1]);
+      gX[0] += gA * (e.Y[2] - e.Y[1]);
+      gY[0] += gA * (e.X[1] - e.X[2]);
+      gX[1] += gA * (e.Y[0] - e.Y[2]);
+      gY[1] += gA * (e.X[2] - e.X[0]);
+      gX[2] += gA * (e.Y[1] - e.Y[0]);
+      gY[2] += gA * (e.X[0] - e.X[1]);
+      // X = fx Qx / Qz + cx, Y = fy Qy / Qz + cy, Z = Qz; Q = R P + t
       for (int i = 0; i < 3; i++) {
-        const float iz = 1.0f / Z[i];
-        const float gQx = gX[i] * v.fx * iz;
-        const float gQy = gY[i] * v.fy * iz;
-        const float gQz = gZ[i] - gX[i] * v.fx * Qx[i] * iz * iz -
-                          gY[i] * v.fy * Qy[i] * iz * iz;
+        const double iz = 1.0 / e.Z[i];
+        const double gQx = gX[i] * v.fx * iz;
+        const double gQy = gY[i] * v.fy * iz;
+        const double gQz = gZ[i] - gX[i] * v.fx * e.Qx[i] * iz * iz -
+                           gY[i] * v.fy * e.Qy[i] * iz * iz;
         for (int cdim = 0; cdim < 3; cdim++)
-          gpos[i * 3 + cdim] = v.w2c_R[0 * 3 + cdim] * gQx +
-                               v.w2c_R[1 * 3 + cdim] * gQy +
-                               v.w2c_R[2 * 3 + cdim] * gQz;
+          gpos[i * 3 + cdim] = static_cast<float>(double(v.w2c_R[0 * 3 + cdim]) * gQx +
+                                                  double(v.w2c_R[1 * 3 + cdim]) * gQy +
+                                                  double(v.w2c_R[2 * 3 + cdim]) * gQz);
       }
     }
   }
@@ -918,9 +1008,11 @@ __global__ void k_depth_to_u16(const float* __restrict__ d, int64_t n,
 // ---------------------------------------------------------------------------
 // host-side launchers (called from ugu_hip.cpp)
 
-int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status) {
+int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status,
+                 const int32_t* tile_base, int32_t n_views, int32_t* tile_view) {
   int m = n > 4 ? n : 4;
-  hipLaunchKernelGGL(k_clear, dim3((m + 255) / 256), dim3(256), 0, st, counts, n, status);
+  hipLaunchKernelGGL(k_clear, dim3((m + 255) / 256), dim3(256), 0, st, counts, n, status,
+                     tile_base, n_views, tile_view);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -944,13 +1036,13 @@ int launch_setup_bin(hipStream_t st, int tw, int th, const DevView* views,
                      int n_views, const DevMesh& mesh, const float4* proj,
                      const DevBins& bins) {
   if (mesh.F <= 0) return 0;
-  dim3 grid((mesh.F + 255) / 256, n_views), block(256);
+  dim3 grid((mesh.F + 255) / 256, (n_views + kSetupViews - 1) / kSetupViews), block(256);
   if (tw == 8 && th == 8)
-    hipLaunchKernelGGL((k_setup_bin<8, 8>), grid, block, 0, st, views, mesh, proj, bins);
+    hipLaunchKernelGGL((k_setup_bin<8, 8>), grid, block, 0, st, views, n_views, mesh, proj, bins);
   else if (tw == 16 && th == 16)
-    hipLaunchKernelGGL((k_setup_bin<16, 16>), grid, block, 0, st, views, mesh, proj, bins);
+    hipLaunchKernelGGL((k_setup_bin<16, 16>), grid, block, 0, st, views, n_views, mesh, proj, bins);
   else if (tw == 32 && th == 8)
-    hipLaunchKernelGGL((k_setup_bin<32, 8>), grid, block, 0, st, views, mesh, proj, bins);
+    hipLaunchKernelGGL((k_setup_bin<32, 8>), grid, block, 0, st, views, n_views, mesh, proj, bins);
   else
     return -2;
   return hipGetLastError() == hipSuccess ? 0 : -1;

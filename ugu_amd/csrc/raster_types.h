@@ -65,6 +65,7 @@ struct DevBins {
   int32_t total_tiles;
   int32_t* status;   // [0] max bin count seen, [1] work-queue head
   const int32_t* tile_base;  // n_views + 1 prefix of tiles per view
+  int32_t* tile_view;        // view of every global tile (filled by k_clear)
   int32_t n_views;
 };
 

@@ -17,7 +17,8 @@
 #include "raster_types.h"
 
 namespace ugu_hip {
-int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status);
+int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status,
+                 const int32_t* tile_base, int32_t n_views, int32_t* tile_view);
 int launch_project(hipStream_t st, const DevView* views, int n_views,
                    const float* verts, int32_t V, float4* proj);
 int launch_face_normals(hipStream_t st, const float* verts, const int32_t* faces,
@@ -65,7 +66,7 @@ struct DevBuf {
 };
 
 struct GraphKey {
-  int n, total_tiles, V, F, tw, th, persistent_blocks, mode;
+  int n, total_tiles, V, F, tw, th, persistent_blocks, mode, cap;
   int64_t buf_gen, mesh_gen;
   DevOptions opt;
   const void* verts;
@@ -97,11 +98,11 @@ struct ugu_hip_ctx {
   bool has_fn = false, has_normals = false, has_nidx = false, has_vc = false,
        has_uv = false, has_uvidx = false, has_matid = false;
   std::vector<int> tex_empty;
-  DevBuf verts, faces, fnormals, normals, nidx, vcolors, uv, uvidx, matid,
+  DevBuf verts, faces, faces_orig, fnormals, normals, nidx, vcolors, uv, uvidx, matid,
       texinfo, texdata;
 
   // work buffers
-  DevBuf proj, counts, bins, status, views, tile_base, stage_out, diff_fn;
+  DevBuf proj, counts, tile_view, bins, status, views, tile_base, stage_out, diff_fn;
   int64_t buf_gen = 0;
   void* h_views = nullptr;
   size_t h_views_bytes = 0;
@@ -155,6 +156,36 @@ int upload(ugu_hip_ctx* c, DevBuf& b, const void* src, size_t n) {
   if (b.ensure(n) < 0) return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, "device allocation failed");
   UGU_HIP_CHECK(c, hipMemcpy(b.p, src, n, hipMemcpyHostToDevice));
   return 0;
+}
+
+// Renumbers the `count` rows referenced by idx[0..n) in order of first use
+// (unreferenced rows last): *remapped receives the renumbered indices and
+// *order the old row id of each new id.
+void relabel(const int32_t* idx, int64_t n, int32_t count, std::vector<int32_t>* remapped,
+             std::vector<int32_t>* order) {
+  std::vector<int32_t> new_of_old(size_t(count), -1);
+  order->clear();
+  order->reserve(size_t(count));
+  remapped->resize(size_t(n));
+  for (int64_t i = 0; i < n; i++) {
+    int32_t& id = new_of_old[size_t(idx[i])];
+    if (id < 0) {
+      id = static_cast<int32_t>(order->size());
+      order->push_back(idx[i]);
+    }
+    (*remapped)[size_t(i)] = id;
+  }
+  for (int32_t v = 0; v < count; v++)
+    if (new_of_old[size_t(v)] < 0) order->push_back(v);
+}
+
+std::vector<float> gather_rows(const float* src, int32_t rows, int cols,
+                               const std::vector<int32_t>& order) {
+  std::vector<float> out(size_t(rows) * cols);
+  for (int32_t r = 0; r < rows; r++)
+    for (int k = 0; k < cols; k++)
+      out[size_t(r) * cols + k] = src[size_t(order[size_t(r)]) * cols + k];
+  return out;
 }
 
 bool check_idx(const int32_t* idx, int64_t n, int32_t limit) {
@@ -299,6 +330,7 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
   int err;
   if ((err = ensure(c, c->proj, size_t(n) * std::max<int32_t>(dm.V, 1) * sizeof(float4)))) return err;
   if ((err = ensure(c, c->counts, size_t(total_tiles) * 4))) return err;
+  if ((err = ensure(c, c->tile_view, size_t(total_tiles) * 4))) return err;
   if ((err = ensure(c, c->status, 64))) return err;
   if ((err = ensure(c, c->views, size_t(n) * sizeof(DevView)))) return err;
   if ((err = ensure(c, c->tile_base, size_t(n + 1) * 4))) return err;
@@ -371,6 +403,7 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
     b.total_tiles = total_tiles;
     b.status = c->status.as<int32_t>();
     b.tile_base = c->tile_base.as<int32_t>();
+    b.tile_view = c->tile_view.as<int32_t>();
     b.n_views = n;
     const DevView* dv = c->views.as<DevView>();
     float4* proj = c->proj.as<float4>();
@@ -383,7 +416,8 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
 
     auto launch_all = [&](bool timed) -> int {
       if (timed) (void)hipEventRecord(c->ev[0], c->stream);
-      if (launch_clear(c->stream, b.counts, total_tiles, b.status)) return -1;
+      if (launch_clear(c->stream, b.counts, total_tiles, b.status, b.tile_base, n, b.tile_view))
+        return -1;
       if (timed) (void)hipEventRecord(c->ev[1], c->stream);
       if (launch_project(c->stream, dv, n, dm.verts, dm.V, proj)) return -1;
       if (timed) (void)hipEventRecord(c->ev[2], c->stream);
@@ -405,6 +439,7 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
       key.th = th;
       key.persistent_blocks = persistent_blocks;
       key.mode = mode;
+      key.cap = b.cap;  // the capacity is baked into the captured launches
       key.buf_gen = c->buf_gen;
       key.mesh_gen = c->mesh_gen;
       key.opt = dopt;
@@ -451,7 +486,7 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
       c->tm.raster_ms += ms[3];
       c->tm.total_ms += ms[0] + ms[1] + ms[2] + ms[3];
     }
-    if (max_count <= c->tiling.bin_capacity) break;
+    if (max_count <= b.cap) break;
     // a bin overflowed: grow to the measured occupancy and render again
     c->tm.retries++;
     int64_t cap = int64_t(max_count) + max_count / 8 + 64;
@@ -530,10 +565,10 @@ void ugu_hip_destroy(ugu_hip_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graphs(c);
-  DevBuf* bufs[] = {&c->verts, &c->faces, &c->fnormals, &c->normals, &c->nidx,
+  DevBuf* bufs[] = {&c->verts, &c->faces, &c->faces_orig, &c->fnormals, &c->normals, &c->nidx,
                     &c->vcolors, &c->uv, &c->uvidx, &c->matid, &c->texinfo,
                     &c->texdata, &c->proj, &c->counts, &c->bins, &c->status,
-                    &c->views, &c->tile_base, &c->stage_out, &c->diff_fn};
+                    &c->views, &c->tile_base, &c->stage_out, &c->diff_fn, &c->tile_view};
   for (DevBuf* b : bufs) b->release();
   if (c->h_views) (void)hipHostFree(c->h_views);
   if (c->h_status) (void)hipHostFree(c->h_status);
@@ -582,15 +617,43 @@ int ugu_hip_prepare_mesh(ugu_hip_ctx* c, const UguMeshDesc* m) {
   UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   drop_graphs(c);
 
+  // Relabel vertices, normals and uvs in first-touch order of their index
+  // arrays: a screen tile's faces are a compact run of the face array, and
+  // after relabelling their vertices are too, so the per-face gathers of the
+  // rasterizer hit the same cache lines. Faces keep their order (face ids and
+  // the lowest-id tie rule are unchanged) and every value is only moved, so
+  // outputs are bitwise identical. The original face array is kept for the
+  // differentiable path, whose per-vertex parameters arrive in user order.
+  std::vector<int32_t> rv, rn, ru, fr, nr, ur;
+  std::vector<float> pv, pn, pc, pu;
+  relabel(m->faces, F * 3, m->num_vertices, &fr, &rv);
+  const int32_t* faces_r = fr.data();
+  pv = gather_rows(m->vertices, m->num_vertices, 3, rv);
+  if (has_vc) pc = gather_rows(m->vertex_colors, m->num_vertices, 3, rv);
+  const int32_t* nidx_r = nullptr;
+  if (m->normal_indices && has_normals) {
+    relabel(m->normal_indices, F * 3, m->num_normals, &nr, &rn);
+    nidx_r = nr.data();
+    pn = gather_rows(m->normals, m->num_normals, 3, rn);
+  }
+  const int32_t* uvidx_r = nullptr;
+  if (m->uv_indices && has_uv) {
+    relabel(m->uv_indices, F * 3, m->num_uv, &ur, &ru);
+    uvidx_r = ur.data();
+    pu = gather_rows(m->uv, m->num_uv, 2, ru);
+  }
+
   int err;
-  if ((err = upload(c, c->verts, m->vertices, size_t(m->num_vertices) * 12))) return err;
-  if ((err = upload(c, c->faces, m->faces, size_t(F) * 12))) return err;
+  if ((err = upload(c, c->verts, pv.data(), size_t(m->num_vertices) * 12))) return err;
+  if ((err = upload(c, c->faces, faces_r, size_t(F) * 12))) return err;
+  if ((err = upload(c, c->faces_orig, m->faces, size_t(F) * 12))) return err;
   if ((err = upload(c, c->fnormals, F > 0 ? m->face_normals : nullptr, size_t(F) * 12))) return err;
-  if ((err = upload(c, c->normals, has_normals ? m->normals : nullptr, size_t(m->num_normals) * 12))) return err;
-  if ((err = upload(c, c->nidx, m->normal_indices, size_t(F) * 12))) return err;
-  if ((err = upload(c, c->vcolors, has_vc ? m->vertex_colors : nullptr, size_t(m->num_vertices) * 12))) return err;
-  if ((err = upload(c, c->uv, has_uv ? m->uv : nullptr, size_t(m->num_uv) * 8))) return err;
-  if ((err = upload(c, c->uvidx, m->uv_indices, size_t(F) * 12))) return err;
+  if ((err = upload(c, c->normals, has_normals ? (nidx_r ? pn.data() : m->normals) : nullptr,
+                    size_t(m->num_normals) * 12))) return err;
+  if ((err = upload(c, c->nidx, nidx_r ? nidx_r : m->normal_indices, size_t(F) * 12))) return err;
+  if ((err = upload(c, c->vcolors, has_vc ? pc.data() : nullptr, size_t(m->num_vertices) * 12))) return err;
+  if ((err = upload(c, c->uv, has_uv ? (uvidx_r ? pu.data() : m->uv) : nullptr, size_t(m->num_uv) * 8))) return err;
+  if ((err = upload(c, c->uvidx, uvidx_r ? uvidx_r : m->uv_indices, size_t(F) * 12))) return err;
   if ((err = upload(c, c->matid, m->material_ids, size_t(F) * 4))) return err;
 
   c->M = m->num_materials;
@@ -604,16 +667,25 @@ int ugu_hip_prepare_mesh(ugu_hip_ctx* c, const UguMeshDesc* m) {
     ti[i].offset = pool;
     ti[i].w = empty ? 1 : t.width;
     ti[i].h = empty ? 1 : t.height;
-    pool += empty ? 4 : (int64_t(t.width) * t.height * 3 + 255) / 256 * 256;
+    pool += empty ? 256 : (int64_t(t.width) * t.height * 4 + 255) / 256 * 256;
   }
   if (c->M > 0) {
     if ((err = upload(c, c->texinfo, ti.data(), ti.size() * sizeof(TexInfo)))) return err;
     if (c->texdata.ensure(size_t(pool)) < 0) return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, "device allocation failed");
+    std::vector<uint8_t> rgba;
     for (int i = 0; i < c->M; i++) {
       const UguTexture& t = m->textures[i];
       if (c->tex_empty[i]) continue;
-      UGU_HIP_CHECK(c, hipMemcpy(c->texdata.as<uint8_t>() + ti[i].offset, t.data,
-                                 size_t(t.width) * t.height * 3, hipMemcpyHostToDevice));
+      // RGB8 -> RGBA8 (alpha unused): one aligned 32-bit load per texel
+      const size_t n = size_t(t.width) * t.height;
+      rgba.assign(n * 4, 0);
+      for (size_t k = 0; k < n; k++) {
+        rgba[k * 4 + 0] = t.data[k * 3 + 0];
+        rgba[k * 4 + 1] = t.data[k * 3 + 1];
+        rgba[k * 4 + 2] = t.data[k * 3 + 2];
+      }
+      UGU_HIP_CHECK(c, hipMemcpy(c->texdata.as<uint8_t>() + ti[i].offset, rgba.data(), n * 4,
+                                 hipMemcpyHostToDevice));
     }
   } else {
     c->texinfo.release();
@@ -709,7 +781,10 @@ int ugu_hip_autotune(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOpti
     outs[i].mask = take(q);
     outs[i].face_id = reinterpret_cast<int32_t*>(take(4 * q));
   }
-  const int shapes[3][2] = {{8, 8}, {16, 16}, {32, 8}};
+  // candidates: every tile shape, with the persistent work queue and with a
+  // plain one-workgroup-per-tile grid
+  const int shapes[6][3] = {{8, 8, 0}, {16, 16, 0}, {32, 8, 0},
+                            {8, 8, 1}, {16, 16, 1}, {32, 8, 1}};
   const UguTiling saved = c->tiling;
   UguTiling best = saved;
   double best_ms = 1e30;
@@ -717,6 +792,7 @@ int ugu_hip_autotune(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOpti
     UguTiling t = saved;
     t.tile_w = sh[0];
     t.tile_h = sh[1];
+    t.persistent = sh[2];
     t.bin_capacity = 256;
     c->tiling = t;
     drop_graphs(c);
@@ -775,6 +851,7 @@ static int diff_mesh(ugu_hip_ctx* c, const UguDiffParams* p, DevMesh* dm) {
   if (p->texture && (!c->has_uv || !c->has_uvidx || p->tex_width < 1 || p->tex_height < 1))
     return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "texture needs uv, uv_indices and a size");
   *dm = dev_mesh(c);
+  dm->faces = c->faces_orig.as<int32_t>();  // parameters are in user vertex order
   dm->verts = p->vertices;
   dm->vcolors = p->vertex_colors;
   dm->ftex = p->texture;

@@ -272,7 +272,8 @@ class Rasterizer:
             outs[i] = _abi.UguOutputs(*[_ptr(d.get(k)) for k in
                                         ("color", "depth", "normal", "mask", "face_id")])
             res.append(d)
-        cams = (_abi.UguCamera * len(cameras))(*[c.to_abi() for c in cameras])
+        cams = (_abi.UguCamera * len(cameras))(
+            *[c if isinstance(c, _abi.UguCamera) else c.to_abi() for c in cameras])
         opt = options_to_abi(self.option_)
         fn = self._lib.ugu_hip_render_batch if graph else self._lib.ugu_hip_render_batch_eager
         self._check(fn(self._ctx, cams, len(cameras), ctypes.byref(opt), outs,
