@@ -34,7 +34,8 @@
  *
  * Threading: one context = one HIP stream; calls on one context must not
  * overlap (the reference's Render is likewise not reentrant). Every call is
- * synchronous: outputs are complete when it returns.
+ * synchronous: outputs are complete when it returns, and it starts only after
+ * the work queued on the caller's stream (ugu_hip_set_stream).
  */
 #ifndef UGU_HIP_H_
 #define UGU_HIP_H_
@@ -168,8 +169,10 @@ void ugu_hip_destroy(ugu_hip_ctx* ctx);
 const char* ugu_hip_last_error(const ugu_hip_ctx* ctx);
 int ugu_hip_abi_version(void);
 
-/* Use a caller-provided hipStream_t (e.g. torch's current stream); null
- * restores the context's own stream. */
+/* Stream ordering with the caller: every later call first waits for the
+ * work already queued on hip_stream (e.g. torch's current stream; null = the
+ * legacy default stream, which is also the initial setting). The kernels run
+ * on the context's own stream; calls return when their work is complete. */
 int ugu_hip_set_stream(ugu_hip_ctx* ctx, void* hip_stream);
 
 /* set_mesh + PrepareMesh: validates indices and uploads the mesh. The device

@@ -148,21 +148,32 @@ def test_backward_matches_float64_and_finite_differences(gpu_available, c4):
             g_gpu_all["texture"] = t.grad.cpu().numpy()
         n_per = 128
         fine = []
+        skipped = []
         for name, g_ref, base in groups:
             flat_ref = g_ref.reshape(-1)
             nz = np.nonzero(np.abs(flat_ref) > 1e-3 * np.abs(flat_ref).max())[0]
             idx = rng.choice(nz, size=min(n_per, len(nz)), replace=False)
             for j in idx:
                 if name == "verts":
-                    # the loss is rational in the positions: central differences
-                    # at h and h/2 combined by Richardson extrapolation (error
-                    # O(h^4)); h is 6% of an edge of these 100k-face triangles
+                    # The loss is rational in the positions. h is 6% of an edge
+                    # of these 100k-face triangles, and at fixed visibility a
+                    # step that large can flip a thin triangle's screen
+                    # orientation (area through 0, a pole of the barycentrics).
+                    # Central differences at h, h/2, h/4 are combined by
+                    # Richardson extrapolation (error O(h^4)); a parameter is
+                    # compared only when the two extrapolations agree, i.e. the
+                    # loss is smooth over the whole step.
                     d1 = central(name, j, base, h)
                     d2 = central(name, j, base, h / 2)
-                    fd = (4.0 * d2 - d1) / 3.0
+                    d3 = central(name, j, base, h / 4)
+                    r1, r2 = (4.0 * d2 - d1) / 3.0, (4.0 * d3 - d2) / 3.0
                     small = central(name, j, base, 1e-5)
                     g = g_gpu_all[name].reshape(-1)[j]
                     fine.append(abs(g - small) / max(abs(small), 1e-12))
+                    if abs(r1 - r2) > 1e-4 * max(abs(r2), 1e-9):
+                        skipped.append(j)
+                        continue
+                    fd = r2
                 else:
                     # linear in colours and texels: central differences are exact
                     fd = central(name, j, base, h)
@@ -176,9 +187,11 @@ def test_backward_matches_float64_and_finite_differences(gpu_available, c4):
         if fine:
             fine = np.array(fine)
             print(f"[diff] positions, central h=1e-5: max rel {fine.max():.3g} "
-                  f"median {np.median(fine):.3g}", flush=True)
+                  f"median {np.median(fine):.3g}; h=1e-3 steps not smooth for "
+                  f"{len(skipped)} of {len(fine)} sampled positions", flush=True)
             assert np.median(fine) < 1e-4
-        assert len(checks) >= 200
+            assert len(skipped) <= 0.25 * len(fine)
+        assert len(checks) + len(skipped) >= 200
         assert normwise < 1e-4
         assert np.median(rel) < 1e-4
 

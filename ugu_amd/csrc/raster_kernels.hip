@@ -100,12 +100,21 @@ UGU_DEV float4 project_point(const DevView& v, float px, float py, float pz) {
   float qx = sum3(v.w2c_R[0] * px, v.w2c_R[1] * py, v.w2c_R[2] * pz) + v.w2c_t[0];
   float qy = sum3(v.w2c_R[3] * px, v.w2c_R[4] * py, v.w2c_R[5] * pz) + v.w2c_t[1];
   float qz = sum3(v.w2c_R[6] * px, v.w2c_R[7] * py, v.w2c_R[8] * pz) + v.w2c_t[2];
+  // fx / qz and fy / qz are the same correctly rounded quotient when fx == fy
+  const float sx = v.fx / qz;
+  const float sy = v.fy == v.fx ? sx : v.fy / qz;
   float4 r;
-  r.x = v.fx / qz * qx + v.cx;
-  r.y = v.fy / qz * qy + v.cy;
+  r.x = sx * qx + v.cx;
+  r.y = sy * qy + v.cy;
   r.z = qz;
   r.w = 0.0f;
   return r;
+}
+
+// Projects vertex `i` of the world-space position array (3 floats per vertex).
+UGU_DEV float4 project_vertex(const DevView& v, const float* __restrict__ verts, int i) {
+  const float* p = verts + int64_t(i) * 3;
+  return project_point(v, p[0], p[1], p[2]);
 }
 
 // Per-face setup of rasterizer.cc:164-193. Returns false when the
@@ -285,13 +294,17 @@ template <int TW, int TH>
 __global__ __launch_bounds__(256) void k_setup_bin(
     const DevView* __restrict__ views, int n_views, DevMesh mesh,
     const float4* __restrict__ proj, DevBins bins) {
+  (void)proj;
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  int i0 = 0, i1 = 0, i2 = 0;
+  float P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (f < mesh.F) {
     const int32_t* fi = mesh.faces + int64_t(f) * 3;
-    i0 = fi[0];
-    i1 = fi[1];
-    i2 = fi[2];
+    for (int k = 0; k < 3; k++) {
+      const float* p = mesh.verts + int64_t(fi[k]) * 3;
+      P[k * 3 + 0] = p[0];
+      P[k * 3 + 1] = p[1];
+      P[k * 3 + 2] = p[2];
+    }
   }
   const int vb = blockIdx.y * kSetupViews;
   const int ve = min(vb + kSetupViews, n_views);
@@ -300,8 +313,9 @@ __global__ __launch_bounds__(256) void k_setup_bin(
     bool has = false;
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
     if (f < mesh.F) {
-      const float4* pv = proj + v.proj_offset;
-      const float4 a = pv[i0], b = pv[i1], c = pv[i2];
+      const float4 a = project_point(v, P[0], P[1], P[2]);
+      const float4 b = project_point(v, P[3], P[4], P[5]);
+      const float4 c = project_point(v, P[6], P[7], P[8]);
       int x0, x1, y0, y1;
       float area;
       if (face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) {
@@ -364,7 +378,9 @@ UGU_DEV void sequential_pixel(const DevView& v, const DevMesh& mesh,
       const int f = bin[i];
       if (f <= cur || f >= best) continue;
       const int32_t* fi = mesh.faces + int64_t(f) * 3;
-      float4 a = pv[fi[0]], b = pv[fi[1]], c = pv[fi[2]];
+      const float4 a = project_vertex(v, mesh.verts, fi[0]);
+      const float4 b = project_vertex(v, mesh.verts, fi[1]);
+      const float4 c = project_vertex(v, mesh.verts, fi[2]);
       int x0, x1, y0, y1;
       float area;
       if (!face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) continue;
@@ -592,11 +608,36 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   const int tyi = lt / v.tiles_x;
   const int txi = lt - tyi * v.tiles_x;
   const int px0 = txi * TW, py0 = tyi * TH;
-  const float4* pv = proj + v.proj_offset;
+  const float4* pv = nullptr;
+  (void)proj;
 
   // this thread's pixel
   const int lx = t % TW, ly = t / TW;
   const int gx = px0 + lx, gy = py0 + ly;
+  if (count == 0) {
+    // no face touches the tile: every pixel keeps the initial values of
+    // ValidateAndInitBeforeRender (util_private.cc:68-87)
+    if (gx < v.W && gy < v.H) {
+      const int64_t pidx = int64_t(gy) * v.W + gx;
+      if (MODE == 0) {
+        if (v.color) {
+          uint8_t* cp = v.color + pidx * 3;
+          cp[0] = cp[1] = cp[2] = 0;
+        }
+        if (v.depth) v.depth[pidx] = 0.0f;
+        if (v.normal) {
+          float* np = v.normal + pidx * 3;
+          np[0] = np[1] = np[2] = 0.0f;
+        }
+        if (v.mask) v.mask[pidx] = 0;
+      } else {
+        float* ip = v.image + pidx * 3;
+        ip[0] = ip[1] = ip[2] = 0.0f;
+      }
+      if (v.face_id) v.face_id[pidx] = -1;
+    }
+    return;
+  }
   float r0, r1, r2;
   ray_w_at(v, static_cast<float>(gx), static_cast<float>(gy), r0, r1, r2);
   s.key[t] = ~0ull;
@@ -625,7 +666,9 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
     if (i < n) {
       const int f = bin[i];
       const int32_t* fi = mesh.faces + int64_t(f) * 3;
-      const float4 a = pv[fi[0]], b = pv[fi[1]], cc = pv[fi[2]];
+      const float4 a = project_vertex(v, mesh.verts, fi[0]);
+      const float4 b = project_vertex(v, mesh.verts, fi[1]);
+      const float4 cc = project_vertex(v, mesh.verts, fi[2]);
       int x0, x1, y0, y1;
       float area;
       if (face_setup(a, b, cc, v.W, v.H, x0, x1, y0, y1, area)) {
@@ -748,9 +791,9 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
       wv0 = fi[0];
       wv1 = fi[1];
       wv2 = fi[2];
-      wa = pv[wv0];
-      wb = pv[wv1];
-      wc = pv[wv2];
+      wa = project_vertex(v, mesh.verts, wv0);
+      wb = project_vertex(v, mesh.verts, wv1);
+      wc = project_vertex(v, mesh.verts, wv2);
       warea = edge_fn(wa.x, wa.y, wb.x, wb.y, wc.x, wc.y);
       if (mesh.fnormals) {
         const float* fn = mesh.fnormals + int64_t(fid) * 3;
@@ -978,7 +1021,11 @@ __global__ __launch_bounds__(256) void k_backward(
     }
   }
   if (g.texture && mesh.ftex) {
-    const int tkey = valid ? ty0 * mesh.ftex_w + tx0 : -1;
+    // lanes share a run only when all four texels agree (the +1 neighbours
+    // are clamped at the texture border)
+    const int tkey = valid ? ((ty0 * mesh.ftex_w + tx0) << 2) | (tx1 == tx0 ? 1 : 0) |
+                                 (ty1 == ty0 ? 2 : 0)
+                           : -1;
     const int prev_t = __shfl_up(tkey, 1);
     const bool thead = valid && (lane == 0 || prev_t != tkey);
     const uint64_t theads = __ballot(thead);

@@ -87,8 +87,10 @@ constexpr size_t kMaxGraphs = 16;
 
 struct ugu_hip_ctx {
   int device = 0;
-  hipStream_t own = nullptr;
-  hipStream_t stream = nullptr;
+  hipStream_t own = nullptr;     // every kernel and copy of the context runs here
+  hipStream_t stream = nullptr;  // == own
+  hipStream_t user = nullptr;    // caller's stream the calls are ordered after (null = legacy default)
+  hipEvent_t ev_user = nullptr;
   std::string err;
 
   // prepared mesh
@@ -135,6 +137,17 @@ int hip_fail(ugu_hip_ctx* c, hipError_t e, const char* where) {
     hipError_t e_ = (expr);                                        \
     if (e_ != hipSuccess) return hip_fail((ctx), e_, #expr);       \
   } while (0)
+
+// Orders the context's stream after everything already queued on the
+// caller's stream (torch's current stream, or the legacy default stream):
+// without it, kernels of this library could read tensors whose producing
+// kernels have not run yet. Calls are host-synchronous, so work the caller
+// queues afterwards sees the results.
+int order_after_user(ugu_hip_ctx* c) {
+  UGU_HIP_CHECK(c, hipEventRecord(c->ev_user, c->user));
+  UGU_HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ev_user, 0));
+  return 0;
+}
 
 int ensure(ugu_hip_ctx* c, DevBuf& b, size_t n) {
   int r = b.ensure(n);
@@ -328,7 +341,7 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
   const int total_tiles = static_cast<int>(total);
 
   int err;
-  if ((err = ensure(c, c->proj, size_t(n) * std::max<int32_t>(dm.V, 1) * sizeof(float4)))) return err;
+  if ((err = ensure(c, c->proj, 16))) return err;
   if ((err = ensure(c, c->counts, size_t(total_tiles) * 4))) return err;
   if ((err = ensure(c, c->tile_view, size_t(total_tiles) * 4))) return err;
   if ((err = ensure(c, c->status, 64))) return err;
@@ -419,7 +432,6 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
       if (launch_clear(c->stream, b.counts, total_tiles, b.status, b.tile_base, n, b.tile_view))
         return -1;
       if (timed) (void)hipEventRecord(c->ev[1], c->stream);
-      if (launch_project(c->stream, dv, n, dm.verts, dm.V, proj)) return -1;
       if (timed) (void)hipEventRecord(c->ev[2], c->stream);
       if (launch_setup_bin(c->stream, tw, th, dv, n, dm, proj, b)) return -1;
       if (timed) (void)hipEventRecord(c->ev[3], c->stream);
@@ -528,6 +540,7 @@ int render_common(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions
     if (err) return err;
   }
   UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int oe = order_after_user(c)) return oe;
   DevMesh dm = dev_mesh(c);
   return run_pipeline(c, cams, n, dm, dev_options(*o), 0, outs, memory, nullptr, nullptr, use_graph);
 }
@@ -549,6 +562,11 @@ ugu_hip_ctx* ugu_hip_create(int device) {
     return nullptr;
   }
   c->stream = c->own;
+  if (hipEventCreateWithFlags(&c->ev_user, hipEventDisableTiming) != hipSuccess) {
+    (void)hipStreamDestroy(c->own);
+    delete c;
+    return nullptr;
+  }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
   if (hipHostMalloc(reinterpret_cast<void**>(&c->h_status), 64, hipHostMallocDefault) != hipSuccess) {
@@ -574,6 +592,7 @@ void ugu_hip_destroy(ugu_hip_ctx* c) {
   if (c->h_status) (void)hipHostFree(c->h_status);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->ev_user) (void)hipEventDestroy(c->ev_user);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
 }
@@ -584,9 +603,7 @@ const char* ugu_hip_last_error(const ugu_hip_ctx* c) {
 
 int ugu_hip_set_stream(ugu_hip_ctx* c, void* s) {
   if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
-  hipStream_t ns = s ? static_cast<hipStream_t>(s) : c->own;
-  if (ns != c->stream) drop_graphs(c);
-  c->stream = ns;
+  c->user = static_cast<hipStream_t>(s);
   return UGU_HIP_OK;
 }
 
@@ -614,6 +631,7 @@ int ugu_hip_prepare_mesh(ugu_hip_ctx* c, const UguMeshDesc* m) {
   if (m->material_ids && !check_idx(m->material_ids, F, m->num_materials))
     return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "material id out of range");
   UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int oe = order_after_user(c)) return oe;
   UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   drop_graphs(c);
 
@@ -840,6 +858,7 @@ int ugu_hip_depth_to_u16(ugu_hip_ctx* c, const float* depth, int64_t n, uint16_t
     return UGU_HIP_OK;
   }
   UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int oe = order_after_user(c)) return oe;
   if (launch_depth_to_u16(c->stream, depth, n, out)) return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
   UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return UGU_HIP_OK;
@@ -872,6 +891,7 @@ int ugu_hip_diff_forward(ugu_hip_ctx* c, const UguCamera* cams, int n,
   for (int i = 0; i < n; i++)
     if (!images[i] || !face_ids[i]) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "null image");
   UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int oe = order_after_user(c)) return oe;
   DevMesh dm;
   int err = diff_mesh(c, p, &dm);
   if (err) return err;
@@ -892,12 +912,12 @@ int ugu_hip_diff_backward(ugu_hip_ctx* c, const UguCamera* cams, int n,
   if (n < 1 || !cams || !grad_images || !face_ids)
     return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "views, gradients and face_ids are required");
   UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int oe = order_after_user(c)) return oe;
   DevMesh dm;
   int err = diff_mesh(c, p, &dm);
   if (err) return err;
   int max_px = 0;
   if ((err = ensure(c, c->views, size_t(n) * sizeof(DevView)))) return err;
-  if ((err = ensure(c, c->proj, size_t(n) * std::max<int32_t>(dm.V, 1) * sizeof(float4)))) return err;
   const size_t view_bytes = size_t(n) * sizeof(DevView) + size_t(n + 1) * 4;
   if (c->h_views_bytes < view_bytes) {
     if (c->h_views) (void)hipHostFree(c->h_views);
@@ -929,14 +949,11 @@ int ugu_hip_diff_backward(ugu_hip_ctx* c, const UguCamera* cams, int n,
   UGU_HIP_CHECK(c, hipMemcpyAsync(c->views.p, c->h_views, size_t(n) * sizeof(DevView),
                                   hipMemcpyHostToDevice, c->stream));
   const DevView* dv = c->views.as<DevView>();
-  float4* proj = c->proj.as<float4>();
-  if (launch_project(c->stream, dv, n, dm.verts, dm.V, proj))
-    return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
   DevGrads g;
   g.verts = grad_vertices;
   g.colors = p->vertex_colors ? grad_vertex_colors : nullptr;
   g.texture = p->texture ? grad_texture : nullptr;
-  if (launch_backward(c->stream, dv, n, max_px, dm, proj, g))
+  if (launch_backward(c->stream, dv, n, max_px, dm, nullptr, g))
     return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
   UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return UGU_HIP_OK;
