@@ -135,9 +135,18 @@ def diff_benchmarks(torch, np, scenes, mesh, cams, args):
     tt = torch.tensor(scenes.checker_noise_texture(2048, 7, dtype=np.float32), device=dev)
     abi = [c.to_abi() for c in cams]
     with torch.no_grad():
-        tg = dr.render(abi, vv, cc * 0.9, tt)
+        tg_flat, _ = dr.render_flat(abi, vv, cc * 0.9, tt)
+    tg, off = [], 0
+    for c in cams:
+        n = c.width * c.height * 3
+        tg.append(tg_flat[off:off + n].view(c.height, c.width, 3))
+        off += n
     pv, pc, pt = (x.clone().requires_grad_(True) for x in (vv, cc, tt))
-    tb = time_calls(torch, lambda: fit_step(dr, abi, pv, pc, pt, tg), max(3, args.steps // 2), 2)
+
+    def batched_step():
+        flat, _ = dr.render_flat(abi, pv, pc, pt)
+        ((flat - tg_flat) ** 2).sum().backward()
+    tb = time_calls(torch, batched_step, max(3, args.steps // 2), 2)
 
     def per_view():
         for a, g in zip(abi, tg):
@@ -157,7 +166,7 @@ def diff_benchmarks(torch, np, scenes, mesh, cams, args):
     a5 = [c.to_abi() for c in vc]
     gt_tex = torch.tensor(scenes.checker_noise_texture(2048, 8, dtype=np.float32), device=dev)
     with torch.no_grad():
-        tg5 = dr.render(a5, vv, None, gt_tex)
+        tg5, _ = dr.render_flat(a5, vv, None, gt_tex)
     learn = torch.full_like(gt_tex, 0.5).requires_grad_(True)
     opt = torch.optim.Adam([learn], lr=1e-2)
     seen = None
@@ -166,15 +175,16 @@ def diff_benchmarks(torch, np, scenes, mesh, cams, args):
         if it == 3:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-        fit_step(dr, a5, vv, None, learn, tg5, opt)
+        flat, _ = dr.render_flat(a5, vv, None, learn)
+        opt.zero_grad(set_to_none=True)
+        ((flat - tg5) ** 2).sum().backward()
+        opt.step()
         if seen is None:
             # texels that receive gradient from at least one view
             seen = (learn.grad.abs().sum(-1) > 0) if learn.grad is not None else None
     torch.cuda.synchronize()
     t5 = (time.perf_counter() - t0) / n_iter
     with torch.no_grad():
-        if seen is None:
-            imgs = dr.render(a5, vv, None, learn)
         m = seen if seen is not None else torch.ones(gt_tex.shape[:2], dtype=torch.bool, device=dev)
         mse = ((learn.clamp(0, 1) - gt_tex) ** 2).mean(-1)[m].mean().item()
     out["inverse_texturing_32view"] = {"iters_per_s": 1.0 / t5, "ms_per_iter": t5 * 1e3,
@@ -196,6 +206,8 @@ def main():
     ap.add_argument("--tile", default="auto",
                     help="auto | 8x8 | 16x16 | 32x8, optionally with :p (persistent) or :g (grid)")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable legs")
+    ap.add_argument("--diff-only", action="store_true",
+                    help="run only the differentiable legs (prints their JSON)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (counter calibration)")
     args = ap.parse_args()
@@ -222,6 +234,9 @@ def main():
     mesh, cam2 = scenes.scene_icosphere8()
     cams = scenes.multiview_cameras(64)
     F = len(mesh.vertex_indices)
+    if args.diff_only:
+        print(json.dumps(diff_benchmarks(torch, np, scenes, mesh, cams, args)))
+        return
     opt = RendererCpuOption(diffuse_color=DiffuseColor.kTexture,
                             interp=ColorInterpolation.kBilinear)
     r = Rasterizer(opt)

@@ -113,6 +113,18 @@ class DiffRenderer:
                                                      _ptr(gv), _ptr(gc), _ptr(gt)))
         return gv, gc, gt
 
+    def render_flat(self, cameras, vertices, colors=None, texture=None):
+        """All views in one flat float32 tensor (view i occupies H_i*W_i*3
+        consecutive values, row-major HWC) and the flat face ids; a loss over
+        many views is then a few kernels instead of a few per view."""
+        cams = [c if isinstance(c, _abi.UguCamera) else c.to_abi() for c in cameras]
+        vertices = _check_param(vertices, "vertices", (3,))
+        if vertices.shape[0] != self.num_vertices:
+            raise ValueError("vertices must match the prepared mesh")
+        colors = _check_param(colors, "colors", (3,))
+        texture = _check_param(texture, "texture", (3,))
+        return RasterizeFunction.apply(vertices, colors, texture, self, cams)
+
     def render(self, cameras, vertices, colors=None, texture=None, return_face_ids=False):
         """Images (list of (H, W, 3) float32 tensors) for each camera."""
         cams = [c if isinstance(c, _abi.UguCamera) else c.to_abi() for c in cameras]
