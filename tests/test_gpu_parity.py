@@ -151,6 +151,18 @@ def test_multiview_batched_equals_per_view_and_oracle(gpu_available, ico_scene):
             assert np.array_equal(bufs[i]["depth"].cpu().numpy(), batched[i]["depth"])
 
 
+@pytest.mark.parametrize("tile,cull", [((16, 16), True), ((8, 8), False), ((32, 8), True)])
+def test_layered_occlusion_multichunk(gpu_available, tile, cull):
+    """Thousands of faces per tile in shuffled id order: faces skipped as
+    hidden behind the tile's current depths must not change any pixel."""
+    tris = TS.layered_grid_scene(seed=3)
+    mesh = TS.mesh_from_pixels(tris, seed=3)
+    cam = TS.exact_camera(96, 64)
+    opt = RendererCpuOption(diffuse_color=DiffuseColor.kVertex, backface_culling=cull)
+    out = _render(mesh, cam, opt, tile, cap=8192)
+    _compare(out, _oracle(mesh, cam, opt), f"layered tile{tile} cull={cull}")
+
+
 @pytest.mark.parametrize("seed", list(range(12)))
 def test_tiny_random_exact_ties(gpu_available, seed):
     w, h = [(8, 8), (16, 16), (32, 32), (13, 29)][seed % 4]

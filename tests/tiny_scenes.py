@@ -92,3 +92,35 @@ def subnormal_depth_scene(w=16, h=16, flip=False):
     m.set_vertex_colors(rng.random((len(verts), 3), dtype=np.float32) * 255.0)
     cam = PinholeCamera(w, h, np.eye(4), (0.0, 0.0), (fx, fx))
     return m, cam
+
+
+def layered_grid_scene(w=96, h=64, cell=1.5, seed=0):
+    """Stacked triangle grids over the whole image, many faces per tile: the
+    multi-chunk path of the rasterizer with depth-based skipping of hidden
+    faces. Layers at equal depths (exact ties), depths 2^-9 and 2^-7 apart
+    (inside and just outside the skip margin), tilted layers, randomly flipped
+    winding (back faces), and face ids shuffled so the nearest layers are not
+    the first ones a tile sees."""
+    rng = np.random.default_rng(seed)
+    # (z at u = v = 0, dz/du, dz/dv)
+    layers = [(2.0, 0.0, 0.0), (2.0, 0.0, 0.0), (2.0 * (1 + 2 ** -9), 0.0, 0.0),
+              (2.0 * (1 + 2 ** -7), 0.0, 0.0), (1.99, 0.0002, 0.0), (1.6, 0.0, 0.01),
+              (3.0, 0.01, 0.0), (4.0, 0.0, 0.0)]
+    us = np.arange(-1.0, w + 1.0, cell)
+    vs = np.arange(-1.0, h + 1.0, cell)
+    tris = []
+    for z0, su, sv in layers:
+        jit = rng.uniform(-0.2, 0.2, size=(len(vs), len(us), 2)) * cell
+        for j in range(len(vs) - 1):
+            for i in range(len(us) - 1):
+                p = [(us[i] + jit[j, i, 0], vs[j] + jit[j, i, 1]),
+                     (us[i + 1] + jit[j, i + 1, 0], vs[j] + jit[j, i + 1, 1]),
+                     (us[i] + jit[j + 1, i, 0], vs[j + 1] + jit[j + 1, i, 1]),
+                     (us[i + 1] + jit[j + 1, i + 1, 0], vs[j + 1] + jit[j + 1, i + 1, 1])]
+                for a, b, c in ((0, 2, 1), (1, 2, 3)):   # facing the camera
+                    corners = [p[a], p[b], p[c]]
+                    if rng.random() < 0.3:
+                        corners = corners[::-1]
+                    tris.append([(u, v, z0 + su * u + sv * v) for u, v in corners])
+    tris = np.asarray(tris)
+    return tris[rng.permutation(len(tris))]

@@ -14,7 +14,7 @@
 //                 64-bit (depth bits << 32 | face id) LDS atomicMin, then
 //                 shades and writes every output pixel of the tile once
 //                 (resolve loop rasterizer.cc:243-301, pixel_shader.h).
-//   k_backward    differentiable backward pass (no reference counterpart).
+//   k_bwd_*       differentiable backward pass (no reference counterpart).
 //
 // Bit-exactness: all coverage arithmetic follows the reference's expression
 // order; this file is compiled with -ffp-contract=off and correctly rounded
@@ -111,10 +111,9 @@ UGU_DEV float4 project_point(const DevView& v, float px, float py, float pz) {
   return r;
 }
 
-// Projects vertex `i` of the world-space position array (3 floats per vertex).
-UGU_DEV float4 project_vertex(const DevView& v, const float* __restrict__ verts, int i) {
-  const float* p = verts + int64_t(i) * 3;
-  return project_point(v, p[0], p[1], p[2]);
+// Projects a vertex position of a face record (.w holds the vertex id).
+UGU_DEV float4 project_soup(const DevView& v, const float4& p) {
+  return project_point(v, p.x, p.y, p.z);
 }
 
 // Per-face setup of rasterizer.cc:164-193. Returns false when the
@@ -193,14 +192,15 @@ UGU_DEV int xcd_remap(int b, int nb) {
 // ---------------------------------------------------------------------------
 // k_clear
 
-// Also fills the tile -> view table (largest v with tile_base[v] <= tile),
+// Two counters per tile (front and back segments of its bin). Also fills the
+// tile -> view table (largest v with tile_base[v] <= tile),
 // so the rasterizer finds its view with one load instead of a search.
 __global__ void k_clear(int32_t* counts, int32_t n, int32_t* status,
                         const int32_t* __restrict__ tile_base, int32_t n_views,
                         int32_t* __restrict__ tile_view) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    counts[i] = 0;
+    reinterpret_cast<int2*>(counts)[i] = make_int2(0, 0);
     int lo = 0, hi = n_views - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -227,8 +227,8 @@ __global__ void k_project(const DevView* __restrict__ views,
 }
 
 // ---------------------------------------------------------------------------
-// k_face_normals: Mesh::CalcFaceNormal (src/mesh.cc:339-349) on device, for
-// the differentiable path where positions change between calls.
+// Mesh::CalcFaceNormal (src/mesh.cc:339-349) pieces, for the differentiable
+// path where positions change between calls.
 
 UGU_DEV void normalize3(float& x, float& y, float& z) {
   float n = sum3(x * x, y * y, z * z);
@@ -240,50 +240,88 @@ UGU_DEV void normalize3(float& x, float& y, float& z) {
   }
 }
 
-__global__ void k_face_normals(const float* __restrict__ verts,
-                               const int32_t* __restrict__ faces, int32_t F,
-                               float* __restrict__ fn) {
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
+
+// ---------------------------------------------------------------------------
+// k_face_soup: one 64-byte record per face (three vertex positions with the
+// vertex ids in .w, then the face normal) so setup and raster fetch a face
+// with one dependent load instead of an index load and three vertex gathers.
+// compute_normals: Mesh::CalcFaceNormal on device (differentiable path, the
+// positions change every call; also written to fn_out); otherwise the mesh's
+// face normals are copied (zero when it has none).
+
+__global__ void k_face_soup(const float* __restrict__ verts, const int32_t* __restrict__ faces,
+                            const float* __restrict__ fnormals, int32_t F, int compute_normals,
+                            float4* __restrict__ soup, float* __restrict__ fn_out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= F) return;
   const int32_t* fi = faces + int64_t(f) * 3;
-  const float* p0 = verts + int64_t(fi[0]) * 3;
-  const float* p1 = verts + int64_t(fi[1]) * 3;
-  const float* p2 = verts + int64_t(fi[2]) * 3;
-  float ax = p1[0] - p0[0], ay = p1[1] - p0[1], az = p1[2] - p0[2];
-  float bx = p2[0] - p0[0], by = p2[1] - p0[1], bz = p2[2] - p0[2];
-  normalize3(ax, ay, az);
-  normalize3(bx, by, bz);
-  float cx = ay * bz - az * by;
-  float cy = az * bx - ax * bz;
-  float cz = ax * by - ay * bx;
-  normalize3(cx, cy, cz);
-  fn[int64_t(f) * 3 + 0] = cx;
-  fn[int64_t(f) * 3 + 1] = cy;
-  fn[int64_t(f) * 3 + 2] = cz;
+  float4 p[3];
+  for (int k = 0; k < 3; k++) {
+    const float* q = verts + int64_t(fi[k]) * 3;
+    p[k] = make_float4(q[0], q[1], q[2], __int_as_float(fi[k]));
+  }
+  float4 n = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (compute_normals) {
+    float ax = p[1].x - p[0].x, ay = p[1].y - p[0].y, az = p[1].z - p[0].z;
+    float bx = p[2].x - p[0].x, by = p[2].y - p[0].y, bz = p[2].z - p[0].z;
+    normalize3(ax, ay, az);
+    normalize3(bx, by, bz);
+    n.x = ay * bz - az * by;
+    n.y = az * bx - ax * bz;
+    n.z = ax * by - ay * bx;
+    normalize3(n.x, n.y, n.z);
+    if (fn_out) {
+      fn_out[int64_t(f) * 3 + 0] = n.x;
+      fn_out[int64_t(f) * 3 + 1] = n.y;
+      fn_out[int64_t(f) * 3 + 2] = n.z;
+    }
+  } else if (fnormals) {
+    n = make_float4(fnormals[int64_t(f) * 3 + 0], fnormals[int64_t(f) * 3 + 1],
+                    fnormals[int64_t(f) * 3 + 2], 0.0f);
+  }
+  float4* r = soup + int64_t(f) * 4;
+  r[0] = p[0];
+  r[1] = p[1];
+  r[2] = p[2];
+  r[3] = n;
 }
 
 // ---------------------------------------------------------------------------
 // k_setup_bin: one thread per (face, view)
 
-// Appends fid to bin `tile` for every lane with `has`; lanes that target the
-// same tile share one atomicAdd (ballot + popcount rank).
-UGU_DEV void wave_append(const DevBins& bins, int tile, int fid, bool has) {
+// Appends fid to a bin segment for every lane with `has`. `seg` = 2 * tile
+// for the front segment (filled upwards from the bin's first slot) and
+// 2 * tile + 1 for the back segment (filled downwards from its last slot).
+// Lanes that target the same segment share one atomicAdd (ballot + popcount
+// rank). When the two segments together overflow the bin they overwrite each
+// other's slots; every slot the rasterizer reads still holds a valid face id,
+// and the host renders again with larger bins.
+UGU_DEV void wave_append(const DevBins& bins, int seg, int fid, bool has) {
   uint64_t todo = __ballot(has);
   const int lane = __lane_id();
   while (todo) {
     const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
-    const int lt = __builtin_amdgcn_readlane(tile, leader);
-    const bool mine = has && tile == lt;
+    const int ls = __builtin_amdgcn_readlane(seg, leader);
+    const bool mine = has && seg == ls;
     const uint64_t grp = __ballot(mine);
     int base = 0;
-    if (lane == leader) base = atomicAdd(&bins.counts[lt], __popcll(grp));
+    if (lane == leader) base = atomicAdd(&bins.counts[ls], __popcll(grp));
     base = __builtin_amdgcn_readlane(base, leader);
     if (mine) {
       const int slot = base + __popcll(grp & ((1ull << lane) - 1ull));
-      if (slot < bins.cap) bins.bins[int64_t(lt) * bins.cap + slot] = fid;
+      if (slot < bins.cap) {
+        const int64_t row = int64_t(ls >> 1) * bins.cap;
+        bins.bins[row + ((ls & 1) ? bins.cap - 1 - slot : slot)] = fid;
+      }
     }
     todo &= ~grp;
   }
+}
+
+// Face id of entry i of a tile's bin: the front segment's nA entries, then the
+// back segment read from the bin's last slot downwards.
+UGU_DEV int bin_entry(const int32_t* bin, int nA, int cap, int i) {
+  return i < nA ? bin[i] : bin[cap - 1 - (i - nA)];
 }
 
 // Views per thread: the face's indices are loaded once and reused for
@@ -298,19 +336,19 @@ __global__ __launch_bounds__(256) void k_setup_bin(
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   float P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (f < mesh.F) {
-    const int32_t* fi = mesh.faces + int64_t(f) * 3;
+    const float4* r = mesh.soup + int64_t(f) * 4;
     for (int k = 0; k < 3; k++) {
-      const float* p = mesh.verts + int64_t(fi[k]) * 3;
-      P[k * 3 + 0] = p[0];
-      P[k * 3 + 1] = p[1];
-      P[k * 3 + 2] = p[2];
+      const float4 q = r[k];
+      P[k * 3 + 0] = q.x;
+      P[k * 3 + 1] = q.y;
+      P[k * 3 + 2] = q.z;
     }
   }
   const int vb = blockIdx.y * kSetupViews;
   const int ve = min(vb + kSetupViews, n_views);
   for (int vi = vb; vi < ve; vi++) {
     const DevView& v = views[vi];
-    bool has = false;
+    bool has = false, back = false;
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
     if (f < mesh.F) {
       const float4 a = project_point(v, P[0], P[1], P[2]);
@@ -320,6 +358,9 @@ __global__ __launch_bounds__(256) void k_setup_bin(
       float area;
       if (face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) {
         has = true;
+        // faces that are clockwise on screen are mostly hidden back faces:
+        // they go to the back segment so the rasterizer sees them last
+        back = !(area > 0.0f);
         tx0 = x0 / TW;
         tx1 = x1 / TW;
         ty0 = y0 / TH;
@@ -328,8 +369,8 @@ __global__ __launch_bounds__(256) void k_setup_bin(
     }
     int tx = tx0, ty = ty0;
     while (__any(has)) {
-      const int tile = has ? v.tile_base + ty * v.tiles_x + tx : -1;
-      wave_append(bins, tile, f, has);
+      const int seg = has ? 2 * (v.tile_base + ty * v.tiles_x + tx) + (back ? 1 : 0) : -1;
+      wave_append(bins, seg, f, has);
       if (has) {
         if (++tx > tx1) {
           tx = tx0;
@@ -354,6 +395,7 @@ struct RasterLds {
   int32_t pref[NT + 1];        // exclusive prefix of pair counts
   int32_t wsum[NT / 64 + 1];
   int32_t flag[NT];            // pixel saw a depth outside [FLT_MIN, inf]
+  uint32_t wmax[NT / 64];       // per-wave max of the pixels' depth bits
   int32_t any_flag;
   int32_t item;
 };
@@ -364,7 +406,7 @@ struct RasterLds {
 // for pixels where some covering face produced a depth below FLT_MIN or NaN,
 // where the packed atomicMin order is not the sequential order.
 UGU_DEV void sequential_pixel(const DevView& v, const DevMesh& mesh,
-                              const float4* pv, const int32_t* bin, int n,
+                              const int32_t* bin, int nA, int cap, int n,
                               int gx, int gy, float r0, float r1, float r2,
                               int& out_fid) {
   int cur = -1;
@@ -375,17 +417,18 @@ UGU_DEV void sequential_pixel(const DevView& v, const DevMesh& mesh,
     int best = INT_MAX;
     float bestz = 0.0f;
     for (int i = 0; i < n; i++) {
-      const int f = bin[i];
+      const int f = bin_entry(bin, nA, cap, i);
       if (f <= cur || f >= best) continue;
-      const int32_t* fi = mesh.faces + int64_t(f) * 3;
-      const float4 a = project_vertex(v, mesh.verts, fi[0]);
-      const float4 b = project_vertex(v, mesh.verts, fi[1]);
-      const float4 c = project_vertex(v, mesh.verts, fi[2]);
+      const float4* rec = mesh.soup + int64_t(f) * 4;
+      const float4 a = project_soup(v, rec[0]);
+      const float4 b = project_soup(v, rec[1]);
+      const float4 c = project_soup(v, rec[2]);
       int x0, x1, y0, y1;
       float area;
       if (!face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) continue;
       if (gx < x0 || gx > x1 || gy < y0 || gy > y1) continue;
-      const bool bf = backface_of(mesh.fnormals, f, r0, r1, r2);
+      const float4 fn = rec[3];
+      const bool bf = mesh.fnormals != nullptr && sum3(fn.x * r0, fn.y * r1, fn.z * r2) > 0.0f;
       float z, w0, w1, w2;
       if (cover_test(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, area, bf, x,
                      y, z, w0, w1, w2)) {
@@ -595,6 +638,26 @@ UGU_DEV void diff_eval(const DevView& v, const DevMesh& m, int fid, int px, int 
   }
 }
 
+// True when every depth the coverage test could compute for this face at a
+// pixel of its bbox exceeds dmax (so no pair of the face can change a key or
+// set the slow-path flag). Bound: with L the larger bbox extent, rounding
+// moves each edge function by at most 8u L^2 (u = 2^-24) and the area by as
+// much, so the normalised weights sum to at most 1 + 32u L^2 / |area|; with
+// 2^-9 L^2 <= |area| that is 1 + 2^-10, five more roundings cost < 2^-21, and
+// z >= min(za, zb, zc) (1 - 2^-10 - 2^-20). Requiring
+// min z > dmax (1 + 2^-8) leaves a wide margin. NaN or huge values fail
+// every comparison, so such faces are never skipped.
+UGU_DEV bool hidden_face(const float4& a, const float4& b, const float4& c, float area,
+                         float dmax) {
+  const float ex = max3_std(a.x, b.x, c.x) - min3_std(a.x, b.x, c.x);
+  const float ey = max3_std(a.y, b.y, c.y) - min3_std(a.y, b.y, c.y);
+  if (!(ex <= 1048576.0f && ey <= 1048576.0f)) return false;
+  const float L = ex > ey ? ex : ey;
+  if (!(L * L * 0.001953125f <= fabsf(area))) return false;
+  const float zlim = dmax * 1.00390625f;
+  return dmax >= 1e-18f && a.z > zlim && b.z > zlim && c.z > zlim;
+}
+
 template <int TW, int TH, int MODE>
 UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
                          const DevMesh& mesh, const float4* proj,
@@ -602,13 +665,13 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   constexpr int NT = TW * TH;
   const int t = threadIdx.x;
 
-  const int count = bins.counts[gtile];
+  const int2 seg = reinterpret_cast<const int2*>(bins.counts)[gtile];
+  const int count = seg.x + seg.y;
   const DevView& v = views[bins.tile_view[gtile]];
   const int lt = gtile - v.tile_base;
   const int tyi = lt / v.tiles_x;
   const int txi = lt - tyi * v.tiles_x;
   const int px0 = txi * TW, py0 = tyi * TH;
-  const float4* pv = nullptr;
   (void)proj;
 
   // this thread's pixel
@@ -647,34 +710,49 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   s.ray[2][t] = r2;
   if (t == 0) s.any_flag = 0;
 
-  const int n = count < bins.cap ? count : bins.cap;
-  // When the whole face list fits one LDS chunk and face ids fit 24 bits,
-  // the depth key also carries the LDS slot of the face (below the id, so
-  // ties still go to the lowest id) and the resolve reads the winner's
-  // staged record instead of gathering it again.
-  const bool slot_key = n <= NT && mesh.F <= (1 << 24);
+  // front segment first, then the back segment (both clipped to the bin)
+  const int nA = seg.x < bins.cap ? seg.x : bins.cap;
+  const int nB = seg.y < bins.cap - nA ? seg.y : bins.cap - nA;
+  const int n = nA + nB;
+  // When face ids fit 24 bits the depth key also carries the LDS slot of the
+  // face (below the id, so ties still go to the lowest id). After every chunk
+  // each thread copies the staged record of its pixel's current winner into
+  // registers when the winner changed, so the resolve needs no second gather
+  // and projection of the winning face.
+  static_assert(NT <= 256, "slot keys hold 8-bit LDS slots");
+  const bool slot_key = mesh.F <= (1 << 24);
+  unsigned long long seen = ~0ull;  // key whose record the registers hold
+  float4 wa = make_float4(0, 0, 0, 0), wb = wa, wc = wa;
+  float warea = 0.0f, wn0 = 0.0f, wn1 = 0.0f, wn2 = 0.0f;
+  int wv0 = 0, wv1 = 0, wv2 = 0;
   // overflowing tiles report their true occupancy; the host re-renders with
   // bins that large (one same-address atomic per overflowing tile only)
   if (t == 0 && count > bins.cap) atomicMax(&bins.status[0], count);
   const int32_t* bin = bins.bins + int64_t(gtile) * bins.cap;
   __syncthreads();
 
+  // Largest depth any in-view pixel of the tile holds after the previous
+  // chunks (NaN while some pixel has none). Keys only decrease, so a face
+  // whose depth provably exceeds it at every pixel can neither win a pixel
+  // nor produce a depth below FLT_MIN: its pairs are skipped.
+  float dmax = __uint_as_float(0xffffffffu);
   for (int base = 0; base < n; base += NT) {
     // stage one chunk of faces
     int c = 0;
     const int i = base + t;
     if (i < n) {
-      const int f = bin[i];
-      const int32_t* fi = mesh.faces + int64_t(f) * 3;
-      const float4 a = project_vertex(v, mesh.verts, fi[0]);
-      const float4 b = project_vertex(v, mesh.verts, fi[1]);
-      const float4 cc = project_vertex(v, mesh.verts, fi[2]);
+      const int f = bin_entry(bin, nA, bins.cap, i);
+      const float4* rec = mesh.soup + int64_t(f) * 4;
+      const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], fn = rec[3];
+      const float4 a = project_soup(v, q0);
+      const float4 b = project_soup(v, q1);
+      const float4 cc = project_soup(v, q2);
       int x0, x1, y0, y1;
       float area;
       if (face_setup(a, b, cc, v.W, v.H, x0, x1, y0, y1, area)) {
         const int lx0 = max(x0, px0) - px0, lx1 = min(x1, px0 + TW - 1) - px0;
         const int ly0 = max(y0, py0) - py0, ly1 = min(y1, py0 + TH - 1) - py0;
-        if (lx0 <= lx1 && ly0 <= ly1) {
+        if (lx0 <= lx1 && ly0 <= ly1 && !hidden_face(a, b, cc, area, dmax)) {
           const int nx = lx1 - lx0 + 1;
           c = nx * (ly1 - ly0 + 1);
           s.box[t] = lx0 | (ly0 << 8) | (nx << 16);
@@ -684,14 +762,11 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
       s.fv[3][t] = b.x; s.fv[4][t] = b.y; s.fv[5][t] = b.z;
       s.fv[6][t] = cc.x; s.fv[7][t] = cc.y; s.fv[8][t] = cc.z;
       s.fv[9][t] = area;
-      if (mesh.fnormals) {
-        const float* fn = mesh.fnormals + int64_t(f) * 3;
-        s.fv[10][t] = fn[0]; s.fv[11][t] = fn[1]; s.fv[12][t] = fn[2];
-      }
+      s.fv[10][t] = fn.x; s.fv[11][t] = fn.y; s.fv[12][t] = fn.z;
       s.fid[t] = f;
-      s.vid[0][t] = fi[0];
-      s.vid[1][t] = fi[1];
-      s.vid[2][t] = fi[2];
+      s.vid[0][t] = __float_as_int(q0.w);
+      s.vid[1][t] = __float_as_int(q1.w);
+      s.vid[2][t] = __float_as_int(q2.w);
     }
     // block exclusive scan of the pair counts
     {
@@ -746,39 +821,52 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
       }
     }
     __syncthreads();
-  }
-
-  // ---- resolve (rasterizer.cc:243-301) for this thread's pixel
-  int fid = -1, slot = -1;
-  if (s.any_flag && s.flag[t]) {
-    sequential_pixel(v, mesh, pv, bin, n, gx, gy, r0, r1, r2, fid);
-  } else {
-    const unsigned long long key = s.key[t];
-    if (key != ~0ull) {
-      const unsigned int low = static_cast<unsigned int>(key & 0xffffffffull);
-      if (slot_key) {
-        fid = static_cast<int>(low >> 8);
-        slot = static_cast<int>(low & 0xffu);
-      } else {
-        fid = static_cast<int>(low);
+    const unsigned long long k = s.key[t];
+    if (base + NT < n) {
+      // depth bits of this pixel: all ones when it has no key yet, 0 outside
+      // the image (no face reaches those pixels)
+      uint32_t db = (gx < v.W && gy < v.H) ? static_cast<uint32_t>(k >> 32) : 0u;
+      for (int off = 32; off > 0; off >>= 1) db = max(db, static_cast<uint32_t>(__shfl_xor(static_cast<int>(db), off)));
+      if ((t & 63) == 0) s.wmax[t >> 6] = db;
+    }
+    if (slot_key) {
+      if (k != seen) {
+        const int sl = static_cast<int>(k & 0xffull);
+        wa = make_float4(s.fv[0][sl], s.fv[1][sl], s.fv[2][sl], 0.0f);
+        wb = make_float4(s.fv[3][sl], s.fv[4][sl], s.fv[5][sl], 0.0f);
+        wc = make_float4(s.fv[6][sl], s.fv[7][sl], s.fv[8][sl], 0.0f);
+        warea = s.fv[9][sl];
+        wn0 = s.fv[10][sl];
+        wn1 = s.fv[11][sl];
+        wn2 = s.fv[12][sl];
+        wv0 = s.vid[0][sl];
+        wv1 = s.vid[1][sl];
+        wv2 = s.vid[2][sl];
+        seen = k;
       }
     }
+    if (base + NT < n) {
+      // the next chunk's staging overwrites the records read above
+      __syncthreads();
+      uint32_t m = s.wmax[0];
+      for (int w = 1; w < NT / 64; w++) m = max(m, s.wmax[w]);
+      dmax = __uint_as_float(m);
+    }
   }
-  // the winner's projected vertices, area, face normal and vertex ids
-  float4 wa = make_float4(0, 0, 0, 0), wb = wa, wc = wa;
-  float warea = 0.0f, wn0 = 0.0f, wn1 = 0.0f, wn2 = 0.0f;
-  int wv0 = 0, wv1 = 0, wv2 = 0;
-  if (slot >= 0) {
-    wa = make_float4(s.fv[0][slot], s.fv[1][slot], s.fv[2][slot], 0.0f);
-    wb = make_float4(s.fv[3][slot], s.fv[4][slot], s.fv[5][slot], 0.0f);
-    wc = make_float4(s.fv[6][slot], s.fv[7][slot], s.fv[8][slot], 0.0f);
-    warea = s.fv[9][slot];
-    wn0 = s.fv[10][slot];
-    wn1 = s.fv[11][slot];
-    wn2 = s.fv[12][slot];
-    wv0 = s.vid[0][slot];
-    wv1 = s.vid[1][slot];
-    wv2 = s.vid[2][slot];
+
+  // ---- resolve (rasterizer.cc:243-301) for this thread's pixel; the
+  // winner's projected vertices, area, face normal and vertex ids are in
+  // registers (slot keys) or gathered again below
+  int fid = -1;
+  bool have = false;
+  if (s.any_flag && s.flag[t]) {
+    sequential_pixel(v, mesh, bin, nA, bins.cap, n, gx, gy, r0, r1, r2, fid);
+  } else if (seen != ~0ull) {
+    fid = static_cast<int>(static_cast<unsigned int>(seen & 0xffffffffull) >> 8);
+    have = true;
+  } else if (!slot_key) {
+    const unsigned long long key = s.key[t];
+    if (key != ~0ull) fid = static_cast<int>(key & 0xffffffffull);
   }
   __syncthreads();  // LDS is reused by the next tile of a persistent block
   if (gx >= v.W || gy >= v.H) return;
@@ -786,21 +874,19 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   float z = 0.0f, w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
   bool bf = false;
   if (fid >= 0) {
-    if (slot < 0) {
-      const int32_t* fi = mesh.faces + int64_t(fid) * 3;
-      wv0 = fi[0];
-      wv1 = fi[1];
-      wv2 = fi[2];
-      wa = project_vertex(v, mesh.verts, wv0);
-      wb = project_vertex(v, mesh.verts, wv1);
-      wc = project_vertex(v, mesh.verts, wv2);
+    if (!have) {
+      const float4* rec = mesh.soup + int64_t(fid) * 4;
+      const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], fn = rec[3];
+      wv0 = __float_as_int(q0.w);
+      wv1 = __float_as_int(q1.w);
+      wv2 = __float_as_int(q2.w);
+      wa = project_soup(v, q0);
+      wb = project_soup(v, q1);
+      wc = project_soup(v, q2);
       warea = edge_fn(wa.x, wa.y, wb.x, wb.y, wc.x, wc.y);
-      if (mesh.fnormals) {
-        const float* fn = mesh.fnormals + int64_t(fid) * 3;
-        wn0 = fn[0];
-        wn1 = fn[1];
-        wn2 = fn[2];
-      }
+      wn0 = fn.x;
+      wn1 = fn.y;
+      wn2 = fn.z;
     }
     bf = mesh.fnormals != nullptr && sum3(wn0 * r0, wn1 * r1, wn2 * r2) > 0.0f;
     cover_test(wa.x, wa.y, wa.z, wb.x, wb.y, wb.z, wc.x, wc.y, wc.z, warea, bf,
@@ -896,10 +982,118 @@ __global__ __launch_bounds__(TW * TH) void k_raster(
 }
 
 // ---------------------------------------------------------------------------
-// k_backward: dL/d(vertices, vertex colours, texture) at fixed visibility.
-// One thread per pixel (row-major, so a wavefront is 64 neighbouring pixels);
-// contributions of contiguous lanes that share a face (or a texel quad) are
-// summed across the wavefront before the atomics.
+// Differentiable backward pass: dL/d(vertices, vertex colours, texture) at
+// fixed visibility, in three kernels.
+//   k_bwd_collect_faces   one thread per face: loops over the views, finds the
+//                         pixels of the face's pixel bbox whose face id is the
+//                         face and appends them as (face, view, pixel) items,
+//                         every face's items contiguous (faces with large
+//                         bboxes are left to the next kernel)
+//   k_bwd_collect_pixels  one thread per pixel: appends the pixels of faces
+//                         with large bboxes (runs of neighbouring pixels)
+//   k_bwd_items           one thread per item: shading derivatives in float64;
+//                         contributions of contiguous items of one face are
+//                         summed across the wavefront (typically all the
+//                         face's pixels in every view), then one atomic per
+//                         vertex component and run
+// The items are grouped by face so a face's gradient becomes a handful of
+// atomics instead of one set per pixel.
+
+// Faces whose pixel bbox in a view has more pixels than this are collected
+// per pixel.
+constexpr int kBwdFaceBoxMax = 64;
+
+// The reference face setup of a face in a view (same arithmetic as the
+// rasterizer, so the bbox contains every pixel the face won) and whether the
+// per-face collector handles it.
+UGU_DEV bool bwd_small_face(const DevView& v, const float* P, int& x0, int& x1, int& y0,
+                            int& y1) {
+  const float4 a = project_point(v, P[0], P[1], P[2]);
+  const float4 b = project_point(v, P[3], P[4], P[5]);
+  const float4 c = project_point(v, P[6], P[7], P[8]);
+  float area;
+  if (!face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) return false;
+  return int64_t(x1 - x0 + 1) * (y1 - y0 + 1) <= kBwdFaceBoxMax;
+}
+
+UGU_DEV void load_face_verts(const DevMesh& m, int f, float P[9]) {
+  const int32_t* fi = m.faces + int64_t(f) * 3;
+  for (int k = 0; k < 3; k++) {
+    const float* p = m.verts + int64_t(fi[k]) * 3;
+    P[k * 3 + 0] = p[0];
+    P[k * 3 + 1] = p[1];
+    P[k * 3 + 2] = p[2];
+  }
+}
+
+__global__ void k_zero_i32(int32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_bwd_collect_faces(
+    const DevView* __restrict__ views, int n_views, DevMesh mesh,
+    int4* __restrict__ items, int32_t* __restrict__ cursor) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  // face 0 is never shaded (rasterizer.cc:250), so it has no gradient
+  const bool live = f > 0 && f < mesh.F;
+  float P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (live) load_face_verts(mesh, f, P);
+  int count = 0;
+  for (int vi = 0; vi < n_views && live; vi++) {
+    const DevView& v = views[vi];
+    int x0, x1, y0, y1;
+    if (!bwd_small_face(v, P, x0, x1, y0, y1)) continue;
+    for (int y = y0; y <= y1; y++)
+      for (int x = x0; x <= x1; x++) count += v.face_id[int64_t(y) * v.W + x] == f;
+  }
+  // one cursor atomic per wavefront; lanes take consecutive ranges
+  const int lane = __lane_id();
+  int incl = count;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int o = __shfl_up(incl, off);
+    if (lane >= off) incl += o;
+  }
+  const int wave_total = __shfl(incl, 63);
+  int base = 0;
+  if (lane == 0 && wave_total > 0) base = atomicAdd(cursor, wave_total);
+  base = __shfl(base, 0);
+  if (count == 0) return;
+  int pos = base + incl - count;
+  for (int vi = 0; vi < n_views; vi++) {
+    const DevView& v = views[vi];
+    int x0, x1, y0, y1;
+    if (!bwd_small_face(v, P, x0, x1, y0, y1)) continue;
+    for (int y = y0; y <= y1; y++)
+      for (int x = x0; x <= x1; x++) {
+        const int px = y * v.W + x;
+        if (v.face_id[px] == f) items[pos++] = make_int4(f, vi, px, 0);
+      }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bwd_collect_pixels(
+    const DevView* __restrict__ views, DevMesh mesh, int4* __restrict__ items,
+    int32_t* __restrict__ cursor) {
+  const DevView& v = views[blockIdx.y];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int fid = p < v.W * v.H ? v.face_id[p] : -1;
+  bool emit = false;
+  if (fid > 0 && fid < mesh.F) {
+    float P[9];
+    load_face_verts(mesh, fid, P);
+    int x0, x1, y0, y1;
+    emit = !bwd_small_face(v, P, x0, x1, y0, y1);
+  }
+  const uint64_t m = __ballot(emit);
+  if (m == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(cursor, __popcll(m));
+  base = __shfl(base, leader);
+  if (emit) items[base + __popcll(m & ((1ull << lane) - 1ull))] = make_int4(fid, blockIdx.y, p, 0);
+}
 
 // Sums `val` over contiguous runs of equal `run` ids; the run's first lane
 // ends up with the run total.
@@ -913,16 +1107,30 @@ UGU_DEV float run_sum(float val, int run) {
   return val;
 }
 
-__global__ __launch_bounds__(256) void k_backward(
-    const DevView* __restrict__ views, DevMesh mesh,
-    const float4* __restrict__ proj, DevGrads g) {
-  (void)proj;
-  const DevView& v = views[blockIdx.y];
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int npx = v.W * v.H;
-  int fid = -1;
-  if (p < npx) fid = v.face_id[p];
-  const bool valid = fid > 0;
+// Texel gradients of a block's items are first summed in an LDS hash table
+// (open addressing on the texel index, linear probing); a block's items cover
+// a few faces in every view that sees them, so many items hit the same texels.
+// Texels that find no free slot go straight to global atomics.
+constexpr int kTexSlots = 2048;
+constexpr int kTexProbes = 8;
+
+UGU_DEV int tex_slot(int key) {
+  return static_cast<int>((static_cast<uint32_t>(key) * 2654435761u) >> 21);  // 11 bits
+}
+
+__global__ __launch_bounds__(256) void k_bwd_items(
+    const DevView* __restrict__ views, DevMesh mesh, const int4* __restrict__ items,
+    const int32_t* __restrict__ cursor, DevGrads g) {
+  static_assert(kTexSlots == 2048, "tex_slot returns 11 bits");
+  __shared__ int32_t hkey[kTexSlots];
+  __shared__ float hval[3][kTexSlots];
+  const int n_items = *cursor;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (int64_t(blockIdx.x) * blockDim.x >= n_items) return;  // whole block idle
+  const bool valid = i < n_items;
+  int4 item = make_int4(-1, 0, 0, 0);
+  if (valid) item = items[i];
+  const int fid = item.x;
 
   float gcol[9], gpos[9], gtex[12];
   for (int k = 0; k < 9; k++) { gcol[k] = 0.0f; gpos[k] = 0.0f; }
@@ -931,6 +1139,8 @@ __global__ __launch_bounds__(256) void k_backward(
   int vid[3] = {0, 0, 0};
 
   if (valid) {
+    const DevView& v = views[item.y];
+    const int p = item.z;
     const int px = p % v.W, py = p / v.W;
     const double G[3] = {v.grad[int64_t(p) * 3 + 0], v.grad[int64_t(p) * 3 + 1],
                          v.grad[int64_t(p) * 3 + 2]};
@@ -945,7 +1155,7 @@ __global__ __launch_bounds__(256) void k_backward(
     for (int k = 0; k < 3; k++) {
       const double gc = G[k] * e.t[k];
       gt[k] = G[k] * e.c[k];
-      for (int i = 0; i < 3; i++) gcol[i * 3 + k] = static_cast<float>(gc * beta[i]);
+      for (int j = 0; j < 3; j++) gcol[j * 3 + k] = static_cast<float>(gc * beta[j]);
       gu += gc * e.dcdu[k] + gt[k] * e.dtdu[k];
       gv += gc * e.dcdv[k] + gt[k] * e.dtdv[k];
     }
@@ -986,14 +1196,14 @@ __global__ __launch_bounds__(256) void k_backward(
       gX[2] += gA * (e.Y[1] - e.Y[0]);
       gY[2] += gA * (e.X[0] - e.X[1]);
       // X = fx Qx / Qz + cx, Y = fy Qy / Qz + cy, Z = Qz; Q = R P + t
-      for (int i = 0; i < 3; i++) {
-        const double iz = 1.0 / e.Z[i];
-        const double gQx = gX[i] * v.fx * iz;
-        const double gQy = gY[i] * v.fy * iz;
-        const double gQz = gZ[i] - gX[i] * v.fx * e.Qx[i] * iz * iz -
-                           gY[i] * v.fy * e.Qy[i] * iz * iz;
+      for (int j = 0; j < 3; j++) {
+        const double iz = 1.0 / e.Z[j];
+        const double gQx = gX[j] * v.fx * iz;
+        const double gQy = gY[j] * v.fy * iz;
+        const double gQz = gZ[j] - gX[j] * v.fx * e.Qx[j] * iz * iz -
+                           gY[j] * v.fy * e.Qy[j] * iz * iz;
         for (int cdim = 0; cdim < 3; cdim++)
-          gpos[i * 3 + cdim] = static_cast<float>(double(v.w2c_R[0 * 3 + cdim]) * gQx +
+          gpos[j * 3 + cdim] = static_cast<float>(double(v.w2c_R[0 * 3 + cdim]) * gQx +
                                                   double(v.w2c_R[1 * 3 + cdim]) * gQy +
                                                   double(v.w2c_R[2 * 3 + cdim]) * gQz);
       }
@@ -1020,23 +1230,37 @@ __global__ __launch_bounds__(256) void k_backward(
       if (head) unsafeAtomicAdd(&g.verts[int64_t(vid[k / 3]) * 3 + (k % 3)], val);
     }
   }
-  if (g.texture && mesh.ftex) {
-    // lanes share a run only when all four texels agree (the +1 neighbours
-    // are clamped at the texture border)
-    const int tkey = valid ? ((ty0 * mesh.ftex_w + tx0) << 2) | (tx1 == tx0 ? 1 : 0) |
-                                 (ty1 == ty0 ? 2 : 0)
-                           : -1;
-    const int prev_t = __shfl_up(tkey, 1);
-    const bool thead = valid && (lane == 0 || prev_t != tkey);
-    const uint64_t theads = __ballot(thead);
-    const bool treduce = theads != vmask;
-    const int trun = valid ? __popcll(theads & ((2ull << lane) - 1ull)) : -1 - lane;
-    const int64_t tw = mesh.ftex_w;
-    const int64_t tidx[4] = {int64_t(ty0) * tw + tx0, int64_t(ty0) * tw + tx1,
-                             int64_t(ty1) * tw + tx0, int64_t(ty1) * tw + tx1};
-    for (int k = 0; k < 12; k++) {
-      const float val = treduce ? run_sum(gtex[k], trun) : gtex[k];
-      if (thead) unsafeAtomicAdd(&g.texture[tidx[k / 3] * 3 + (k % 3)], val);
+  if (g.texture && mesh.ftex) {  // uniform over the block
+    for (int j = threadIdx.x; j < kTexSlots; j += blockDim.x) {
+      hkey[j] = -1;
+      hval[0][j] = hval[1][j] = hval[2][j] = 0.0f;
+    }
+    __syncthreads();
+    if (valid) {
+      const int tw = mesh.ftex_w;
+      const int tidx[4] = {ty0 * tw + tx0, ty0 * tw + tx1, ty1 * tw + tx0, ty1 * tw + tx1};
+      for (int q = 0; q < 4; q++) {
+        const int key = tidx[q];
+        int h = tex_slot(key);
+        bool done = false;
+        for (int probe = 0; probe < kTexProbes && !done; probe++) {
+          const int old = atomicCAS(&hkey[h], -1, key);
+          if (old == -1 || old == key) {
+            for (int k = 0; k < 3; k++) atomicAdd(&hval[k][h], gtex[q * 3 + k]);
+            done = true;
+          }
+          h = (h + 1) & (kTexSlots - 1);
+        }
+        if (!done)
+          for (int k = 0; k < 3; k++)
+            unsafeAtomicAdd(&g.texture[int64_t(key) * 3 + k], gtex[q * 3 + k]);
+      }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kTexSlots; j += blockDim.x) {
+      const int key = hkey[j];
+      if (key >= 0)
+        for (int k = 0; k < 3; k++) unsafeAtomicAdd(&g.texture[int64_t(key) * 3 + k], hval[k][j]);
     }
   }
 }
@@ -1071,11 +1295,12 @@ int launch_project(hipStream_t st, const DevView* views, int n_views,
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_face_normals(hipStream_t st, const float* verts, const int32_t* faces,
-                        int32_t F, float* fn) {
+int launch_face_soup(hipStream_t st, const float* verts, const int32_t* faces,
+                     const float* fnormals, int32_t F, int compute_normals, float4* soup,
+                     float* fn_out) {
   if (F <= 0) return 0;
-  hipLaunchKernelGGL(k_face_normals, dim3((F + 255) / 256), dim3(256), 0, st,
-                     verts, faces, F, fn);
+  hipLaunchKernelGGL(k_face_soup, dim3((F + 255) / 256), dim3(256), 0, st, verts, faces,
+                     fnormals, F, compute_normals, soup, fn_out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1143,11 +1368,18 @@ int raster_blocks_per_cu(int tw, int th, int mode) {
 }
 
 int launch_backward(hipStream_t st, const DevView* views, int n_views,
-                    int max_pixels, const DevMesh& mesh, const float4* proj,
-                    const DevGrads& g) {
-  if (max_pixels <= 0) return 0;
-  hipLaunchKernelGGL(k_backward, dim3((max_pixels + 255) / 256, n_views), dim3(256), 0,
-                     st, views, mesh, proj, g);
+                    int max_pixels, int64_t total_pixels, const DevMesh& mesh,
+                    const DevGrads& g, int4* items, int32_t* cursor) {
+  if (max_pixels <= 0 || total_pixels <= 0) return 0;
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, cursor, 1);
+  if (mesh.F > 1)
+    hipLaunchKernelGGL(k_bwd_collect_faces, dim3((mesh.F + 255) / 256), dim3(256), 0, st,
+                       views, n_views, mesh, items, cursor);
+  hipLaunchKernelGGL(k_bwd_collect_pixels, dim3((max_pixels + 255) / 256, n_views), dim3(256),
+                     0, st, views, mesh, items, cursor);
+  // at most one item per pixel; blocks past the item count exit at once
+  hipLaunchKernelGGL(k_bwd_items, dim3(static_cast<unsigned>((total_pixels + 255) / 256)),
+                     dim3(256), 0, st, views, mesh, items, cursor, g);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -45,6 +45,9 @@ struct DevMesh {
   const int32_t* matid;     // F or null
   const TexInfo* tex;       // per material
   const uint8_t* texdata;   // u8 RGB texture pool
+  const float4* soup;       // F x 4: the face's three vertex positions (.w =
+                            // vertex id bits) and its face normal (zero when
+                            // the mesh has none)
   const float* ftex;        // differentiable path: f32 RGB texture or null
   int32_t ftex_w, ftex_h;
 };
@@ -59,8 +62,9 @@ struct DevOptions {
 };
 
 struct DevBins {
-  int32_t* counts;   // one counter per global tile
-  int32_t* bins;     // global tile t owns bins[t*cap .. t*cap+cap)
+  int32_t* counts;   // per global tile: front / back segment counters
+  int32_t* bins;     // global tile t owns bins[t*cap .. t*cap+cap): front
+                     // segment from the start, back segment from the end
   int32_t cap;
   int32_t total_tiles;
   int32_t* status;   // [0] max bin count seen, [1] work-queue head

@@ -21,8 +21,9 @@ int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status,
                  const int32_t* tile_base, int32_t n_views, int32_t* tile_view);
 int launch_project(hipStream_t st, const DevView* views, int n_views,
                    const float* verts, int32_t V, float4* proj);
-int launch_face_normals(hipStream_t st, const float* verts, const int32_t* faces,
-                        int32_t F, float* fn);
+int launch_face_soup(hipStream_t st, const float* verts, const int32_t* faces,
+                     const float* fnormals, int32_t F, int compute_normals, float4* soup,
+                     float* fn_out);
 int launch_setup_bin(hipStream_t st, int tw, int th, const DevView* views,
                      int n_views, const DevMesh& mesh, const float4* proj,
                      const DevBins& bins);
@@ -31,8 +32,8 @@ int launch_raster(hipStream_t st, int tw, int th, int mode, const DevView* views
                   const DevOptions& opt, int persistent_blocks);
 int raster_blocks_per_cu(int tw, int th, int mode);
 int launch_backward(hipStream_t st, const DevView* views, int n_views,
-                    int max_pixels, const DevMesh& mesh, const float4* proj,
-                    const DevGrads& g);
+                    int max_pixels, int64_t total_pixels, const DevMesh& mesh,
+                    const DevGrads& g, int4* items, int32_t* cursor);
 int launch_depth_to_u16(hipStream_t st, const float* d, int64_t n, uint16_t* out);
 }  // namespace ugu_hip
 
@@ -105,6 +106,8 @@ struct ugu_hip_ctx {
 
   // work buffers
   DevBuf proj, counts, tile_view, bins, status, views, tile_base, stage_out, diff_fn;
+  DevBuf bwd_items;  // (face, view, pixel) items of the backward pass, 16 B each
+  DevBuf soup, diff_soup;  // 64-byte face records (prepared mesh / differentiable path)
   int64_t buf_gen = 0;
   void* h_views = nullptr;
   size_t h_views_bytes = 0;
@@ -288,6 +291,7 @@ DevMesh dev_mesh(const ugu_hip_ctx* c) {
   m.verts = c->verts.as<float>();
   m.faces = c->faces.as<int32_t>();
   m.fnormals = c->has_fn ? c->fnormals.as<float>() : nullptr;
+  m.soup = c->soup.as<float4>();
   m.normals = c->has_normals ? c->normals.as<float>() : nullptr;
   m.nidx = c->has_nidx ? c->nidx.as<int32_t>() : nullptr;
   m.vcolors = c->has_vc ? c->vcolors.as<float>() : nullptr;
@@ -342,7 +346,7 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
 
   int err;
   if ((err = ensure(c, c->proj, 16))) return err;
-  if ((err = ensure(c, c->counts, size_t(total_tiles) * 4))) return err;
+  if ((err = ensure(c, c->counts, size_t(total_tiles) * 8))) return err;  // 2 per tile
   if ((err = ensure(c, c->tile_view, size_t(total_tiles) * 4))) return err;
   if ((err = ensure(c, c->status, 64))) return err;
   if ((err = ensure(c, c->views, size_t(n) * sizeof(DevView)))) return err;
@@ -586,7 +590,8 @@ void ugu_hip_destroy(ugu_hip_ctx* c) {
   DevBuf* bufs[] = {&c->verts, &c->faces, &c->faces_orig, &c->fnormals, &c->normals, &c->nidx,
                     &c->vcolors, &c->uv, &c->uvidx, &c->matid, &c->texinfo,
                     &c->texdata, &c->proj, &c->counts, &c->bins, &c->status,
-                    &c->views, &c->tile_base, &c->stage_out, &c->diff_fn, &c->tile_view};
+                    &c->views, &c->tile_base, &c->stage_out, &c->diff_fn, &c->tile_view,
+                    &c->bwd_items, &c->soup, &c->diff_soup};
   for (DevBuf* b : bufs) b->release();
   if (c->h_views) (void)hipHostFree(c->h_views);
   if (c->h_status) (void)hipHostFree(c->h_status);
@@ -721,6 +726,14 @@ int ugu_hip_prepare_mesh(ugu_hip_ctx* c, const UguMeshDesc* m) {
   c->has_uv = has_uv;
   c->has_uvidx = m->uv_indices != nullptr && F > 0;
   c->has_matid = m->material_ids != nullptr && F > 0;
+  if (F > 0) {
+    if ((err = ensure(c, c->soup, size_t(F) * 64))) return err;
+    if (launch_face_soup(c->stream, c->verts.as<float>(), c->faces.as<int32_t>(),
+                         c->has_fn ? c->fnormals.as<float>() : nullptr, F, 0,
+                         c->soup.as<float4>(), nullptr))
+      return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
+    UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  }
   c->mesh_gen++;
   c->prepared = true;
   return UGU_HIP_OK;
@@ -869,6 +882,8 @@ static int diff_mesh(ugu_hip_ctx* c, const UguDiffParams* p, DevMesh* dm) {
   if (!p || !p->vertices) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "vertices are required");
   if (p->texture && (!c->has_uv || !c->has_uvidx || p->tex_width < 1 || p->tex_height < 1))
     return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "texture needs uv, uv_indices and a size");
+  if (p->texture && int64_t(p->tex_width) * p->tex_height > INT32_MAX)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "texture has more than 2^31-1 texels");
   *dm = dev_mesh(c);
   dm->faces = c->faces_orig.as<int32_t>();  // parameters are in user vertex order
   dm->verts = p->vertices;
@@ -878,7 +893,9 @@ static int diff_mesh(ugu_hip_ctx* c, const UguDiffParams* p, DevMesh* dm) {
   dm->ftex_h = p->texture ? p->tex_height : 0;
   int err = ensure(c, c->diff_fn, size_t(std::max(c->F, 1)) * 12);
   if (err) return err;
+  if ((err = ensure(c, c->diff_soup, size_t(std::max(c->F, 1)) * 64))) return err;
   dm->fnormals = c->diff_fn.as<float>();
+  dm->soup = c->diff_soup.as<float4>();
   return 0;
 }
 
@@ -895,7 +912,9 @@ int ugu_hip_diff_forward(ugu_hip_ctx* c, const UguCamera* cams, int n,
   DevMesh dm;
   int err = diff_mesh(c, p, &dm);
   if (err) return err;
-  if (launch_face_normals(c->stream, dm.verts, dm.faces, dm.F, c->diff_fn.as<float>()))
+  // face records and normals from this call's positions
+  if (launch_face_soup(c->stream, dm.verts, dm.faces, nullptr, dm.F, 1, c->diff_soup.as<float4>(),
+                       c->diff_fn.as<float>()))
     return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
   UguOptions o{};
   o.backface_culling = p->backface_culling;
@@ -917,6 +936,7 @@ int ugu_hip_diff_backward(ugu_hip_ctx* c, const UguCamera* cams, int n,
   int err = diff_mesh(c, p, &dm);
   if (err) return err;
   int max_px = 0;
+  int64_t total_px = 0;
   if ((err = ensure(c, c->views, size_t(n) * sizeof(DevView)))) return err;
   const size_t view_bytes = size_t(n) * sizeof(DevView) + size_t(n + 1) * 4;
   if (c->h_views_bytes < view_bytes) {
@@ -945,7 +965,11 @@ int ugu_hip_diff_backward(ugu_hip_ctx* c, const UguCamera* cams, int n,
     v.face_id = const_cast<int32_t*>(face_ids[i]);
     v.grad = grad_images[i];
     max_px = std::max(max_px, v.W * v.H);
+    total_px += int64_t(v.W) * v.H;
   }
+  if (total_px > INT32_MAX) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "too many pixels");
+  if ((err = ensure(c, c->bwd_items, size_t(total_px) * 16))) return err;
+  if ((err = ensure(c, c->status, 64))) return err;
   UGU_HIP_CHECK(c, hipMemcpyAsync(c->views.p, c->h_views, size_t(n) * sizeof(DevView),
                                   hipMemcpyHostToDevice, c->stream));
   const DevView* dv = c->views.as<DevView>();
@@ -953,7 +977,9 @@ int ugu_hip_diff_backward(ugu_hip_ctx* c, const UguCamera* cams, int n,
   g.verts = grad_vertices;
   g.colors = p->vertex_colors ? grad_vertex_colors : nullptr;
   g.texture = p->texture ? grad_texture : nullptr;
-  if (launch_backward(c->stream, dv, n, max_px, dm, nullptr, g))
+  // item cursor: word 8 of the status block (the forward pipeline uses 0..3)
+  if (launch_backward(c->stream, dv, n, max_px, total_px, dm, g,
+                      c->bwd_items.as<int4>(), c->status.as<int32_t>() + 8))
     return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
   UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return UGU_HIP_OK;
