@@ -111,9 +111,15 @@ UGU_DEV float4 project_point(const DevView& v, float px, float py, float pz) {
   return r;
 }
 
-// Projects a vertex position of a face record (.w holds the vertex id).
-UGU_DEV float4 project_soup(const DevView& v, const float4& p) {
-  return project_point(v, p.x, p.y, p.z);
+// A face record is three float4: (x0 y0 z0 x1) (y1 z1 x2 y2) (z2 nx ny nz),
+// the face's vertex positions and its face normal.
+UGU_DEV void load_face_rec(const float4* soup, int f, float P[9], float N[3]) {
+  const float4* r = soup + int64_t(f) * 3;
+  const float4 a = r[0], b = r[1], c = r[2];
+  P[0] = a.x; P[1] = a.y; P[2] = a.z;
+  P[3] = a.w; P[4] = b.x; P[5] = b.y;
+  P[6] = b.z; P[7] = b.w; P[8] = c.x;
+  N[0] = c.y; N[1] = c.z; N[2] = c.w;
 }
 
 // Per-face setup of rasterizer.cc:164-193. Returns false when the
@@ -242,12 +248,12 @@ UGU_DEV void normalize3(float& x, float& y, float& z) {
 
 
 // ---------------------------------------------------------------------------
-// k_face_soup: one 64-byte record per face (three vertex positions with the
-// vertex ids in .w, then the face normal) so setup and raster fetch a face
-// with one dependent load instead of an index load and three vertex gathers.
-// compute_normals: Mesh::CalcFaceNormal on device (differentiable path, the
-// positions change every call; also written to fn_out); otherwise the mesh's
-// face normals are copied (zero when it has none).
+// k_face_soup: one 48-byte record per face (three vertex positions, then the
+// face normal) so setup and raster fetch a face with one dependent load
+// instead of an index load and three vertex gathers. compute_normals:
+// Mesh::CalcFaceNormal on device (differentiable path, the positions change
+// every call; also written to fn_out); otherwise the mesh's face normals are
+// copied (zero when it has none).
 
 __global__ void k_face_soup(const float* __restrict__ verts, const int32_t* __restrict__ faces,
                             const float* __restrict__ fnormals, int32_t F, int compute_normals,
@@ -255,73 +261,111 @@ __global__ void k_face_soup(const float* __restrict__ verts, const int32_t* __re
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= F) return;
   const int32_t* fi = faces + int64_t(f) * 3;
-  float4 p[3];
+  float P[9];
   for (int k = 0; k < 3; k++) {
     const float* q = verts + int64_t(fi[k]) * 3;
-    p[k] = make_float4(q[0], q[1], q[2], __int_as_float(fi[k]));
+    P[k * 3 + 0] = q[0];
+    P[k * 3 + 1] = q[1];
+    P[k * 3 + 2] = q[2];
   }
-  float4 n = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f;
   if (compute_normals) {
-    float ax = p[1].x - p[0].x, ay = p[1].y - p[0].y, az = p[1].z - p[0].z;
-    float bx = p[2].x - p[0].x, by = p[2].y - p[0].y, bz = p[2].z - p[0].z;
+    float ax = P[3] - P[0], ay = P[4] - P[1], az = P[5] - P[2];
+    float bx = P[6] - P[0], by = P[7] - P[1], bz = P[8] - P[2];
     normalize3(ax, ay, az);
     normalize3(bx, by, bz);
-    n.x = ay * bz - az * by;
-    n.y = az * bx - ax * bz;
-    n.z = ax * by - ay * bx;
-    normalize3(n.x, n.y, n.z);
+    nx = ay * bz - az * by;
+    ny = az * bx - ax * bz;
+    nz = ax * by - ay * bx;
+    normalize3(nx, ny, nz);
     if (fn_out) {
-      fn_out[int64_t(f) * 3 + 0] = n.x;
-      fn_out[int64_t(f) * 3 + 1] = n.y;
-      fn_out[int64_t(f) * 3 + 2] = n.z;
+      fn_out[int64_t(f) * 3 + 0] = nx;
+      fn_out[int64_t(f) * 3 + 1] = ny;
+      fn_out[int64_t(f) * 3 + 2] = nz;
     }
   } else if (fnormals) {
-    n = make_float4(fnormals[int64_t(f) * 3 + 0], fnormals[int64_t(f) * 3 + 1],
-                    fnormals[int64_t(f) * 3 + 2], 0.0f);
+    nx = fnormals[int64_t(f) * 3 + 0];
+    ny = fnormals[int64_t(f) * 3 + 1];
+    nz = fnormals[int64_t(f) * 3 + 2];
   }
-  float4* r = soup + int64_t(f) * 4;
-  r[0] = p[0];
-  r[1] = p[1];
-  r[2] = p[2];
-  r[3] = n;
+  float4* r = soup + int64_t(f) * 3;
+  r[0] = make_float4(P[0], P[1], P[2], P[3]);
+  r[1] = make_float4(P[4], P[5], P[6], P[7]);
+  r[2] = make_float4(P[8], nx, ny, nz);
 }
 
 // ---------------------------------------------------------------------------
 // k_setup_bin: one thread per (face, view)
 
-// Appends fid to a bin segment for every lane with `has`. `seg` = 2 * tile
+// Appends `entry` to a bin segment for every lane with `has`. `seg` = 2 * tile
 // for the front segment (filled upwards from the bin's first slot) and
 // 2 * tile + 1 for the back segment (filled downwards from its last slot).
-// Lanes that target the same segment share one atomicAdd (ballot + popcount
-// rank). When the two segments together overflow the bin they overwrite each
-// other's slots; every slot the rasterizer reads still holds a valid face id,
+// Lanes that target the same segment form a group (ballot + popcount rank);
+// the group leaders then issue their atomicAdds together, so a wavefront
+// waits for one atomic round trip per call however many segments it hits.
+// When the two segments together overflow the bin they overwrite each
+// other's slots; every slot the rasterizer reads still holds a valid entry,
 // and the host renders again with larger bins.
-UGU_DEV void wave_append(const DevBins& bins, int seg, int fid, bool has) {
+UGU_DEV void wave_append(const DevBins& bins, int seg, int2 entry, bool has) {
   uint64_t todo = __ballot(has);
+  if (!todo) return;
   const int lane = __lane_id();
+  int leader = 0, rank = 0, size = 0;
   while (todo) {
-    const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
-    const int ls = __builtin_amdgcn_readlane(seg, leader);
+    const int l = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+    const int ls = __builtin_amdgcn_readlane(seg, l);
     const bool mine = has && seg == ls;
     const uint64_t grp = __ballot(mine);
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&bins.counts[ls], __popcll(grp));
-    base = __builtin_amdgcn_readlane(base, leader);
     if (mine) {
-      const int slot = base + __popcll(grp & ((1ull << lane) - 1ull));
-      if (slot < bins.cap) {
-        const int64_t row = int64_t(ls >> 1) * bins.cap;
-        bins.bins[row + ((ls & 1) ? bins.cap - 1 - slot : slot)] = fid;
-      }
+      leader = l;
+      rank = __popcll(grp & ((1ull << lane) - 1ull));
     }
+    if (lane == l) size = __popcll(grp);
     todo &= ~grp;
+  }
+  int base = 0;
+  if (has && lane == leader) base = atomicAdd(&bins.counts[seg], size);
+  base = __shfl(base, leader);
+  if (has) {
+    const int slot = base + rank;
+    if (slot < bins.cap) {
+      const int64_t row = int64_t(seg >> 1) * bins.cap;
+      bins.bins[row + ((seg & 1) ? bins.cap - 1 - slot : slot)] = entry;
+    }
   }
 }
 
-// Face id of entry i of a tile's bin: the front segment's nA entries, then the
-// back segment read from the bin's last slot downwards.
-UGU_DEV int bin_entry(const int32_t* bin, int nA, int cap, int i) {
+// Entry i of a tile's bin: the front segment's nA entries, then the back
+// segment read from the bin's last slot downwards.
+UGU_DEV int2 bin_entry(const int2* bin, int nA, int cap, int i) {
   return i < nA ? bin[i] : bin[cap - 1 - (i - nA)];
+}
+
+// Skip depth of a face in a view: a lower bound of every depth the coverage
+// test (cover_test) can compute for the face at a pixel of its bbox, or -1
+// when no useful bound exists. Once every in-view pixel of a tile holds a depth
+// d <= dmax, a face whose skip depth exceeds dmax (1 + 2^-8) can neither win a
+// pixel nor produce a depth below FLT_MIN, so the rasterizer drops it without
+// fetching it.
+// Bound: with L the larger bbox extent, rounding moves each edge function by
+// at most 8u L^2 (u = 2^-24) and the area by as much, so the normalised
+// weights sum to at most 1 + 32u L^2 / |area|; with 2^-9 L^2 <= |area| that is
+// 1 + 2^-10, five more roundings cost < 2^-21, and
+// z >= min(za, zb, zc) (1 - 2^-10 - 2^-20). NaN or huge values fail every
+// comparison and give -1.
+UGU_DEV float skip_depth(const float4& a, const float4& b, const float4& c, float area) {
+  const float ex = max3_std(a.x, b.x, c.x) - min3_std(a.x, b.x, c.x);
+  const float ey = max3_std(a.y, b.y, c.y) - min3_std(a.y, b.y, c.y);
+  if (!(ex <= 1048576.0f && ey <= 1048576.0f)) return -1.0f;
+  const float L = ex > ey ? ex : ey;
+  if (!(L * L * 0.001953125f <= fabsf(area))) return -1.0f;
+  if (!(a.z > 0.0f && b.z > 0.0f && c.z > 0.0f)) return -1.0f;
+  return min3_std(a.z, b.z, c.z);
+}
+
+// True when the tile's current depth bound proves the face hidden.
+UGU_DEV bool skip_face(float zskip, float dmax) {
+  return dmax >= 1e-18f && zskip > dmax * 1.00390625f;
 }
 
 // Views per thread: the face's indices are loaded once and reused for
@@ -336,19 +380,15 @@ __global__ __launch_bounds__(256) void k_setup_bin(
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   float P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (f < mesh.F) {
-    const float4* r = mesh.soup + int64_t(f) * 4;
-    for (int k = 0; k < 3; k++) {
-      const float4 q = r[k];
-      P[k * 3 + 0] = q.x;
-      P[k * 3 + 1] = q.y;
-      P[k * 3 + 2] = q.z;
-    }
+    float N[3];
+    load_face_rec(mesh.soup, f, P, N);
   }
   const int vb = blockIdx.y * kSetupViews;
   const int ve = min(vb + kSetupViews, n_views);
   for (int vi = vb; vi < ve; vi++) {
     const DevView& v = views[vi];
     bool has = false, back = false;
+    float zskip = -1.0f;
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
     if (f < mesh.F) {
       const float4 a = project_point(v, P[0], P[1], P[2]);
@@ -361,6 +401,7 @@ __global__ __launch_bounds__(256) void k_setup_bin(
         // faces that are clockwise on screen are mostly hidden back faces:
         // they go to the back segment so the rasterizer sees them last
         back = !(area > 0.0f);
+        zskip = skip_depth(a, b, c, area);
         tx0 = x0 / TW;
         tx1 = x1 / TW;
         ty0 = y0 / TH;
@@ -370,7 +411,7 @@ __global__ __launch_bounds__(256) void k_setup_bin(
     int tx = tx0, ty = ty0;
     while (__any(has)) {
       const int seg = has ? 2 * (v.tile_base + ty * v.tiles_x + tx) + (back ? 1 : 0) : -1;
-      wave_append(bins, seg, f, has);
+      wave_append(bins, seg, make_int2(f, __float_as_int(zskip)), has);
       if (has) {
         if (++tx > tx1) {
           tx = tx0;
@@ -390,7 +431,6 @@ struct RasterLds {
   float ray[3][NT];            // ray_w of each tile pixel
   float fv[13][NT];            // staged faces: 3 projected verts, area, face normal
   int32_t fid[NT];
-  int32_t vid[3][NT];          // staged face vertex indices
   int32_t box[NT];             // lx0 | ly0 << 8 | nx << 16 (tile-local pixel bbox)
   int32_t pref[NT + 1];        // exclusive prefix of pair counts
   int32_t wsum[NT / 64 + 1];
@@ -406,7 +446,7 @@ struct RasterLds {
 // for pixels where some covering face produced a depth below FLT_MIN or NaN,
 // where the packed atomicMin order is not the sequential order.
 UGU_DEV void sequential_pixel(const DevView& v, const DevMesh& mesh,
-                              const int32_t* bin, int nA, int cap, int n,
+                              const int2* bin, int nA, int cap, int n,
                               int gx, int gy, float r0, float r1, float r2,
                               int& out_fid) {
   int cur = -1;
@@ -417,18 +457,18 @@ UGU_DEV void sequential_pixel(const DevView& v, const DevMesh& mesh,
     int best = INT_MAX;
     float bestz = 0.0f;
     for (int i = 0; i < n; i++) {
-      const int f = bin_entry(bin, nA, cap, i);
+      const int f = bin_entry(bin, nA, cap, i).x;
       if (f <= cur || f >= best) continue;
-      const float4* rec = mesh.soup + int64_t(f) * 4;
-      const float4 a = project_soup(v, rec[0]);
-      const float4 b = project_soup(v, rec[1]);
-      const float4 c = project_soup(v, rec[2]);
+      float P[9], N[3];
+      load_face_rec(mesh.soup, f, P, N);
+      const float4 a = project_point(v, P[0], P[1], P[2]);
+      const float4 b = project_point(v, P[3], P[4], P[5]);
+      const float4 c = project_point(v, P[6], P[7], P[8]);
       int x0, x1, y0, y1;
       float area;
       if (!face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) continue;
       if (gx < x0 || gx > x1 || gy < y0 || gy > y1) continue;
-      const float4 fn = rec[3];
-      const bool bf = mesh.fnormals != nullptr && sum3(fn.x * r0, fn.y * r1, fn.z * r2) > 0.0f;
+      const bool bf = mesh.fnormals != nullptr && sum3(N[0] * r0, N[1] * r1, N[2] * r2) > 0.0f;
       float z, w0, w1, w2;
       if (cover_test(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, area, bf, x,
                      y, z, w0, w1, w2)) {
@@ -448,14 +488,15 @@ UGU_DEV void sequential_pixel(const DevView& v, const DevMesh& mesh,
 
 // Colour of one shaded pixel of the reference path: colourizer then shader
 // (pixel_shader.h:215-218 with the classes at :222-422).
-UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, int v0, int v1,
-                      int v2, float u, float vv, float r0, float r1, float r2, float n0,
-                      float n1, float n2, uint8_t c[3]) {
+UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, float u, float vv,
+                      float r0, float r1, float r2, float n0, float n1, float n2,
+                      uint8_t c[3]) {
   if (o.diffuse_color == 2) {
     // DiffuseVertexColorColorizer (pixel_shader.h:234-256)
-    const float* c0 = m.vcolors + int64_t(v0) * 3;
-    const float* c1 = m.vcolors + int64_t(v1) * 3;
-    const float* c2 = m.vcolors + int64_t(v2) * 3;
+    const int32_t* fi = m.faces + int64_t(fid) * 3;
+    const float* c0 = m.vcolors + int64_t(fi[0]) * 3;
+    const float* c1 = m.vcolors + int64_t(fi[1]) * 3;
+    const float* c2 = m.vcolors + int64_t(fi[2]) * 3;
     const float w = 1.0f - u - vv;
     for (int k = 0; k < 3; k++) c[k] = f2u8_x86(w * c0[k] + u * c1[k] + vv * c2[k]);
   } else if (o.diffuse_color == 1) {
@@ -638,26 +679,6 @@ UGU_DEV void diff_eval(const DevView& v, const DevMesh& m, int fid, int px, int 
   }
 }
 
-// True when every depth the coverage test could compute for this face at a
-// pixel of its bbox exceeds dmax (so no pair of the face can change a key or
-// set the slow-path flag). Bound: with L the larger bbox extent, rounding
-// moves each edge function by at most 8u L^2 (u = 2^-24) and the area by as
-// much, so the normalised weights sum to at most 1 + 32u L^2 / |area|; with
-// 2^-9 L^2 <= |area| that is 1 + 2^-10, five more roundings cost < 2^-21, and
-// z >= min(za, zb, zc) (1 - 2^-10 - 2^-20). Requiring
-// min z > dmax (1 + 2^-8) leaves a wide margin. NaN or huge values fail
-// every comparison, so such faces are never skipped.
-UGU_DEV bool hidden_face(const float4& a, const float4& b, const float4& c, float area,
-                         float dmax) {
-  const float ex = max3_std(a.x, b.x, c.x) - min3_std(a.x, b.x, c.x);
-  const float ey = max3_std(a.y, b.y, c.y) - min3_std(a.y, b.y, c.y);
-  if (!(ex <= 1048576.0f && ey <= 1048576.0f)) return false;
-  const float L = ex > ey ? ex : ey;
-  if (!(L * L * 0.001953125f <= fabsf(area))) return false;
-  const float zlim = dmax * 1.00390625f;
-  return dmax >= 1e-18f && a.z > zlim && b.z > zlim && c.z > zlim;
-}
-
 template <int TW, int TH, int MODE>
 UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
                          const DevMesh& mesh, const float4* proj,
@@ -724,35 +745,41 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   unsigned long long seen = ~0ull;  // key whose record the registers hold
   float4 wa = make_float4(0, 0, 0, 0), wb = wa, wc = wa;
   float warea = 0.0f, wn0 = 0.0f, wn1 = 0.0f, wn2 = 0.0f;
-  int wv0 = 0, wv1 = 0, wv2 = 0;
   // overflowing tiles report their true occupancy; the host re-renders with
   // bins that large (one same-address atomic per overflowing tile only)
   if (t == 0 && count > bins.cap) atomicMax(&bins.status[0], count);
-  const int32_t* bin = bins.bins + int64_t(gtile) * bins.cap;
+  const int2* bin = bins.bins + int64_t(gtile) * bins.cap;
   __syncthreads();
 
   // Largest depth any in-view pixel of the tile holds after the previous
   // chunks (NaN while some pixel has none). Keys only decrease, so a face
-  // whose depth provably exceeds it at every pixel can neither win a pixel
-  // nor produce a depth below FLT_MIN: its pairs are skipped.
+  // whose skip depth exceeds it is dropped before it is fetched. The back
+  // segment starts a new chunk so that all of its faces are tested against
+  // the depths the front faces left.
   float dmax = __uint_as_float(0xffffffffu);
-  for (int base = 0; base < n; base += NT) {
+  const int chunks_a = (nA + NT - 1) / NT, chunks = chunks_a + (nB + NT - 1) / NT;
+  for (int ch = 0; ch < chunks; ch++) {
+    const int base = ch < chunks_a ? ch * NT : nA + (ch - chunks_a) * NT;
+    const int end = ch < chunks_a ? nA : n;
+    const bool more = ch + 1 < chunks;
     // stage one chunk of faces
     int c = 0;
     const int i = base + t;
-    if (i < n) {
-      const int f = bin_entry(bin, nA, bins.cap, i);
-      const float4* rec = mesh.soup + int64_t(f) * 4;
-      const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], fn = rec[3];
-      const float4 a = project_soup(v, q0);
-      const float4 b = project_soup(v, q1);
-      const float4 cc = project_soup(v, q2);
+    int2 e = make_int2(0, 0);
+    if (i < end) e = bin_entry(bin, nA, bins.cap, i);
+    if (i < end && !skip_face(__int_as_float(e.y), dmax)) {
+      const int f = e.x;
+      float P[9], N[3];
+      load_face_rec(mesh.soup, f, P, N);
+      const float4 a = project_point(v, P[0], P[1], P[2]);
+      const float4 b = project_point(v, P[3], P[4], P[5]);
+      const float4 cc = project_point(v, P[6], P[7], P[8]);
       int x0, x1, y0, y1;
       float area;
       if (face_setup(a, b, cc, v.W, v.H, x0, x1, y0, y1, area)) {
         const int lx0 = max(x0, px0) - px0, lx1 = min(x1, px0 + TW - 1) - px0;
         const int ly0 = max(y0, py0) - py0, ly1 = min(y1, py0 + TH - 1) - py0;
-        if (lx0 <= lx1 && ly0 <= ly1 && !hidden_face(a, b, cc, area, dmax)) {
+        if (lx0 <= lx1 && ly0 <= ly1) {
           const int nx = lx1 - lx0 + 1;
           c = nx * (ly1 - ly0 + 1);
           s.box[t] = lx0 | (ly0 << 8) | (nx << 16);
@@ -762,11 +789,8 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
       s.fv[3][t] = b.x; s.fv[4][t] = b.y; s.fv[5][t] = b.z;
       s.fv[6][t] = cc.x; s.fv[7][t] = cc.y; s.fv[8][t] = cc.z;
       s.fv[9][t] = area;
-      s.fv[10][t] = fn.x; s.fv[11][t] = fn.y; s.fv[12][t] = fn.z;
+      s.fv[10][t] = N[0]; s.fv[11][t] = N[1]; s.fv[12][t] = N[2];
       s.fid[t] = f;
-      s.vid[0][t] = __float_as_int(q0.w);
-      s.vid[1][t] = __float_as_int(q1.w);
-      s.vid[2][t] = __float_as_int(q2.w);
     }
     // block exclusive scan of the pair counts
     {
@@ -822,7 +846,7 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
     }
     __syncthreads();
     const unsigned long long k = s.key[t];
-    if (base + NT < n) {
+    if (more) {
       // depth bits of this pixel: all ones when it has no key yet, 0 outside
       // the image (no face reaches those pixels)
       uint32_t db = (gx < v.W && gy < v.H) ? static_cast<uint32_t>(k >> 32) : 0u;
@@ -839,13 +863,10 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
         wn0 = s.fv[10][sl];
         wn1 = s.fv[11][sl];
         wn2 = s.fv[12][sl];
-        wv0 = s.vid[0][sl];
-        wv1 = s.vid[1][sl];
-        wv2 = s.vid[2][sl];
         seen = k;
       }
     }
-    if (base + NT < n) {
+    if (more) {
       // the next chunk's staging overwrites the records read above
       __syncthreads();
       uint32_t m = s.wmax[0];
@@ -875,18 +896,15 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   bool bf = false;
   if (fid >= 0) {
     if (!have) {
-      const float4* rec = mesh.soup + int64_t(fid) * 4;
-      const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], fn = rec[3];
-      wv0 = __float_as_int(q0.w);
-      wv1 = __float_as_int(q1.w);
-      wv2 = __float_as_int(q2.w);
-      wa = project_soup(v, q0);
-      wb = project_soup(v, q1);
-      wc = project_soup(v, q2);
+      float P[9], N[3];
+      load_face_rec(mesh.soup, fid, P, N);
+      wa = project_point(v, P[0], P[1], P[2]);
+      wb = project_point(v, P[3], P[4], P[5]);
+      wc = project_point(v, P[6], P[7], P[8]);
       warea = edge_fn(wa.x, wa.y, wb.x, wb.y, wc.x, wc.y);
-      wn0 = fn.x;
-      wn1 = fn.y;
-      wn2 = fn.z;
+      wn0 = N[0];
+      wn1 = N[1];
+      wn2 = N[2];
     }
     bf = mesh.fnormals != nullptr && sum3(wn0 * r0, wn1 * r1, wn2 * r2) > 0.0f;
     cover_test(wa.x, wa.y, wa.z, wb.x, wb.y, wb.z, wc.x, wc.y, wc.z, warea, bf,
@@ -922,7 +940,7 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
         nrm[1] = sum3(v.w2c_R[3] * n0, v.w2c_R[4] * n1, v.w2c_R[5] * n2);
         nrm[2] = sum3(v.w2c_R[6] * n0, v.w2c_R[7] * n1, v.w2c_R[8] * n2);
       }
-      if (v.color) shade_u8(mesh, opt, fid, wv0, wv1, wv2, w1, w2, r0, r1, r2, n0, n1, n2, col);
+      if (v.color) shade_u8(mesh, opt, fid, w1, w2, r0, r1, r2, n0, n1, n2, col);
     }
     if (v.color) {
       uint8_t* cp = v.color + pidx * 3;

@@ -45,9 +45,9 @@ struct DevMesh {
   const int32_t* matid;     // F or null
   const TexInfo* tex;       // per material
   const uint8_t* texdata;   // u8 RGB texture pool
-  const float4* soup;       // F x 4: the face's three vertex positions (.w =
-                            // vertex id bits) and its face normal (zero when
-                            // the mesh has none)
+  const float4* soup;       // F x 3: the face's three vertex positions and its
+                            // face normal (zero when the mesh has none),
+                            // packed (x0 y0 z0 x1)(y1 z1 x2 y2)(z2 nx ny nz)
   const float* ftex;        // differentiable path: f32 RGB texture or null
   int32_t ftex_w, ftex_h;
 };
@@ -63,8 +63,9 @@ struct DevOptions {
 
 struct DevBins {
   int32_t* counts;   // per global tile: front / back segment counters
-  int32_t* bins;     // global tile t owns bins[t*cap .. t*cap+cap): front
-                     // segment from the start, back segment from the end
+  int2* bins;        // global tile t owns bins[t*cap .. t*cap+cap): front
+                     // segment from the start, back segment from the end;
+                     // entry = (face id, bits of its skip depth)
   int32_t cap;
   int32_t total_tiles;
   int32_t* status;   // [0] max bin count seen, [1] work-queue head

@@ -107,7 +107,7 @@ struct ugu_hip_ctx {
   // work buffers
   DevBuf proj, counts, tile_view, bins, status, views, tile_base, stage_out, diff_fn;
   DevBuf bwd_items;  // (face, view, pixel) items of the backward pass, 16 B each
-  DevBuf soup, diff_soup;  // 64-byte face records (prepared mesh / differentiable path)
+  DevBuf soup, diff_soup;  // 48-byte face records (prepared mesh / differentiable path)
   int64_t buf_gen = 0;
   void* h_views = nullptr;
   size_t h_views_bytes = 0;
@@ -412,10 +412,10 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
 
   c->tm = UguTimings{};
   for (int attempt = 0; attempt < 8; attempt++) {
-    if ((err = ensure(c, c->bins, size_t(total_tiles) * size_t(c->tiling.bin_capacity) * 4))) return err;
+    if ((err = ensure(c, c->bins, size_t(total_tiles) * size_t(c->tiling.bin_capacity) * 8))) return err;
     DevBins b;
     b.counts = c->counts.as<int32_t>();
-    b.bins = c->bins.as<int32_t>();
+    b.bins = c->bins.as<int2>();
     b.cap = c->tiling.bin_capacity;
     b.total_tiles = total_tiles;
     b.status = c->status.as<int32_t>();
@@ -507,7 +507,7 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
     c->tm.retries++;
     int64_t cap = int64_t(max_count) + max_count / 8 + 64;
     cap = (cap + 63) / 64 * 64;
-    if (cap * int64_t(total_tiles) * 4 > (int64_t(64) << 30))
+    if (cap * int64_t(total_tiles) * 8 > (int64_t(64) << 30))
       return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, "bins would exceed 64 GiB");
     c->tiling.bin_capacity = static_cast<int32_t>(cap);
     if (attempt == 7) return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, "bin capacity did not converge");
@@ -727,7 +727,7 @@ int ugu_hip_prepare_mesh(ugu_hip_ctx* c, const UguMeshDesc* m) {
   c->has_uvidx = m->uv_indices != nullptr && F > 0;
   c->has_matid = m->material_ids != nullptr && F > 0;
   if (F > 0) {
-    if ((err = ensure(c, c->soup, size_t(F) * 64))) return err;
+    if ((err = ensure(c, c->soup, size_t(F) * 48))) return err;
     if (launch_face_soup(c->stream, c->verts.as<float>(), c->faces.as<int32_t>(),
                          c->has_fn ? c->fnormals.as<float>() : nullptr, F, 0,
                          c->soup.as<float4>(), nullptr))
@@ -893,7 +893,7 @@ static int diff_mesh(ugu_hip_ctx* c, const UguDiffParams* p, DevMesh* dm) {
   dm->ftex_h = p->texture ? p->tex_height : 0;
   int err = ensure(c, c->diff_fn, size_t(std::max(c->F, 1)) * 12);
   if (err) return err;
-  if ((err = ensure(c, c->diff_soup, size_t(std::max(c->F, 1)) * 64))) return err;
+  if ((err = ensure(c, c->diff_soup, size_t(std::max(c->F, 1)) * 48))) return err;
   dm->fnormals = c->diff_fn.as<float>();
   dm->soup = c->diff_soup.as<float4>();
   return 0;
