@@ -372,6 +372,10 @@ UGU_DEV bool skip_face(float zskip, float dmax) {
 // kSetupViews consecutive views (grid.y = ceil(n_views / kSetupViews)).
 constexpr int kSetupViews = 8;
 
+// Chunks whose faces all have at most this many (face, pixel) pairs expand
+// them through an LDS pair list; otherwise by binary search over the prefix.
+constexpr int kScatterMaxPairs = 32;
+
 template <int TW, int TH>
 __global__ __launch_bounds__(256) void k_setup_bin(
     const DevView* __restrict__ views, int n_views, DevMesh mesh,
@@ -432,9 +436,12 @@ struct RasterLds {
   float fv[13][NT];            // staged faces: 3 projected verts, area, face normal
   int32_t fid[NT];
   int32_t box[NT];             // lx0 | ly0 << 8 | nx << 16 (tile-local pixel bbox)
-  int32_t pref[NT + 1];        // exclusive prefix of pair counts
+  // exclusive prefix of pair counts (binary-search expansion) or the
+  // (face slot | pair index << 8) list of one round (pair-list expansion)
+  int32_t pairs[2 * NT + 1];
   int32_t wsum[NT / 64 + 1];
-  int32_t flag[NT];            // pixel saw a depth outside [FLT_MIN, inf]
+  int32_t wbig[NT / 64];       // per-wave largest pair count of a face
+  uint32_t flag[NT / 32];      // pixel saw a depth outside [FLT_MIN, inf] (bit per pixel)
   uint32_t wmax[NT / 64];       // per-wave max of the pixels' depth bits
   int32_t any_flag;
   int32_t item;
@@ -725,7 +732,7 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   float r0, r1, r2;
   ray_w_at(v, static_cast<float>(gx), static_cast<float>(gy), r0, r1, r2);
   s.key[t] = ~0ull;
-  s.flag[t] = 0;
+  if (t < NT / 32) s.flag[t] = 0u;
   s.ray[0][t] = r0;
   s.ray[1][t] = r1;
   s.ray[2][t] = r2;
@@ -792,32 +799,28 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
       s.fv[10][t] = N[0]; s.fv[11][t] = N[1]; s.fv[12][t] = N[2];
       s.fid[t] = f;
     }
-    // block exclusive scan of the pair counts
-    {
-      const int lane = t & 63, wid = t >> 6;
-      int x = c;
-      for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off);
-        if (lane >= off) x += y;
-      }
-      if (lane == 63) s.wsum[wid] = x;
-      __syncthreads();
-      int wb = 0;
-      for (int w = 0; w < wid; w++) wb += s.wsum[w];
-      s.pref[t] = wb + x - c;
-      if (t == NT - 1) s.pref[NT] = wb + x;
-      __syncthreads();
+    // block exclusive scan of the pair counts; a chunk without any pair
+    // (every face skipped or off the tile) ends at the barrier
+    const int lane = t & 63, wid = t >> 6;
+    int x = c, cmax = c;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+      cmax = max(cmax, __shfl_xor(cmax, off));
     }
-    const int total = s.pref[NT];
+    if (lane == 63) s.wsum[wid] = x;
+    if (lane == 0) s.wbig[wid] = cmax;
+    if (!__syncthreads_or(c > 0)) continue;
+    int wbase = 0, total = 0, big = 0;
+    for (int w = 0; w < NT / 64; w++) {
+      if (w < wid) wbase += s.wsum[w];
+      total += s.wsum[w];
+      big = max(big, s.wbig[w]);
+    }
+    const int pre = wbase + x - c;  // first pair of this thread's face
     const bool has_fn = mesh.fnormals != nullptr;
-    for (int p = t; p < total; p += NT) {
-      int jl = 0, jh = NT - 1;
-      while (jl < jh) {
-        const int mid = (jl + jh + 1) >> 1;
-        if (s.pref[mid] <= p) jl = mid; else jh = mid - 1;
-      }
-      const int j = jl;
-      const int r = p - s.pref[j];
+    // One (face, pixel) pair: coverage, depth and the packed LDS atomicMin.
+    auto test_pair = [&](int j, int r) {
       const int bx = s.box[j];
       const int nx = (bx >> 16) & 0xff;
       const int qy = r / nx;
@@ -839,12 +842,41 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
               (static_cast<unsigned long long>(__float_as_uint(z)) << 32) | lowbits;
           atomicMin(&s.key[pix], key);
         } else {
-          s.flag[pix] = 1;
+          atomicOr(&s.flag[pix >> 5], 1u << (pix & 31));
           s.any_flag = 1;
         }
       }
+    };
+    if (big <= kScatterMaxPairs) {
+      // small faces: every face writes its pair list (slot, index) into LDS,
+      // then every lane takes one pair
+      for (int r0 = 0; r0 < total; r0 += 2 * NT) {
+        const int lo = max(pre, r0), hi = min(pre + c, r0 + 2 * NT);
+        for (int q = lo; q < hi; q++) s.pairs[q - r0] = t | ((q - pre) << 8);
+        __syncthreads();
+        const int np = min(total - r0, 2 * NT);
+        for (int pp = t; pp < np; pp += NT) {
+          const int pr = s.pairs[pp];
+          test_pair(pr & 0xff, pr >> 8);
+        }
+        __syncthreads();
+      }
+    } else {
+      // some face covers many pixels: binary search of the pair's face over
+      // the prefix (balanced whatever the face sizes)
+      s.pairs[t] = pre;
+      if (t == NT - 1) s.pairs[NT] = total;
+      __syncthreads();
+      for (int p = t; p < total; p += NT) {
+        int jl = 0, jh = NT - 1;
+        while (jl < jh) {
+          const int mid = (jl + jh + 1) >> 1;
+          if (s.pairs[mid] <= p) jl = mid; else jh = mid - 1;
+        }
+        test_pair(jl, p - s.pairs[jl]);
+      }
+      __syncthreads();
     }
-    __syncthreads();
     const unsigned long long k = s.key[t];
     if (more) {
       // depth bits of this pixel: all ones when it has no key yet, 0 outside
@@ -880,7 +912,7 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   // registers (slot keys) or gathered again below
   int fid = -1;
   bool have = false;
-  if (s.any_flag && s.flag[t]) {
+  if (s.any_flag && ((s.flag[t >> 5] >> (t & 31)) & 1u)) {
     sequential_pixel(v, mesh, bin, nA, bins.cap, n, gx, gy, r0, r1, r2, fid);
   } else if (seen != ~0ull) {
     fid = static_cast<int>(static_cast<unsigned int>(seen & 0xffffffffull) >> 8);
@@ -974,7 +1006,10 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
 }
 
 template <int TW, int TH, int MODE>
-__global__ __launch_bounds__(TW * TH) void k_raster(
+// At least 6 waves per SIMD: the kernel is latency-bound (dependent gathers,
+// LDS round trips, barriers), so occupancy beats the few registers the
+// compiler then keeps in scratch.
+__global__ __launch_bounds__(TW * TH, 6) void k_raster(
     const DevView* __restrict__ views, DevMesh mesh,
     const float4* __restrict__ proj, DevBins bins, DevOptions opt,
     int persistent) {
