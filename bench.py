@@ -64,6 +64,26 @@ def algorithmic_bytes(mesh, cams, texture_touched=0):
     return b_in, 24 * px
 
 
+PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "raster_64view_icosphere8.md")
+
+
+def measured_traffic(tiling):
+    """HBM bytes per k_raster launch of this workload: FETCH_SIZE x 2 +
+    WRITE_SIZE x 1 (factors from a 1 GiB copy in the same profile) of the
+    committed rocprofv3 summary, if it was taken with the same tiling."""
+    tag = f"{tiling['tile_w']}x{tiling['tile_h']}:{'p' if tiling['persistent'] else 'g'}"
+    try:
+        with open(PROFILE_SUMMARY) as f:
+            for line in f:
+                parts = line.split()
+                if (len(parts) == 4 and parts[0] == "traffic_per_launch_bytes"
+                        and parts[1] == "k_raster" and parts[2] == tag):
+                    return int(parts[3])
+    except OSError:
+        pass
+    return None
+
+
 def time_calls(torch, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -266,7 +286,7 @@ def main():
 
     # ---- per-kernel timings (HIP events on the library's stream, eager launches)
     r.enable_profiling(True)
-    kt = {"project_ms": 0.0, "setup_bin_ms": 0.0, "raster_ms": 0.0, "clear_ms": 0.0}
+    kt = {"clear_ms": 0.0, "setup_bin_ms": 0.0, "raster_ms": 0.0}
     n_prof = 5
     for _ in range(n_prof):
         r.render_device(abi_cams, bufs, graph=False)
@@ -275,9 +295,10 @@ def main():
             kt[k] += t[k] / n_prof
     r.enable_profiling(False)
     b_in, b_out = algorithmic_bytes(mesh, cams)
-    dom = max(("project_ms", "setup_bin_ms", "raster_ms"), key=lambda k: kt[k])
+    traffic = measured_traffic(tiling)
+    dom = max(("setup_bin_ms", "raster_ms"), key=lambda k: kt[k])
     raster_gbs = (b_in + b_out) / (kt["raster_ms"] * 1e-3) / 1e9
-    pipeline_ms = kt["clear_ms"] + kt["project_ms"] + kt["setup_bin_ms"] + kt["raster_ms"]
+    pipeline_ms = kt["clear_ms"] + kt["setup_bin_ms"] + kt["raster_ms"]
     log(f"kernels {kt} dominant {dom}")
 
     extra = {}
@@ -322,10 +343,10 @@ def main():
         log("CPU oracle baseline (bounded sample)")
         done, t_cpu, threads = cpu_baseline(mesh, cams, opt, args.cpu_budget)
         cpu_val = F * done / t_cpu / 1e6
-        cpu = {"value": cpu_val, "unit": "Mtri*views/s", "cores": 1,
-               "ray_table_threads": threads, "kind": "port",
+        cpu = {"value": cpu_val, "unit": "Mtri*views/s", "cores": threads, "kind": "port",
                "sample": f"first {done} of the 64 views, C++ restatement of Render() "
-                         f"(single-threaded raster loops, OpenMP ray table), {t_cpu:.1f} s",
+                         f"(single-threaded vertex/face/resolve loops as the reference; its "
+                         f"ray table on {threads} OpenMP threads), {t_cpu:.1f} s",
                "host_cpus": os.cpu_count()}
         extra["gpu_vs_cpu"] = value / cpu_val
 
@@ -349,7 +370,7 @@ def main():
         "mpix_per_s": mpix,
         "roofline": {"bound": "hbm", "kernel": "k_raster",
                      "achieved": raster_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": raster_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": raster_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": b_in + b_out,
                      "kernel_ms": kt["raster_ms"]},
         "kernels_ms": kt,

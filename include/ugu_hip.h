@@ -155,7 +155,6 @@ typedef struct {
  * context's stream (filled only when profiling is enabled). */
 typedef struct {
   float clear_ms;
-  float project_ms;
   float setup_bin_ms;
   float raster_ms;
   float total_ms;

@@ -83,7 +83,7 @@ class UguTiling(ctypes.Structure):
 
 class UguTimings(ctypes.Structure):
     _fields_ = [
-        ("clear_ms", _f), ("project_ms", _f), ("setup_bin_ms", _f),
+        ("clear_ms", _f), ("setup_bin_ms", _f),
         ("raster_ms", _f), ("total_ms", _f), ("max_bin_count", _i), ("retries", _i),
     ]
 

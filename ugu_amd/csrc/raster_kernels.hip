@@ -1,18 +1,17 @@
 // HIP kernels of the gfx950 rasterizer.
 //
 // Pipeline for a batch of views (a single view is a batch of one):
+//   k_face_soup   48-byte face records (positions, face normal) per mesh
 //   k_clear       zero per-tile bin counters and the status words
-//   k_project     vertex loop of Rasterizer::Impl::Render
-//                 (src/renderer/cpu/rasterizer.cc:125-136, camera.h:380-387)
 //   k_setup_bin   per-face setup of the face loop (rasterizer.cc:162-193):
 //                 z<0 reject, bbox, off-screen cull, pixel bbox, area; then
-//                 appends the face id to every screen tile its pixel bbox
-//                 touches (wavefront-aggregated atomics)
+//                 appends (face id, skip depth) to the front or back segment
+//                 of every screen tile its pixel bbox touches
 //   k_raster      one workgroup per screen tile: stages the tile's faces in
-//                 LDS, expands (face, pixel) pairs with a block prefix sum so
-//                 every lane tests one pair, resolves depth with a packed
-//                 64-bit (depth bits << 32 | face id) LDS atomicMin, then
-//                 shades and writes every output pixel of the tile once
+//                 LDS (skipping faces proven hidden), expands (face, pixel)
+//                 pairs so every lane tests one pair, resolves depth with a
+//                 packed 64-bit (depth bits << 32 | face id) LDS atomicMin,
+//                 then shades and writes every output pixel of the tile once
 //                 (resolve loop rasterizer.cc:243-301, pixel_shader.h).
 //   k_bwd_*       differentiable backward pass (no reference counterpart).
 //
@@ -215,21 +214,6 @@ __global__ void k_clear(int32_t* counts, int32_t n, int32_t* status,
     tile_view[i] = lo;
   }
   if (i < 4) status[i] = 0;
-}
-
-// ---------------------------------------------------------------------------
-// k_project: one thread per (vertex, view)
-
-__global__ void k_project(const DevView* __restrict__ views,
-                          const float* __restrict__ verts, int32_t V,
-                          float4* __restrict__ proj) {
-  const DevView& v = views[blockIdx.y];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= V) return;
-  float px = verts[int64_t(i) * 3 + 0];
-  float py = verts[int64_t(i) * 3 + 1];
-  float pz = verts[int64_t(i) * 3 + 2];
-  proj[v.proj_offset + i] = project_point(v, px, py, pz);
 }
 
 // ---------------------------------------------------------------------------
@@ -729,6 +713,17 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
     }
     return;
   }
+  // front segment first, then the back segment (both clipped to the bin)
+  const int nA = seg.x < bins.cap ? seg.x : bins.cap;
+  const int nB = seg.y < bins.cap - nA ? seg.y : bins.cap - nA;
+  const int n = nA + nB;
+  const int2* bin = bins.bins + int64_t(gtile) * bins.cap;
+  // the first chunk's entries are loaded while the rays are computed; every
+  // chunk then loads the next chunk's entries before its own pairs
+  const int chunks_a = (nA + NT - 1) / NT, chunks = chunks_a + (nB + NT - 1) / NT;
+  int2 e_next = make_int2(0, 0);
+  if (t < (chunks_a > 0 ? nA : n)) e_next = bin_entry(bin, nA, bins.cap, t);
+
   float r0, r1, r2;
   ray_w_at(v, static_cast<float>(gx), static_cast<float>(gy), r0, r1, r2);
   s.key[t] = ~0ull;
@@ -737,11 +732,6 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   s.ray[1][t] = r1;
   s.ray[2][t] = r2;
   if (t == 0) s.any_flag = 0;
-
-  // front segment first, then the back segment (both clipped to the bin)
-  const int nA = seg.x < bins.cap ? seg.x : bins.cap;
-  const int nB = seg.y < bins.cap - nA ? seg.y : bins.cap - nA;
-  const int n = nA + nB;
   // When face ids fit 24 bits the depth key also carries the LDS slot of the
   // face (below the id, so ties still go to the lowest id). After every chunk
   // each thread copies the staged record of its pixel's current winner into
@@ -755,7 +745,6 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   // overflowing tiles report their true occupancy; the host re-renders with
   // bins that large (one same-address atomic per overflowing tile only)
   if (t == 0 && count > bins.cap) atomicMax(&bins.status[0], count);
-  const int2* bin = bins.bins + int64_t(gtile) * bins.cap;
   __syncthreads();
 
   // Largest depth any in-view pixel of the tile holds after the previous
@@ -764,16 +753,19 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   // segment starts a new chunk so that all of its faces are tested against
   // the depths the front faces left.
   float dmax = __uint_as_float(0xffffffffu);
-  const int chunks_a = (nA + NT - 1) / NT, chunks = chunks_a + (nB + NT - 1) / NT;
   for (int ch = 0; ch < chunks; ch++) {
     const int base = ch < chunks_a ? ch * NT : nA + (ch - chunks_a) * NT;
     const int end = ch < chunks_a ? nA : n;
     const bool more = ch + 1 < chunks;
+    const int i = base + t;
+    const int2 e = e_next;
+    if (more) {
+      const int nbase = ch + 1 < chunks_a ? (ch + 1) * NT : nA + (ch + 1 - chunks_a) * NT;
+      const int nend = ch + 1 < chunks_a ? nA : n;
+      e_next = nbase + t < nend ? bin_entry(bin, nA, bins.cap, nbase + t) : make_int2(0, 0);
+    }
     // stage one chunk of faces
     int c = 0;
-    const int i = base + t;
-    int2 e = make_int2(0, 0);
-    if (i < end) e = bin_entry(bin, nA, bins.cap, i);
     if (i < end && !skip_face(__int_as_float(e.y), dmax)) {
       const int f = e.x;
       float P[9], N[3];
@@ -1337,14 +1329,6 @@ int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status,
   int m = n > 4 ? n : 4;
   hipLaunchKernelGGL(k_clear, dim3((m + 255) / 256), dim3(256), 0, st, counts, n, status,
                      tile_base, n_views, tile_view);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_project(hipStream_t st, const DevView* views, int n_views,
-                   const float* verts, int32_t V, float4* proj) {
-  if (V <= 0) return 0;
-  hipLaunchKernelGGL(k_project, dim3((V + 255) / 256, n_views), dim3(256), 0, st,
-                     views, verts, V, proj);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -19,8 +19,6 @@
 namespace ugu_hip {
 int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status,
                  const int32_t* tile_base, int32_t n_views, int32_t* tile_view);
-int launch_project(hipStream_t st, const DevView* views, int n_views,
-                   const float* verts, int32_t V, float4* proj);
 int launch_face_soup(hipStream_t st, const float* verts, const int32_t* faces,
                      const float* fnormals, int32_t F, int compute_normals, float4* soup,
                      float* fn_out);
@@ -436,11 +434,10 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
       if (launch_clear(c->stream, b.counts, total_tiles, b.status, b.tile_base, n, b.tile_view))
         return -1;
       if (timed) (void)hipEventRecord(c->ev[1], c->stream);
-      if (timed) (void)hipEventRecord(c->ev[2], c->stream);
       if (launch_setup_bin(c->stream, tw, th, dv, n, dm, proj, b)) return -1;
-      if (timed) (void)hipEventRecord(c->ev[3], c->stream);
+      if (timed) (void)hipEventRecord(c->ev[2], c->stream);
       if (launch_raster(c->stream, tw, th, mode, dv, dm, proj, b, dopt, persistent_blocks)) return -1;
-      if (timed) (void)hipEventRecord(c->ev[4], c->stream);
+      if (timed) (void)hipEventRecord(c->ev[3], c->stream);
       return 0;
     };
 
@@ -494,13 +491,12 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
     const int max_count = c->h_status[0];
     c->tm.max_bin_count = std::max(c->tm.max_bin_count, max_count);
     if (c->profile) {
-      float ms[4] = {0, 0, 0, 0};
-      for (int k = 0; k < 4; k++) (void)hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]);
+      float ms[3] = {0, 0, 0};
+      for (int k = 0; k < 3; k++) (void)hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]);
       c->tm.clear_ms += ms[0];
-      c->tm.project_ms += ms[1];
-      c->tm.setup_bin_ms += ms[2];
-      c->tm.raster_ms += ms[3];
-      c->tm.total_ms += ms[0] + ms[1] + ms[2] + ms[3];
+      c->tm.setup_bin_ms += ms[1];
+      c->tm.raster_ms += ms[2];
+      c->tm.total_ms += ms[0] + ms[1] + ms[2];
     }
     if (max_count <= b.cap) break;
     // a bin overflowed: grow to the measured occupancy and render again
