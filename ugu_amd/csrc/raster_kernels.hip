@@ -1028,116 +1028,117 @@ __global__ __launch_bounds__(TW * TH, 6) void k_raster(
 
 // ---------------------------------------------------------------------------
 // Differentiable backward pass: dL/d(vertices, vertex colours, texture) at
-// fixed visibility, in three kernels.
-//   k_bwd_collect_faces   one thread per face: loops over the views, finds the
-//                         pixels of the face's pixel bbox whose face id is the
-//                         face and appends them as (face, view, pixel) items,
-//                         every face's items contiguous (faces with large
-//                         bboxes are left to the next kernel)
-//   k_bwd_collect_pixels  one thread per pixel: appends the pixels of faces
-//                         with large bboxes (runs of neighbouring pixels)
-//   k_bwd_items           one thread per item: shading derivatives in float64;
-//                         contributions of contiguous items of one face are
-//                         summed across the wavefront (typically all the
-//                         face's pixels in every view), then one atomic per
-//                         vertex component and run
-// The items are grouped by face so a face's gradient becomes a handful of
-// atomics instead of one set per pixel.
-
-// Faces whose pixel bbox in a view has more pixels than this are collected
-// per pixel.
-constexpr int kBwdFaceBoxMax = 64;
-
-// The reference face setup of a face in a view (same arithmetic as the
-// rasterizer, so the bbox contains every pixel the face won) and whether the
-// per-face collector handles it.
-UGU_DEV bool bwd_small_face(const DevView& v, const float* P, int& x0, int& x1, int& y0,
-                            int& y1) {
-  const float4 a = project_point(v, P[0], P[1], P[2]);
-  const float4 b = project_point(v, P[3], P[4], P[5]);
-  const float4 c = project_point(v, P[6], P[7], P[8]);
-  float area;
-  if (!face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) return false;
-  return int64_t(x1 - x0 + 1) * (y1 - y0 + 1) <= kBwdFaceBoxMax;
-}
-
-UGU_DEV void load_face_verts(const DevMesh& m, int f, float P[9]) {
-  const int32_t* fi = m.faces + int64_t(f) * 3;
-  for (int k = 0; k < 3; k++) {
-    const float* p = m.verts + int64_t(fi[k]) * 3;
-    P[k * 3 + 0] = p[0];
-    P[k * 3 + 1] = p[1];
-    P[k * 3 + 2] = p[2];
-  }
-}
+// fixed visibility. The covered pixels of all views are sorted by face id
+// into (face, view, pixel) items with a counting sort:
+//   k_bwd_count    one thread per pixel: per-face pixel counts
+//   k_scan_*       exclusive scan of the counts (face offsets, item total)
+//   k_bwd_scatter  one thread per pixel: item at offset + (count - 1), the
+//                  counts decrement back to zero
+//   k_bwd_items    one thread per item: shading derivatives in float64;
+//                  contributions of contiguous items of one face (all its
+//                  pixels in every view) are summed across the wavefront,
+//                  then one atomic per vertex component and run
+// Grouping by face turns a face's gradient into a handful of atomics instead
+// of one set per pixel.
 
 __global__ void k_zero_i32(int32_t* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_bwd_collect_faces(
-    const DevView* __restrict__ views, int n_views, DevMesh mesh,
-    int4* __restrict__ items, int32_t* __restrict__ cursor) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  // face 0 is never shaded (rasterizer.cc:250), so it has no gradient
-  const bool live = f > 0 && f < mesh.F;
-  float P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (live) load_face_verts(mesh, f, P);
-  int count = 0;
-  for (int vi = 0; vi < n_views && live; vi++) {
-    const DevView& v = views[vi];
-    int x0, x1, y0, y1;
-    if (!bwd_small_face(v, P, x0, x1, y0, y1)) continue;
-    for (int y = y0; y <= y1; y++)
-      for (int x = x0; x <= x1; x++) count += v.face_id[int64_t(y) * v.W + x] == f;
-  }
-  // one cursor atomic per wavefront; lanes take consecutive ranges
-  const int lane = __lane_id();
-  int incl = count;
-  for (int off = 1; off < 64; off <<= 1) {
-    const int o = __shfl_up(incl, off);
-    if (lane >= off) incl += o;
-  }
-  const int wave_total = __shfl(incl, 63);
-  int base = 0;
-  if (lane == 0 && wave_total > 0) base = atomicAdd(cursor, wave_total);
-  base = __shfl(base, 0);
-  if (count == 0) return;
-  int pos = base + incl - count;
-  for (int vi = 0; vi < n_views; vi++) {
-    const DevView& v = views[vi];
-    int x0, x1, y0, y1;
-    if (!bwd_small_face(v, P, x0, x1, y0, y1)) continue;
-    for (int y = y0; y <= y1; y++)
-      for (int x = x0; x <= x1; x++) {
-        const int px = y * v.W + x;
-        if (v.face_id[px] == f) items[pos++] = make_int4(f, vi, px, 0);
-      }
+// face 0 is never shaded (rasterizer.cc:250), so it has no gradient
+__global__ __launch_bounds__(256) void k_bwd_count(const DevView* __restrict__ views,
+                                                   int32_t F, int32_t* __restrict__ cnt) {
+  const DevView& v = views[blockIdx.y];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= v.W * v.H) return;
+  const int fid = v.face_id[p];
+  if (fid > 0 && fid < F) atomicAdd(&cnt[fid], 1);
+}
+
+__global__ __launch_bounds__(256) void k_bwd_scatter(const DevView* __restrict__ views,
+                                                     int32_t F, const int32_t* __restrict__ off,
+                                                     int32_t* __restrict__ cnt,
+                                                     int4* __restrict__ items) {
+  const DevView& v = views[blockIdx.y];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= v.W * v.H) return;
+  const int fid = v.face_id[p];
+  if (fid > 0 && fid < F) {
+    const int pos = off[fid] + atomicAdd(&cnt[fid], -1) - 1;
+    items[pos] = make_int4(fid, blockIdx.y, p, 0);
   }
 }
 
-__global__ __launch_bounds__(256) void k_bwd_collect_pixels(
-    const DevView* __restrict__ views, DevMesh mesh, int4* __restrict__ items,
-    int32_t* __restrict__ cursor) {
-  const DevView& v = views[blockIdx.y];
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int fid = p < v.W * v.H ? v.face_id[p] : -1;
-  bool emit = false;
-  if (fid > 0 && fid < mesh.F) {
-    float P[9];
-    load_face_verts(mesh, fid, P);
-    int x0, x1, y0, y1;
-    emit = !bwd_small_face(v, P, x0, x1, y0, y1);
+// Exclusive scan of n counters (n + 1 outputs, out[n] = total) in three
+// kernels: per-block scans of 4096 values, a scan of the block sums, the add.
+constexpr int kScanBlock = 4096;
+
+__global__ __launch_bounds__(1024) void k_scan_blocks(const int32_t* __restrict__ in, int n,
+                                                      int32_t* __restrict__ out,
+                                                      int32_t* __restrict__ sums) {
+  __shared__ int32_t wsum[16];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int64_t b0 = int64_t(blockIdx.x) * kScanBlock + t * 4;
+  int v[4], tot = 0;
+  for (int k = 0; k < 4; k++) {
+    v[k] = b0 + k < n ? in[b0 + k] : 0;
+    tot += v[k];
   }
-  const uint64_t m = __ballot(emit);
-  if (m == 0) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(cursor, __popcll(m));
-  base = __shfl(base, leader);
-  if (emit) items[base + __popcll(m & ((1ull << lane) - 1ull))] = make_int4(fid, blockIdx.y, p, 0);
+  int x = tot;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  int base = 0, all = 0;
+  for (int w = 0; w < 16; w++) {
+    if (w < wid) base += wsum[w];
+    all += wsum[w];
+  }
+  base += x - tot;
+  for (int k = 0; k < 4; k++) {
+    if (b0 + k < n) out[b0 + k] = base;
+    base += v[k];
+  }
+  if (t == 0) sums[blockIdx.x] = all;
+}
+
+// one block: exclusive scan of the block sums in place, total in sums[nb]
+__global__ __launch_bounds__(1024) void k_scan_sums(int32_t* __restrict__ sums, int nb) {
+  __shared__ int32_t wsum[16];
+  __shared__ int32_t carry;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < nb; c0 += 1024) {
+    const int val = c0 + t < nb ? sums[c0 + t] : 0;
+    int x = val;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    int base = carry, all = 0;
+    for (int w = 0; w < 16; w++) {
+      if (w < wid) base += wsum[w];
+      all += wsum[w];
+    }
+    if (c0 + t < nb) sums[c0 + t] = base + x - val;
+    __syncthreads();
+    if (t == 0) carry += all;
+    __syncthreads();
+  }
+  if (t == 0) sums[nb] = carry;
+}
+
+__global__ void k_scan_add(int32_t* __restrict__ out, int n, const int32_t* __restrict__ sums,
+                           int nb) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += sums[i / kScanBlock];
+  if (i == 0) out[n] = sums[nb];
 }
 
 // Sums `val` over contiguous runs of equal `run` ids; the run's first lane
@@ -1404,19 +1405,30 @@ int raster_blocks_per_cu(int tw, int th, int mode) {
   return e == hipSuccess ? nb : 0;
 }
 
+// Scratch of the backward pass: cnt (F + 1), off (F + 1) and the block sums.
+int64_t backward_scratch_ints(int32_t F) {
+  return 2 * (int64_t(F) + 1) + (int64_t(F) + kScanBlock - 1) / kScanBlock + 1;
+}
+
 int launch_backward(hipStream_t st, const DevView* views, int n_views,
                     int max_pixels, int64_t total_pixels, const DevMesh& mesh,
-                    const DevGrads& g, int4* items, int32_t* cursor) {
-  if (max_pixels <= 0 || total_pixels <= 0) return 0;
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, cursor, 1);
-  if (mesh.F > 1)
-    hipLaunchKernelGGL(k_bwd_collect_faces, dim3((mesh.F + 255) / 256), dim3(256), 0, st,
-                       views, n_views, mesh, items, cursor);
-  hipLaunchKernelGGL(k_bwd_collect_pixels, dim3((max_pixels + 255) / 256, n_views), dim3(256),
-                     0, st, views, mesh, items, cursor);
-  // at most one item per pixel; blocks past the item count exit at once
+                    const DevGrads& g, int4* items, int32_t* scratch) {
+  if (max_pixels <= 0 || total_pixels <= 0 || mesh.F <= 0) return 0;
+  const int F = mesh.F;
+  int32_t* cnt = scratch;
+  int32_t* off = scratch + (F + 1);
+  int32_t* sums = off + (F + 1);
+  const int nb = (F + kScanBlock - 1) / kScanBlock;
+  const dim3 pgrid((max_pixels + 255) / 256, n_views);
+  hipLaunchKernelGGL(k_zero_i32, dim3((F + 256) / 256), dim3(256), 0, st, cnt, F + 1);
+  hipLaunchKernelGGL(k_bwd_count, pgrid, dim3(256), 0, st, views, F, cnt);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, cnt, F, off, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, sums, nb);
+  hipLaunchKernelGGL(k_scan_add, dim3((F + 255) / 256), dim3(256), 0, st, off, F, sums, nb);
+  hipLaunchKernelGGL(k_bwd_scatter, pgrid, dim3(256), 0, st, views, F, off, cnt, items);
+  // at most one item per pixel; blocks past the item count (off[F]) exit at once
   hipLaunchKernelGGL(k_bwd_items, dim3(static_cast<unsigned>((total_pixels + 255) / 256)),
-                     dim3(256), 0, st, views, mesh, items, cursor, g);
+                     dim3(256), 0, st, views, mesh, items, off + F, g);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -29,9 +29,10 @@ int launch_raster(hipStream_t st, int tw, int th, int mode, const DevView* views
                   const DevMesh& mesh, const float4* proj, const DevBins& bins,
                   const DevOptions& opt, int persistent_blocks);
 int raster_blocks_per_cu(int tw, int th, int mode);
+int64_t backward_scratch_ints(int32_t F);
 int launch_backward(hipStream_t st, const DevView* views, int n_views,
                     int max_pixels, int64_t total_pixels, const DevMesh& mesh,
-                    const DevGrads& g, int4* items, int32_t* cursor);
+                    const DevGrads& g, int4* items, int32_t* scratch);
 int launch_depth_to_u16(hipStream_t st, const float* d, int64_t n, uint16_t* out);
 }  // namespace ugu_hip
 
@@ -105,6 +106,7 @@ struct ugu_hip_ctx {
   // work buffers
   DevBuf proj, counts, tile_view, bins, status, views, tile_base, stage_out, diff_fn;
   DevBuf bwd_items;  // (face, view, pixel) items of the backward pass, 16 B each
+  DevBuf bwd_scratch;  // per-face counts and offsets of the backward pass
   DevBuf soup, diff_soup;  // 48-byte face records (prepared mesh / differentiable path)
   int64_t buf_gen = 0;
   void* h_views = nullptr;
@@ -587,7 +589,7 @@ void ugu_hip_destroy(ugu_hip_ctx* c) {
                     &c->vcolors, &c->uv, &c->uvidx, &c->matid, &c->texinfo,
                     &c->texdata, &c->proj, &c->counts, &c->bins, &c->status,
                     &c->views, &c->tile_base, &c->stage_out, &c->diff_fn, &c->tile_view,
-                    &c->bwd_items, &c->soup, &c->diff_soup};
+                    &c->bwd_items, &c->bwd_scratch, &c->soup, &c->diff_soup};
   for (DevBuf* b : bufs) b->release();
   if (c->h_views) (void)hipHostFree(c->h_views);
   if (c->h_status) (void)hipHostFree(c->h_status);
@@ -965,7 +967,7 @@ int ugu_hip_diff_backward(ugu_hip_ctx* c, const UguCamera* cams, int n,
   }
   if (total_px > INT32_MAX) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "too many pixels");
   if ((err = ensure(c, c->bwd_items, size_t(total_px) * 16))) return err;
-  if ((err = ensure(c, c->status, 64))) return err;
+  if ((err = ensure(c, c->bwd_scratch, size_t(backward_scratch_ints(dm.F)) * 4))) return err;
   UGU_HIP_CHECK(c, hipMemcpyAsync(c->views.p, c->h_views, size_t(n) * sizeof(DevView),
                                   hipMemcpyHostToDevice, c->stream));
   const DevView* dv = c->views.as<DevView>();
@@ -973,9 +975,8 @@ int ugu_hip_diff_backward(ugu_hip_ctx* c, const UguCamera* cams, int n,
   g.verts = grad_vertices;
   g.colors = p->vertex_colors ? grad_vertex_colors : nullptr;
   g.texture = p->texture ? grad_texture : nullptr;
-  // item cursor: word 8 of the status block (the forward pipeline uses 0..3)
-  if (launch_backward(c->stream, dv, n, max_px, total_px, dm, g,
-                      c->bwd_items.as<int4>(), c->status.as<int32_t>() + 8))
+  if (launch_backward(c->stream, dv, n, max_px, total_px, dm, g, c->bwd_items.as<int4>(),
+                      c->bwd_scratch.as<int32_t>()))
     return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
   UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return UGU_HIP_OK;
