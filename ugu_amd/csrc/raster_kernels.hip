@@ -197,15 +197,16 @@ UGU_DEV int xcd_remap(int b, int nb) {
 // ---------------------------------------------------------------------------
 // k_clear
 
-// Two counters per tile (front and back segments of its bin). Also fills the
-// tile -> view table (largest v with tile_base[v] <= tile),
+// Four words per tile: front and back segment counters and the minimum skip
+// depth of the back segment. Also fills the tile -> view table (largest v with tile_base[v] <= tile),
 // so the rasterizer finds its view with one load instead of a search.
 __global__ void k_clear(int32_t* counts, int32_t n, int32_t* status,
                         const int32_t* __restrict__ tile_base, int32_t n_views,
                         int32_t* __restrict__ tile_view) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    reinterpret_cast<int2*>(counts)[i] = make_int2(0, 0);
+    // front count, back count, back-segment skip-depth minimum (bits; +inf)
+    reinterpret_cast<int4*>(counts)[i] = make_int4(0, 0, 0x7f800000, 0);
     int lo = 0, hi = n_views - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -281,9 +282,9 @@ __global__ void k_face_soup(const float* __restrict__ verts, const int32_t* __re
 // ---------------------------------------------------------------------------
 // k_setup_bin: one thread per (face, view)
 
-// Appends `entry` to a bin segment for every lane with `has`. `seg` = 2 * tile
+// Appends `entry` to a bin segment for every lane with `has`. `seg` = 4 * tile
 // for the front segment (filled upwards from the bin's first slot) and
-// 2 * tile + 1 for the back segment (filled downwards from its last slot).
+// 4 * tile + 1 for the back segment (filled downwards from its last slot).
 // Lanes that target the same segment form a group (ballot + popcount rank);
 // the group leaders then issue their atomicAdds together, so a wavefront
 // waits for one atomic round trip per call however many segments it hits.
@@ -294,6 +295,17 @@ UGU_DEV void wave_append(const DevBins& bins, int seg, int2 entry, bool has) {
   uint64_t todo = __ballot(has);
   if (!todo) return;
   const int lane = __lane_id();
+  // Lower bound of the skip depths of this wavefront's back-segment entries
+  // (0 when one has no bound): the leaders of back groups fold it into their
+  // tile's back-segment minimum.
+  const bool back = has && (seg & 1);
+  uint32_t zmin = 0xffffffffu;
+  if (__ballot(back)) {
+    const float zs = __int_as_float(entry.y);
+    zmin = back ? (zs >= 0.0f ? __float_as_uint(zs) : 0u) : 0xffffffffu;
+    for (int off = 32; off > 0; off >>= 1)
+      zmin = min(zmin, static_cast<uint32_t>(__shfl_xor(static_cast<int>(zmin), off)));
+  }
   int leader = 0, rank = 0, size = 0;
   while (todo) {
     const int l = __ffsll(static_cast<unsigned long long>(todo)) - 1;
@@ -308,12 +320,15 @@ UGU_DEV void wave_append(const DevBins& bins, int seg, int2 entry, bool has) {
     todo &= ~grp;
   }
   int base = 0;
-  if (has && lane == leader) base = atomicAdd(&bins.counts[seg], size);
+  if (has && lane == leader) {
+    base = atomicAdd(&bins.counts[seg], size);
+    if (back) atomicMin(reinterpret_cast<unsigned int*>(bins.counts) + (seg + 1), zmin);
+  }
   base = __shfl(base, leader);
   if (has) {
     const int slot = base + rank;
     if (slot < bins.cap) {
-      const int64_t row = int64_t(seg >> 1) * bins.cap;
+      const int64_t row = int64_t(seg >> 2) * bins.cap;
       bins.bins[row + ((seg & 1) ? bins.cap - 1 - slot : slot)] = entry;
     }
   }
@@ -398,7 +413,7 @@ __global__ __launch_bounds__(256) void k_setup_bin(
     }
     int tx = tx0, ty = ty0;
     while (__any(has)) {
-      const int seg = has ? 2 * (v.tile_base + ty * v.tiles_x + tx) + (back ? 1 : 0) : -1;
+      const int seg = has ? 4 * (v.tile_base + ty * v.tiles_x + tx) + (back ? 1 : 0) : -1;
       wave_append(bins, seg, make_int2(f, __float_as_int(zskip)), has);
       if (has) {
         if (++tx > tx1) {
@@ -677,7 +692,7 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   constexpr int NT = TW * TH;
   const int t = threadIdx.x;
 
-  const int2 seg = reinterpret_cast<const int2*>(bins.counts)[gtile];
+  const int4 seg = reinterpret_cast<const int4*>(bins.counts)[gtile];
   const int count = seg.x + seg.y;
   const DevView& v = views[bins.tile_view[gtile]];
   const int lt = gtile - v.tile_base;
@@ -754,6 +769,9 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   // the depths the front faces left.
   float dmax = __uint_as_float(0xffffffffu);
   for (int ch = 0; ch < chunks; ch++) {
+    // the whole back segment is skipped when even its smallest skip depth
+    // exceeds the tile's depth bound
+    if (ch == chunks_a && skip_face(__uint_as_float(static_cast<uint32_t>(seg.z)), dmax)) break;
     const int base = ch < chunks_a ? ch * NT : nA + (ch - chunks_a) * NT;
     const int end = ch < chunks_a ? nA : n;
     const bool more = ch + 1 < chunks;

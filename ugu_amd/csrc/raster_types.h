@@ -62,7 +62,8 @@ struct DevOptions {
 };
 
 struct DevBins {
-  int32_t* counts;   // per global tile: front / back segment counters
+  int32_t* counts;   // per global tile: front / back segment counters, minimum
+                     // skip depth of the back segment (float bits), pad
   int2* bins;        // global tile t owns bins[t*cap .. t*cap+cap): front
                      // segment from the start, back segment from the end;
                      // entry = (face id, bits of its skip depth)

@@ -346,7 +346,7 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
 
   int err;
   if ((err = ensure(c, c->proj, 16))) return err;
-  if ((err = ensure(c, c->counts, size_t(total_tiles) * 8))) return err;  // 2 per tile
+  if ((err = ensure(c, c->counts, size_t(total_tiles) * 16))) return err;  // 4 per tile
   if ((err = ensure(c, c->tile_view, size_t(total_tiles) * 4))) return err;
   if ((err = ensure(c, c->status, 64))) return err;
   if ((err = ensure(c, c->views, size_t(n) * sizeof(DevView)))) return err;
