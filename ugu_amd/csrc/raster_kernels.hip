@@ -380,9 +380,36 @@ __global__ __launch_bounds__(256) void k_setup_bin(
     const DevView* __restrict__ views, int n_views, DevMesh mesh,
     const float4* __restrict__ proj, DevBins bins) {
   (void)proj;
+  // The block's 256 faces share their vertices (about 150 distinct ones on a
+  // regular mesh): every distinct vertex is projected once per view into LDS
+  // and the faces read their corners from there, about 0.6 projections per
+  // face instead of three. Without the block lists each face projects its
+  // own corners.
+  __shared__ float4 sp[768];
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool shared = mesh.blk_off != nullptr;
+  int vbase = 0, nu = 0;
   float P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (f < mesh.F) {
+  int loc[3] = {0, 0, 0};
+  if (shared) {
+    vbase = mesh.blk_off[blockIdx.x];
+    nu = mesh.blk_off[blockIdx.x + 1] - vbase;
+    if (f < mesh.F) {
+      const uint32_t lc = mesh.face_loc[f];
+      loc[0] = lc & 1023u;
+      loc[1] = (lc >> 10) & 1023u;
+      loc[2] = lc >> 20;
+    }
+    for (int j = 0; j < 3; j++) {
+      const int k = threadIdx.x + j * 256;
+      if (k < nu) {
+        const float* q = mesh.verts + int64_t(mesh.blk_vid[vbase + k]) * 3;
+        P[j * 3 + 0] = q[0];
+        P[j * 3 + 1] = q[1];
+        P[j * 3 + 2] = q[2];
+      }
+    }
+  } else if (f < mesh.F) {
     float N[3];
     load_face_rec(mesh.soup, f, P, N);
   }
@@ -393,10 +420,23 @@ __global__ __launch_bounds__(256) void k_setup_bin(
     bool has = false, back = false;
     float zskip = -1.0f;
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+    float4 a, b, c;
+    if (shared) {
+      for (int j = 0; j < 3; j++) {
+        const int k = threadIdx.x + j * 256;
+        if (k < nu) sp[k] = project_point(v, P[j * 3 + 0], P[j * 3 + 1], P[j * 3 + 2]);
+      }
+      __syncthreads();
+      a = sp[loc[0]];
+      b = sp[loc[1]];
+      c = sp[loc[2]];
+      __syncthreads();  // the next view overwrites the window
+    } else {
+      a = project_point(v, P[0], P[1], P[2]);
+      b = project_point(v, P[3], P[4], P[5]);
+      c = project_point(v, P[6], P[7], P[8]);
+    }
     if (f < mesh.F) {
-      const float4 a = project_point(v, P[0], P[1], P[2]);
-      const float4 b = project_point(v, P[3], P[4], P[5]);
-      const float4 c = project_point(v, P[6], P[7], P[8]);
       int x0, x1, y0, y1;
       float area;
       if (face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) {

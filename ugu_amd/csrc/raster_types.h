@@ -48,6 +48,11 @@ struct DevMesh {
   const float4* soup;       // F x 3: the face's three vertex positions and its
                             // face normal (zero when the mesh has none),
                             // packed (x0 y0 z0 x1)(y1 z1 x2 y2)(z2 nx ny nz)
+  const int32_t* blk_off;   // per 256-face setup block: first entry of its
+                            // distinct vertices in blk_vid (nblocks + 1), or null
+  const int32_t* blk_vid;   // distinct vertex ids of each block
+  const uint32_t* face_loc; // per face: its corners' indices in the block's
+                            // list, 10 bits each
   const float* ftex;        // differentiable path: f32 RGB texture or null
   int32_t ftex_w, ftex_h;
 };

@@ -108,6 +108,8 @@ struct ugu_hip_ctx {
   DevBuf bwd_items;  // (face, view, pixel) items of the backward pass, 16 B each
   DevBuf bwd_scratch;  // per-face counts and offsets of the backward pass
   DevBuf soup, diff_soup;  // 48-byte face records (prepared mesh / differentiable path)
+  DevBuf blk_off, blk_vid, face_loc;           // setup blocks' vertex lists (prepared mesh)
+  DevBuf blk_off_o, blk_vid_o, face_loc_o;     // the same for the original face array
   int64_t buf_gen = 0;
   void* h_views = nullptr;
   size_t h_views_bytes = 0;
@@ -204,6 +206,44 @@ std::vector<float> gather_rows(const float* src, int32_t rows, int cols,
   return out;
 }
 
+// Distinct vertices of every block of 256 faces (the setup kernel's blocks)
+// and each face's corners as 10-bit indices into its block's list.
+int upload_block_lists(ugu_hip_ctx* c, const int32_t* faces, int64_t F, DevBuf* off,
+                       DevBuf* vid, DevBuf* loc) {
+  if (F <= 0) {
+    off->release();
+    vid->release();
+    loc->release();
+    return 0;
+  }
+  const int64_t nb = (F + 255) / 256;
+  std::vector<int32_t> o(size_t(nb + 1)), ids;
+  std::vector<uint32_t> l(static_cast<size_t>(F));
+  ids.reserve(size_t(F));
+  std::vector<int32_t> u;
+  for (int64_t b = 0; b < nb; b++) {
+    const int64_t f0 = b * 256, f1 = std::min<int64_t>(f0 + 256, F);
+    u.assign(faces + f0 * 3, faces + f1 * 3);
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    o[size_t(b)] = static_cast<int32_t>(ids.size());
+    ids.insert(ids.end(), u.begin(), u.end());
+    for (int64_t f = f0; f < f1; f++) {
+      uint32_t packed = 0;
+      for (int k = 0; k < 3; k++) {
+        const auto it = std::lower_bound(u.begin(), u.end(), faces[f * 3 + k]);
+        packed |= static_cast<uint32_t>(it - u.begin()) << (10 * k);
+      }
+      l[size_t(f)] = packed;
+    }
+  }
+  o[size_t(nb)] = static_cast<int32_t>(ids.size());
+  int err;
+  if ((err = upload(c, *off, o.data(), o.size() * 4))) return err;
+  if ((err = upload(c, *vid, ids.data(), ids.size() * 4))) return err;
+  return upload(c, *loc, l.data(), l.size() * 4);
+}
+
 bool check_idx(const int32_t* idx, int64_t n, int32_t limit) {
   for (int64_t i = 0; i < n; i++)
     if (idx[i] < 0 || idx[i] >= limit) return false;
@@ -292,6 +332,9 @@ DevMesh dev_mesh(const ugu_hip_ctx* c) {
   m.faces = c->faces.as<int32_t>();
   m.fnormals = c->has_fn ? c->fnormals.as<float>() : nullptr;
   m.soup = c->soup.as<float4>();
+  m.blk_off = c->F > 0 ? c->blk_off.as<int32_t>() : nullptr;
+  m.blk_vid = c->blk_vid.as<int32_t>();
+  m.face_loc = c->face_loc.as<uint32_t>();
   m.normals = c->has_normals ? c->normals.as<float>() : nullptr;
   m.nidx = c->has_nidx ? c->nidx.as<int32_t>() : nullptr;
   m.vcolors = c->has_vc ? c->vcolors.as<float>() : nullptr;
@@ -589,7 +632,8 @@ void ugu_hip_destroy(ugu_hip_ctx* c) {
                     &c->vcolors, &c->uv, &c->uvidx, &c->matid, &c->texinfo,
                     &c->texdata, &c->proj, &c->counts, &c->bins, &c->status,
                     &c->views, &c->tile_base, &c->stage_out, &c->diff_fn, &c->tile_view,
-                    &c->bwd_items, &c->bwd_scratch, &c->soup, &c->diff_soup};
+                    &c->bwd_items, &c->bwd_scratch, &c->soup, &c->diff_soup, &c->blk_off,
+                    &c->blk_vid, &c->face_loc, &c->blk_off_o, &c->blk_vid_o, &c->face_loc_o};
   for (DevBuf* b : bufs) b->release();
   if (c->h_views) (void)hipHostFree(c->h_views);
   if (c->h_status) (void)hipHostFree(c->h_status);
@@ -665,6 +709,9 @@ int ugu_hip_prepare_mesh(ugu_hip_ctx* c, const UguMeshDesc* m) {
   }
 
   int err;
+  if ((err = upload_block_lists(c, faces_r, F, &c->blk_off, &c->blk_vid, &c->face_loc))) return err;
+  if ((err = upload_block_lists(c, m->faces, F, &c->blk_off_o, &c->blk_vid_o, &c->face_loc_o)))
+    return err;
   if ((err = upload(c, c->verts, pv.data(), size_t(m->num_vertices) * 12))) return err;
   if ((err = upload(c, c->faces, faces_r, size_t(F) * 12))) return err;
   if ((err = upload(c, c->faces_orig, m->faces, size_t(F) * 12))) return err;
@@ -894,6 +941,10 @@ static int diff_mesh(ugu_hip_ctx* c, const UguDiffParams* p, DevMesh* dm) {
   if ((err = ensure(c, c->diff_soup, size_t(std::max(c->F, 1)) * 48))) return err;
   dm->fnormals = c->diff_fn.as<float>();
   dm->soup = c->diff_soup.as<float4>();
+  // setup block lists of the original face array (user vertex ids)
+  dm->blk_off = c->F > 0 ? c->blk_off_o.as<int32_t>() : nullptr;
+  dm->blk_vid = c->blk_vid_o.as<int32_t>();
+  dm->face_loc = c->face_loc_o.as<uint32_t>();
   return 0;
 }
 
