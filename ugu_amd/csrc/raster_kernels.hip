@@ -869,14 +869,11 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
     }
     const int pre = wbase + x - c;  // first pair of this thread's face
     const bool has_fn = mesh.fnormals != nullptr;
-    // One (face, pixel) pair: coverage, depth and the packed LDS atomicMin.
-    auto test_pair = [&](int j, int r) {
-      const int bx = s.box[j];
-      const int nx = (bx >> 16) & 0xff;
-      const int qy = r / nx;
-      const int pxl = (bx & 0xff) + (r - qy * nx);
-      const int pyl = ((bx >> 8) & 0xff) + qy;
-      const int pix = pyl * TW + pxl;
+    // One (face slot, tile pixel) pair: coverage, depth and the packed LDS
+    // atomicMin.
+    auto test_pair = [&](int j, int pix) {
+      const int pxl = static_cast<int>(static_cast<unsigned>(pix) % TW);
+      const int pyl = static_cast<int>(static_cast<unsigned>(pix) / TW);
       const float rx = s.ray[0][pix], ry = s.ray[1][pix], rz = s.ray[2][pix];
       const bool bf = has_fn && sum3(s.fv[10][j] * rx, s.fv[11][j] * ry, s.fv[12][j] * rz) > 0.0f;
       float z, w0, w1, w2;
@@ -898,11 +895,26 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
       }
     };
     if (big <= kScatterMaxPairs) {
-      // small faces: every face writes its pair list (slot, index) into LDS,
-      // then every lane takes one pair
+      // small faces: every face writes its pairs (slot | tile pixel << 8)
+      // into an LDS list, then every lane takes one pair
       for (int r0 = 0; r0 < total; r0 += 2 * NT) {
         const int lo = max(pre, r0), hi = min(pre + c, r0 + 2 * NT);
-        for (int q = lo; q < hi; q++) s.pairs[q - r0] = t | ((q - pre) << 8);
+        if (lo < hi) {
+          const int bx = s.box[t];
+          const int nx = (bx >> 16) & 0xff, x0l = bx & 0xff;
+          int ry = (lo - pre) / nx;
+          int rx = (lo - pre) - ry * nx;
+          int pix = (((bx >> 8) & 0xff) + ry) * TW + x0l + rx;
+          for (int q = lo; q < hi; q++) {
+            s.pairs[q - r0] = t | (pix << 8);
+            if (++rx == nx) {
+              rx = 0;
+              pix += TW - nx + 1;
+            } else {
+              pix++;
+            }
+          }
+        }
         __syncthreads();
         const int np = min(total - r0, 2 * NT);
         for (int pp = t; pp < np; pp += NT) {
@@ -923,7 +935,11 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
           const int mid = (jl + jh + 1) >> 1;
           if (s.pairs[mid] <= p) jl = mid; else jh = mid - 1;
         }
-        test_pair(jl, p - s.pairs[jl]);
+        const int r = p - s.pairs[jl];
+        const int bx = s.box[jl];
+        const int nx = (bx >> 16) & 0xff;
+        const int qy = r / nx;
+        test_pair(jl, (((bx >> 8) & 0xff) + qy) * TW + (bx & 0xff) + (r - qy * nx));
       }
       __syncthreads();
     }
