@@ -8,8 +8,9 @@
 
 #include "ugu/renderer/cpu/renderer.h"
 
-// Feature test for the batched extension below (absent from UGU's header).
+// Feature tests for the extensions below (absent from UGU's header).
 #define UGU_RASTERIZER_HAS_RENDER_BATCH 1
+#define UGU_RASTERIZER_HAS_RENDER_GBUFFER 1
 
 namespace ugu {
 
@@ -45,6 +46,17 @@ class Rasterizer : public RendererCpu {
                    std::vector<Image3b>* color, std::vector<Image1f>* depth,
                    std::vector<Image3f>* normal, std::vector<Image1b>* mask,
                    std::vector<Image1i>* face_id) const;
+
+  // Extension (no counterpart in ugu::Rasterizer; UGU fills GBuffer from
+  // OpenGL): the current camera's G-buffer from the rasterizer's resolve.
+  // color = shaded = the Render colour; normal_cam = the Render normal;
+  // stencil = the Render mask; face_id = the Render face id (-1 where no
+  // face); normal_wld = the shading normal in world coordinates; pos_wld,
+  // pos_cam = the surface point (perspective-correct); bary = (u, v,
+  // (1 - u) - v) and uv = (s, t, (1 - s) - t) with u, v the weights the
+  // colourizers use; geo_id = 0 on covered pixels, -1 elsewhere; depth_01
+  // (an OpenGL depth-buffer value) stays 0. Uncovered pixels are 0.
+  bool RenderGBuffer(GBuffer* gbuf) const;
 };
 
 }  // namespace ugu

@@ -19,6 +19,11 @@
  *   ugu_hip_render_batch
  *       no single reference call: replaces the per-view loop of Render calls
  *       (examples/ex02_renderer.cc:135-169)
+ *   ugu_hip_render_gbuffer
+ *       no CPU-rasterizer counterpart: the fields of ugu::GBuffer
+ *       (include/ugu/renderer/base.h:34-77, filled by the OpenGL renderer in
+ *       src/renderer/gl/renderer.cc:501-647) that the rasterizer's resolve
+ *       can produce, next to the Render outputs
  *   ugu_hip_depth_to_u16
  *       the ConvertTo step of Rasterizer::RenderW (src/renderer/cpu/rasterizer.cc:329-344,
  *       include/ugu/image_proc.h:453-520)
@@ -47,7 +52,7 @@
 extern "C" {
 #endif
 
-#define UGU_HIP_ABI_VERSION 1
+#define UGU_HIP_ABI_VERSION 2
 
 typedef struct ugu_hip_ctx ugu_hip_ctx;
 
@@ -186,8 +191,8 @@ int ugu_hip_render(ugu_hip_ctx* ctx, const UguCamera* camera,
                    int memory);
 
 /* Render n views of possibly different resolutions in one batched pipeline
- * (clear, project, setup+bin, raster launched back to back, captured once in
- * a hipGraph and replayed). outputs[i] belongs to cameras[i]. Results are
+ * (clear, setup+bin, raster launched back to back, captured once in a
+ * hipGraph and replayed). outputs[i] belongs to cameras[i]. Results are
  * bitwise identical to n separate ugu_hip_render calls. */
 int ugu_hip_render_batch(ugu_hip_ctx* ctx, const UguCamera* cameras, int n,
                          const UguOptions* options, const UguOutputs* outputs,
@@ -198,6 +203,36 @@ int ugu_hip_render_batch(ugu_hip_ctx* ctx, const UguCamera* cameras, int n,
 int ugu_hip_render_batch_eager(ugu_hip_ctx* ctx, const UguCamera* cameras,
                                int n, const UguOptions* options,
                                const UguOutputs* outputs, int memory);
+
+/* G-buffer planes of a view (any may be null; memory kind as the outputs).
+ * A pixel is covered where the mask output is 255 (face id > 0 and not
+ * culled); with u = w1, v = w2 the perspective-correct weights of the face's
+ * second and third vertex (the colourizers' u, v) and w = (1 - u) - v:
+ *   bary       (u, v, (1 - u) - v)                       as GBuffer::bary
+ *   uv         (s, t, (1 - s) - t), (s, t) = w uv0 + u uv1 + v uv2, the
+ *              texture coordinate the texture colourizers sample (zero if the
+ *              mesh has no uv)
+ *   pos_wld    w P0 + u P1 + v P2 (world position on the face)
+ *   pos_cam    w2c_t + w2c_R pos_wld
+ *   normal_wld the world-space shading normal (face normal, or the
+ *              interpolated vertex normal as the reference's Render uses it;
+ *              the normal output is w2c_R times this)
+ *   geo_id     0 (one geometry)
+ * Every other pixel is 0 (geo_id -1). Float planes are H x W x 3. */
+typedef struct {
+  float* bary;
+  float* uv;
+  float* pos_wld;
+  float* pos_cam;
+  float* normal_wld;
+  int32_t* geo_id;
+} UguGBuffer;
+
+/* ugu_hip_render_batch plus G-buffer planes: gbuffers[i] belongs to
+ * cameras[i]; the Render outputs are bitwise the same as without them. */
+int ugu_hip_render_gbuffer(ugu_hip_ctx* ctx, const UguCamera* cameras, int n,
+                           const UguOptions* options, const UguOutputs* outputs,
+                           const UguGBuffer* gbuffers, int memory);
 
 /* Tiling control and the autotuner. ugu_hip_autotune renders the given views
  * with every tile size, sizes the bin capacity from the measured maximum bin

@@ -35,6 +35,11 @@ class OracleCamera(ctypes.Structure):
                 ("w2c_R", _f * 9), ("w2c_t", _f * 3), ("c2w_R", _f * 9)]
 
 
+class OracleGBuf(ctypes.Structure):
+    _fields_ = [("bary", _P), ("uv", _P), ("pos_wld", _P), ("pos_cam", _P),
+                ("normal_wld", _P), ("geo_id", _P)]
+
+
 class OracleOptions(ctypes.Structure):
     _fields_ = [("diffuse_color", _i), ("interp", _i), ("shading_normal", _i),
                 ("diffuse_shading", _i), ("depth_scale", _f), ("backface_culling", _i),
@@ -63,6 +68,11 @@ def lib():
         L.oracle_render.argtypes = [ctypes.POINTER(OracleMesh), ctypes.POINTER(OracleCamera), _P,
                                     ctypes.POINTER(OracleOptions), _P, _P, _P, _P, _P,
                                     ctypes.POINTER(ctypes.c_double)]
+        L.oracle_render_gbuffer.restype = ctypes.c_int
+        L.oracle_render_gbuffer.argtypes = [ctypes.POINTER(OracleMesh), ctypes.POINTER(OracleCamera),
+                                            _P, ctypes.POINTER(OracleOptions), _P, _P, _P, _P, _P,
+                                            ctypes.POINTER(OracleGBuf),
+                                            ctypes.POINTER(ctypes.c_double)]
         L.oracle_depth_to_u16.restype = None
         L.oracle_depth_to_u16.argtypes = [_P, ctypes.c_int64, _P]
         L.oracle_edge_function.restype = ctypes.c_float
@@ -186,6 +196,31 @@ def render(mesh, cam, options, c2w_t=None, outputs=("color", "depth", "normal", 
     rc = lib().oracle_render(ctypes.byref(hold.m), ctypes.byref(cam), _p(ct), ctypes.byref(opt),
                              _p(out.get("color")), _p(out.get("depth")), _p(out.get("normal")),
                              _p(out.get("mask")), _p(out.get("face_id")), ctypes.byref(ms))
+    return rc, out, ms.value
+
+
+GBUFFER_PLANES = ("bary", "uv", "pos_wld", "pos_cam", "normal_wld", "geo_id")
+
+
+def render_gbuffer(mesh, cam, options, c2w_t=None):
+    """oracle_render_gbuffer: the five Render outputs and the six G-buffer
+    planes (ugu_oracle.cc OracleGBuf) -> (rc, dict, elapsed_ms)."""
+    h, w = cam.height, cam.width
+    out = {"color": np.zeros((h, w, 3), np.uint8), "depth": np.zeros((h, w), np.float32),
+           "normal": np.zeros((h, w, 3), np.float32), "mask": np.zeros((h, w), np.uint8),
+           "face_id": np.zeros((h, w), np.int32)}
+    for k in GBUFFER_PLANES[:5]:
+        out[k] = np.zeros((h, w, 3), np.float32)
+    out["geo_id"] = np.zeros((h, w), np.int32)
+    g = OracleGBuf(*(_p(out[k]) for k in GBUFFER_PLANES))
+    hold = _MeshHold(mesh)
+    opt = options_struct(options)
+    ct = None if c2w_t is None else np.ascontiguousarray(c2w_t, np.float32)
+    ms = ctypes.c_double(0.0)
+    rc = lib().oracle_render_gbuffer(ctypes.byref(hold.m), ctypes.byref(cam), _p(ct),
+                                     ctypes.byref(opt), _p(out["color"]), _p(out["depth"]),
+                                     _p(out["normal"]), _p(out["mask"]), _p(out["face_id"]),
+                                     ctypes.byref(g), ctypes.byref(ms))
     return rc, out, ms.value
 
 

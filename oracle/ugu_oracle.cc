@@ -85,6 +85,19 @@ struct OracleCamera {
   float c2w_R[9];  // row-major, = c2w.matrix().block<3,3>(0,0).cast<float>()
 };
 
+// G-buffer planes (null = not requested): the fields of ugu::GBuffer
+// (include/ugu/renderer/base.h:34-77) the rasterizer's resolve can fill, as
+// defined by the library's UguGBuffer: covered = mask 255; u, v = the
+// winner's weights w1, w2; w = (1 - u) - v.
+struct OracleGBuf {
+  float* bary;        // (u, v, (1 - u) - v)
+  float* uv;          // (s, t, (1 - s) - t), (s, t) interpolated as the colourizers do
+  float* pos_wld;     // w P0 + u P1 + v P2
+  float* pos_cam;     // w2c_t + w2c_R pos_wld
+  float* normal_wld;  // shading normal
+  int32_t* geo_id;    // 0, -1 where not covered
+};
+
 struct OracleOptions {
   int32_t diffuse_color;    // 0 none, 1 texture, 2 vertex
   int32_t interp;           // 0 nn, 1 bilinear
@@ -382,19 +395,39 @@ extern "C" {
 // c2w_t (camera position) only feeds the org_ray_w table, as in InitRayTable.
 // Returns 0 or a negative error code; *elapsed_ms receives the wall time of
 // the main loop (rasterizer.cc:121-122,303-304), ray table included.
+int oracle_render_gbuffer(const OracleMesh* mesh, const OracleCamera* cam,
+                          const float* c2w_t, const OracleOptions* opt, uint8_t* color,
+                          float* depth, float* normal, uint8_t* mask, int32_t* face_id,
+                          const OracleGBuf* gbuf, double* elapsed_ms);
+
 int oracle_render(const OracleMesh* mesh, const OracleCamera* cam,
                   const float* c2w_t, const OracleOptions* opt, uint8_t* color,
                   float* depth, float* normal, uint8_t* mask, int32_t* face_id,
                   double* elapsed_ms) {
+  return oracle_render_gbuffer(mesh, cam, c2w_t, opt, color, depth, normal, mask, face_id,
+                               nullptr, elapsed_ms);
+}
+
+// oracle_render plus the G-buffer planes of *gbuf (may be null).
+int oracle_render_gbuffer(const OracleMesh* mesh, const OracleCamera* cam,
+                          const float* c2w_t, const OracleOptions* opt, uint8_t* color,
+                          float* depth, float* normal, uint8_t* mask, int32_t* face_id,
+                          const OracleGBuf* gbuf, double* elapsed_ms) {
   if (cam == nullptr) return kErrNoCamera;
   if (mesh == nullptr) return kErrMeshNotInitialized;
   const OracleMesh& m = *mesh;
   const OracleOptions& o = *opt;
-  int err = Validate(m, o, color || depth || normal || mask || face_id);
+  const OracleGBuf g = gbuf ? *gbuf : OracleGBuf{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  const bool any_g = g.bary || g.uv || g.pos_wld || g.pos_cam || g.normal_wld || g.geo_id;
+  int err = Validate(m, o, color || depth || normal || mask || face_id || any_g);
   if (err) return err;
 
   const int W = cam->width, H = cam->height;
   const size_t npx = static_cast<size_t>(W) * H;
+  float* const gplanes[5] = {g.bary, g.uv, g.pos_wld, g.pos_cam, g.normal_wld};
+  for (float* gp : gplanes)
+    if (gp) std::fill(gp, gp + npx * 3, 0.0f);
+  if (g.geo_id) std::fill(g.geo_id, g.geo_id + npx, -1);
   // ValidateAndInitBeforeRender: colour 0, depth 0, normal 0, mask 0, fid -1
   if (color) std::memset(color, 0, npx * 3);
   if (depth) std::fill(depth, depth + npx, 0.0f);
@@ -553,6 +586,39 @@ int oracle_render(const OracleMesh* mesh, const OracleCamera* cam,
             ShadePixel(sc, color + pi * 3, w2, w1, fid, ray_w, light_dir, sn);
           else
             ShadePixel(sc, color + pi * 3, w1, w2, fid, ray_w, light_dir, sn);
+        }
+        if (any_g) {
+          const float u = w1, v = w2, w = 1.0f - u - v;
+          if (g.bary) {
+            g.bary[pi * 3 + 0] = MUT(14) ? v : u;
+            g.bary[pi * 3 + 1] = MUT(14) ? u : v;
+            g.bary[pi * 3 + 2] = 1.0f - u - v;
+          }
+          if (g.uv && m.uv && m.uv_indices) {
+            float st[2];
+            interp_uv(m, fid, u, v, st);
+            g.uv[pi * 3 + 0] = st[0];
+            g.uv[pi * 3 + 1] = st[1];
+            g.uv[pi * 3 + 2] = 1.0f - st[0] - st[1];
+          }
+          if (g.pos_wld || g.pos_cam) {
+            const int32_t* f = m.faces + static_cast<int64_t>(fid) * 3;
+            const V3 p0 = load3(m.vertices, f[0]), p1 = load3(m.vertices, f[1]),
+                     p2 = load3(m.vertices, f[2]);
+            V3 pw;
+            for (int k = 0; k < 3; k++) pw[k] = w * p0[k] + u * p1[k] + v * p2[k];
+            float Rt[9];
+            for (int a = 0; a < 3; a++)
+              for (int b = 0; b < 3; b++) Rt[a * 3 + b] = cam->w2c_R[b * 3 + a];
+            const V3 rp = matvec(MUT(15) ? Rt : cam->w2c_R, pw);
+            for (int k = 0; k < 3; k++) {
+              if (g.pos_wld) g.pos_wld[pi * 3 + k] = pw[k];
+              if (g.pos_cam) g.pos_cam[pi * 3 + k] = cam->w2c_t[k] + rp[k];
+            }
+          }
+          if (g.normal_wld)
+            for (int k = 0; k < 3; k++) g.normal_wld[pi * 3 + k] = sn[k];
+          if (g.geo_id) g.geo_id[pi] = 0;
         }
       }
     }

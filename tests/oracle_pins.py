@@ -373,6 +373,86 @@ def pin_depth_u16():
     return True, ""
 
 
+def pin_gbuffer():
+    """G-buffer planes on the tilted quad against ray-plane hits: world and
+    camera positions, the uv of an affine uv map, barycentrics of the hit
+    point in its face; normal output = w2c_R * normal_wld in float32; geo_id
+    0 exactly where the mask is set."""
+    uvs = None
+    m, cam = _tilted_quad()
+    uvs = np.stack([(m.vertices[:, 0] + 1.0) / 2.0, (m.vertices[:, 1] + 0.8) / 1.6], 1)
+    m.set_uv(uvs.astype(np.float32))
+    m.set_uv_indices(m.vertex_indices)
+    # a second pose: the same world, camera rotated about y and moved, so that
+    # pos_cam differs from pos_wld by a non-symmetric rotation
+    th = 0.3
+    Ry = np.array([[math.cos(th), 0, math.sin(th)], [0, 1, 0], [-math.sin(th), 0, math.cos(th)]])
+    for pose in ("identity", "rotated"):
+        c2w = np.eye(4)
+        if pose == "rotated":
+            c2w[:3, :3] = Ry
+            c2w[:3, 3] = (-0.6, 0.1, 0.2)
+        cam2 = PinholeCamera(64, 48, c2w, (31.5, 23.5), (40.0, 40.0))
+        opt = RendererCpuOption(diffuse_color=DiffuseColor.kVertex)
+        rc, out, _ = O.render_gbuffer(m, O.camera_from_abi(cam2.to_abi()), opt,
+                                      c2w_t=cam2.c2w[:3, 3])
+        if rc != 0:
+            return False, f"render_gbuffer rc {rc}"
+        a = cam2.to_abi()
+        R = np.array(a.w2c_R[:], np.float64).reshape(3, 3)
+        t = np.array(a.w2c_t[:], np.float64)
+        C = R.T
+        o = -R.T @ t                      # camera centre
+        ys, xs = np.mgrid[0:48, 0:64].astype(np.float64)
+        d = np.stack([(xs - 31.5) / 40.0, (ys - 23.5) / 40.0, np.ones_like(xs)], -1) @ C.T
+        # plane z = 2 + 0.8 x  <=>  n . X = 2 with n = (-0.8, 0, 1)
+        n = np.array([-0.8, 0.0, 1.0])
+        s_ = (2.0 - o @ n) / (d @ n)
+        hit = o + d * s_[..., None]
+        cov = out["mask"] == 255
+        inner = cov & (np.abs(hit[..., 0]) < 0.97) & (np.abs(hit[..., 1]) < 0.77)
+        if inner.sum() < 300:
+            return False, f"{pose}: quad not covered"
+        if not np.allclose(out["pos_wld"][inner], hit[inner], rtol=2e-5, atol=2e-5):
+            return False, f"{pose}: pos_wld is not the ray-plane hit"
+        pc = hit @ R.T + t
+        if not np.allclose(out["pos_cam"][inner], pc[inner], rtol=2e-5, atol=2e-5):
+            return False, f"{pose}: pos_cam != w2c_R pos_wld + w2c_t"
+        uv_exp = np.stack([(hit[..., 0] + 1) / 2, (hit[..., 1] + 0.8) / 1.6], -1)
+        if not np.allclose(out["uv"][inner][:, :2], uv_exp[inner], atol=2e-5):
+            return False, f"{pose}: uv is not the affine map at the hit"
+        if not np.allclose(out["uv"][inner][:, 2], 1 - uv_exp[inner].sum(-1), atol=4e-5):
+            return False, f"{pose}: uv third component"
+        V = m.vertices.astype(np.float64)
+        fid = out["face_id"]
+        for f in (0, 1):
+            sel = inner & (fid == f)
+            if f == 0 or sel.sum() == 0:
+                continue              # face 0 is never shaded
+            P0, P1, P2 = V[m.vertex_indices[f]]
+            # hit = P0 + b1 (P1 - P0) + b2 (P2 - P0), least squares on the plane
+            M = np.stack([P1 - P0, P2 - P0], 1)
+            sol = np.linalg.lstsq(M, (hit[sel] - P0).T, rcond=None)[0]
+            got = out["bary"][sel]
+            if not (np.allclose(got[:, 0], sol[0], atol=2e-5) and
+                    np.allclose(got[:, 1], sol[1], atol=2e-5) and
+                    np.allclose(got[:, 2], 1 - sol[0] - sol[1], atol=4e-5)):
+                return False, f"{pose}: barycentrics are not those of the hit point"
+        # normal output is w2c_R (float32, Eigen order) times normal_wld
+        nw = out["normal_wld"].astype(np.float32)
+        Rf = np.array(a.w2c_R[:], np.float32).reshape(3, 3)
+        rot = np.stack([Rf[r, 0] * nw[..., 0] + (Rf[r, 1] * nw[..., 1] + Rf[r, 2] * nw[..., 2])
+                        for r in range(3)], -1)
+        if not np.array_equal(rot[cov], out["normal"][cov]):
+            return False, f"{pose}: normal != w2c_R normal_wld"
+        if not (np.all(out["geo_id"][cov] == 0) and np.all(out["geo_id"][~cov] == -1)):
+            return False, f"{pose}: geo_id"
+        if np.any(out["pos_wld"][~cov] != 0) or np.any(out["bary"][~cov] != 0):
+            return False, f"{pose}: uncovered pixels must be zero"
+    return True, ""
+
+
 ALL_PINS = [pin_brute_force, pin_ties_lowest_id, pin_face_zero_not_shaded,
             pin_perspective_correct, pin_texture_sampling, pin_sphere_closed_forms,
-            pin_normals_rotated_camera, pin_lambertian, pin_oren_nayar, pin_depth_u16]
+            pin_normals_rotated_camera, pin_lambertian, pin_oren_nayar, pin_depth_u16,
+            pin_gbuffer]

@@ -111,6 +111,22 @@ def test_cpp_rasterizer_matches_oracle(gpu_available, tmp_path, variant):
         np.testing.assert_allclose(got["normal"], ref["normal"], rtol=1e-5, atol=1e-7)
         dw = np.fromfile(pfx + "depthw.u16", np.uint16).reshape(h, w)
         assert np.array_equal(dw, O.depth_to_u16(ref["depth"]))
+        if i == 0:
+            # RenderGBuffer: the Render planes it shares, and the G-buffer
+            # planes against the oracle's restatement
+            gp = os.path.join(str(out), "gbuf0_")
+            rd = lambda n, dt, ch: np.fromfile(gp + n, dt).reshape((h, w, ch) if ch else (h, w))
+            assert np.array_equal(rd("color.u8", np.uint8, 3), got["color"])
+            assert np.array_equal(rd("shaded.u8", np.uint8, 3), got["color"])
+            assert np.array_equal(rd("stencil.u8", np.uint8, 0), got["mask"])
+            assert np.array_equal(rd("face_id.i32", np.int32, 0), got["face_id"])
+            assert np.array_equal(rd("normal_cam.f32", np.float32, 3), got["normal"])
+            rcg, refg, _ = O.render_gbuffer(mesh, oc, opt, c2w_t=c.c2w[:3, 3])
+            assert rcg == 0
+            assert np.array_equal(rd("geo_id.i32", np.int32, 0), refg["geo_id"])
+            for k in ("bary", "uv", "pos_wld", "pos_cam", "normal_wld"):
+                np.testing.assert_allclose(rd(k + ".f32", np.float32, 3), refg[k], rtol=1e-5,
+                                           atol=1e-6, err_msg=f"RenderGBuffer {k}")
         bpfx = os.path.join(str(out), f"batch{i}_")
         assert np.array_equal(np.fromfile(bpfx + "face_id.i32", np.int32).reshape(h, w),
                               got["face_id"])

@@ -30,6 +30,8 @@ MUTATIONS = {
     10: "projection adds cx to y",
     12: "normal output rotated by R^T",
     13: "back-face culling ignored at resolve",
+    14: "G-buffer barycentrics u and v swapped",
+    15: "G-buffer camera position rotated by R^T",
 }
 
 

@@ -77,6 +77,14 @@ class UguOutputs(ctypes.Structure):
     ]
 
 
+class UguGBuffer(ctypes.Structure):
+    _fields_ = [
+        ("bary", ctypes.c_void_p), ("uv", ctypes.c_void_p), ("pos_wld", ctypes.c_void_p),
+        ("pos_cam", ctypes.c_void_p), ("normal_wld", ctypes.c_void_p),
+        ("geo_id", ctypes.c_void_p),
+    ]
+
+
 class UguTiling(ctypes.Structure):
     _fields_ = [("tile_w", _i), ("tile_h", _i), ("bin_capacity", _i), ("persistent", _i)]
 
@@ -113,6 +121,10 @@ SIGNATURES = {
     "ugu_hip_render_batch_eager": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,
                                                   ctypes.POINTER(UguOptions),
                                                   ctypes.POINTER(UguOutputs), ctypes.c_int]),
+    "ugu_hip_render_gbuffer": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,
+                                              ctypes.POINTER(UguOptions),
+                                              ctypes.POINTER(UguOutputs),
+                                              ctypes.POINTER(UguGBuffer), ctypes.c_int]),
     "ugu_hip_set_tiling": (ctypes.c_int, [_P, ctypes.POINTER(UguTiling)]),
     "ugu_hip_get_tiling": (ctypes.c_int, [_P, ctypes.POINTER(UguTiling)]),
     "ugu_hip_autotune": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,

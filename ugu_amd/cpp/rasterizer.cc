@@ -12,6 +12,7 @@
 #include "ugu/renderer/cpu/rasterizer.h"
 
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <utility>
 
@@ -281,6 +282,40 @@ class Rasterizer::Impl {
     return RenderViews(cams, pc.data(), pd.data(), pn.data(), pm.data(), pf.data());
   }
 
+  bool RenderGBuffer(GBuffer* gbuf) const {
+    if (gbuf == nullptr) {
+      LOGE("gbuf is nullptr");
+      return false;
+    }
+    if (!Validate(mesh_initialized_, camera_, mesh_, option_, true)) return false;
+    UguCamera cam;
+    if (!to_abi(*camera_, &cam)) {
+      LOGE("only PinholeCamera is supported by the GPU rasterizer\n");
+      return false;
+    }
+    gbuf->Init(cam.width, cam.height);
+    UguOutputs out;
+    out.color = gbuf->color.data;
+    out.depth = nullptr;
+    out.normal = reinterpret_cast<float*>(gbuf->normal_cam.data);
+    out.mask = gbuf->stencil.data;
+    out.face_id = reinterpret_cast<int32_t*>(gbuf->face_id.data);
+    UguGBuffer g;
+    g.bary = reinterpret_cast<float*>(gbuf->bary.data);
+    g.uv = reinterpret_cast<float*>(gbuf->uv.data);
+    g.pos_wld = reinterpret_cast<float*>(gbuf->pos_wld.data);
+    g.pos_cam = reinterpret_cast<float*>(gbuf->pos_cam.data);
+    g.normal_wld = reinterpret_cast<float*>(gbuf->normal_wld.data);
+    g.geo_id = reinterpret_cast<int32_t*>(gbuf->geo_id.data);
+    const UguOptions opt = to_abi(option_);
+    if (ugu_hip_render_gbuffer(ctx_, &cam, 1, &opt, &out, &g, UGU_HIP_MEMORY_HOST) != UGU_HIP_OK) {
+      LOGE("%s", ugu_hip_last_error(ctx_));
+      return false;
+    }
+    std::memcpy(gbuf->shaded.data, gbuf->color.data, gbuf->color.total() * 3);
+    return true;
+  }
+
   bool RenderW(Image3b* color, Image1w* depth, Image3f* normal, Image1b* mask,
                Image1i* face_id) const {
     if (depth == nullptr) {
@@ -335,6 +370,10 @@ bool Rasterizer::RenderBatch(const std::vector<std::shared_ptr<const Camera>>& c
                              std::vector<Image1i>* face_id) const {
   return pimpl_->RenderBatch(cameras, color, depth, normal, mask, face_id);
 }
+#endif
+
+#ifdef UGU_RASTERIZER_HAS_RENDER_GBUFFER
+bool Rasterizer::RenderGBuffer(GBuffer* gbuf) const { return pimpl_->RenderGBuffer(gbuf); }
 #endif
 
 }  // namespace ugu

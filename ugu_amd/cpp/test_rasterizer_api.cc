@@ -181,6 +181,24 @@ int main(int argc, char** argv) {
     }
     cf_out.write(reinterpret_cast<const char*>(cam_f), sizeof(cam_f));
   }
+  {
+    // G-buffer extension for the first camera
+    renderer.set_camera(cameras[0]);
+    ugu::GBuffer gbuf;
+    check(renderer.RenderGBuffer(&gbuf), "render_gbuffer");
+    const std::string pfx = out + "/gbuf0_";
+    write_img(pfx + "color.u8", gbuf.color);
+    write_img(pfx + "shaded.u8", gbuf.shaded);
+    write_img(pfx + "normal_cam.f32", gbuf.normal_cam);
+    write_img(pfx + "normal_wld.f32", gbuf.normal_wld);
+    write_img(pfx + "pos_cam.f32", gbuf.pos_cam);
+    write_img(pfx + "pos_wld.f32", gbuf.pos_wld);
+    write_img(pfx + "stencil.u8", gbuf.stencil);
+    write_img(pfx + "face_id.i32", gbuf.face_id);
+    write_img(pfx + "uv.f32", gbuf.uv);
+    write_img(pfx + "bary.f32", gbuf.bary);
+    write_img(pfx + "geo_id.i32", gbuf.geo_id);
+  }
   std::vector<std::shared_ptr<const ugu::Camera>> all(cameras.begin(), cameras.end());
   std::vector<ugu::Image1i> bfid;
   std::vector<ugu::Image3b> bcol;

@@ -725,6 +725,61 @@ UGU_DEV void diff_eval(const DevView& v, const DevMesh& m, int fid, int px, int 
   }
 }
 
+// G-buffer planes of one pixel (see UguGBuffer in ugu_hip.h). u, v: the
+// winner's perspective-correct weights of its second and third vertex; P: its
+// world vertex positions (null when no position plane is written); uvm: the
+// mesh when the uv plane is written and the mesh has uvs; n: the shading normal.
+UGU_DEV void write_gbuffer(const DevView& v, int64_t pidx, bool covered, float u, float vv,
+                           const float* P, const DevMesh* uvm, float n0, float n1, float n2,
+                           int fid = 0) {
+  float b[3] = {0.0f, 0.0f, 0.0f}, t[3] = {0.0f, 0.0f, 0.0f};
+  float pw[3] = {0.0f, 0.0f, 0.0f}, pc[3] = {0.0f, 0.0f, 0.0f};
+  if (covered) {
+    const float w = 1.0f - u - vv;
+    b[0] = u;
+    b[1] = vv;
+    b[2] = 1.0f - u - vv;
+    if (uvm) {
+      const int32_t* ui = uvm->uvidx + int64_t(fid) * 3;
+      const float* a = uvm->uv + int64_t(ui[0]) * 2;
+      const float* bb = uvm->uv + int64_t(ui[1]) * 2;
+      const float* c = uvm->uv + int64_t(ui[2]) * 2;
+      t[0] = w * a[0] + u * bb[0] + vv * c[0];
+      t[1] = w * a[1] + u * bb[1] + vv * c[1];
+      t[2] = 1.0f - t[0] - t[1];
+    }
+    if (P) {
+      for (int k = 0; k < 3; k++) pw[k] = w * P[k] + u * P[3 + k] + vv * P[6 + k];
+      for (int r = 0; r < 3; r++)
+        pc[r] = v.w2c_t[r] + sum3(v.w2c_R[r * 3 + 0] * pw[0], v.w2c_R[r * 3 + 1] * pw[1],
+                                  v.w2c_R[r * 3 + 2] * pw[2]);
+    }
+  } else {
+    n0 = n1 = n2 = 0.0f;
+  }
+  if (v.bary) {
+    float* o = v.bary + pidx * 3;
+    o[0] = b[0]; o[1] = b[1]; o[2] = b[2];
+  }
+  if (v.uvo) {
+    float* o = v.uvo + pidx * 3;
+    o[0] = t[0]; o[1] = t[1]; o[2] = t[2];
+  }
+  if (v.pos_wld) {
+    float* o = v.pos_wld + pidx * 3;
+    o[0] = pw[0]; o[1] = pw[1]; o[2] = pw[2];
+  }
+  if (v.pos_cam) {
+    float* o = v.pos_cam + pidx * 3;
+    o[0] = pc[0]; o[1] = pc[1]; o[2] = pc[2];
+  }
+  if (v.nrm_wld) {
+    float* o = v.nrm_wld + pidx * 3;
+    o[0] = n0; o[1] = n1; o[2] = n2;
+  }
+  if (v.geo_id) v.geo_id[pidx] = covered ? 0 : -1;
+}
+
 template <int TW, int TH, int MODE>
 UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
                          const DevMesh& mesh, const float4* proj,
@@ -760,6 +815,7 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
           np[0] = np[1] = np[2] = 0.0f;
         }
         if (v.mask) v.mask[pidx] = 0;
+        write_gbuffer(v, pidx, false, 0.0f, 0.0f, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
       } else {
         float* ip = v.image + pidx * 3;
         ip[0] = ip[1] = ip[2] = 0.0f;
@@ -1039,6 +1095,15 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
         nrm[2] = sum3(v.w2c_R[6] * n0, v.w2c_R[7] * n1, v.w2c_R[8] * n2);
       }
       if (v.color) shade_u8(mesh, opt, fid, w1, w2, r0, r1, r2, n0, n1, n2, col);
+      if (v.bary || v.uvo || v.pos_wld || v.pos_cam || v.nrm_wld || v.geo_id) {
+        float P[9], N[3];
+        const bool pos = v.pos_wld || v.pos_cam;
+        if (pos) load_face_rec(mesh.soup, fid, P, N);
+        write_gbuffer(v, pidx, true, w1, w2, pos ? P : nullptr,
+                      v.uvo && mesh.uv && mesh.uvidx ? &mesh : nullptr, n0, n1, n2, fid);
+      }
+    } else {
+      write_gbuffer(v, pidx, false, 0.0f, 0.0f, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
     }
     if (v.color) {
       uint8_t* cp = v.color + pidx * 3;

@@ -25,6 +25,13 @@ struct DevView {
   int32_t* face_id;
   float* image;        // differentiable forward: H x W x 3 f32
   const float* grad;   // differentiable backward: H x W x 3 f32
+  // G-buffer planes (null: not written)
+  float* bary;
+  float* uvo;
+  float* pos_wld;
+  float* pos_cam;
+  float* nrm_wld;
+  int32_t* geo_id;
 };
 
 struct TexInfo {

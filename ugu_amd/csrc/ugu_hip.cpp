@@ -354,16 +354,49 @@ struct OutReq {
 
 // Runs the pipeline for n views. mode 0: reference outputs; mode 1:
 // differentiable forward (outputs[i] unused, images/face_ids used).
+// Output planes of a render call: the five Render outputs, then the G-buffer
+// planes. kPlaneBytes = bytes per pixel.
+constexpr int kPlanes = 11;
+constexpr int kPlaneBytes[kPlanes] = {3, 4, 12, 1, 4, 12, 12, 12, 12, 12, 4};
+
+void plane_ptrs(const UguOutputs& o, const UguGBuffer* g, void* p[kPlanes]) {
+  p[0] = o.color;
+  p[1] = o.depth;
+  p[2] = o.normal;
+  p[3] = o.mask;
+  p[4] = o.face_id;
+  p[5] = g ? g->bary : nullptr;
+  p[6] = g ? g->uv : nullptr;
+  p[7] = g ? g->pos_wld : nullptr;
+  p[8] = g ? g->pos_cam : nullptr;
+  p[9] = g ? g->normal_wld : nullptr;
+  p[10] = g ? g->geo_id : nullptr;
+}
+
+void set_view_planes(DevView& v, void* const p[kPlanes]) {
+  v.color = static_cast<uint8_t*>(p[0]);
+  v.depth = static_cast<float*>(p[1]);
+  v.normal = static_cast<float*>(p[2]);
+  v.mask = static_cast<uint8_t*>(p[3]);
+  v.face_id = static_cast<int32_t*>(p[4]);
+  v.bary = static_cast<float*>(p[5]);
+  v.uvo = static_cast<float*>(p[6]);
+  v.pos_wld = static_cast<float*>(p[7]);
+  v.pos_cam = static_cast<float*>(p[8]);
+  v.nrm_wld = static_cast<float*>(p[9]);
+  v.geo_id = static_cast<int32_t*>(p[10]);
+}
+
 int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm,
                  const DevOptions& dopt, int mode, const UguOutputs* outs,
                  int memory, float* const* images, int32_t* const* fids,
-                 bool use_graph) {
+                 bool use_graph, const UguGBuffer* gbufs = nullptr) {
   const int tw = c->tiling.tile_w, th = c->tiling.tile_h;
   // layout
   c->h_tile_base.resize(n + 1);
   int64_t total = 0;
   int64_t stage_bytes = 0;
-  std::vector<int64_t> stage_off(size_t(n) * 5, -1);
+  std::vector<int64_t> stage_off(size_t(n) * kPlanes, -1);
   for (int i = 0; i < n; i++) {
     int err = check_camera(c, cams[i]);
     if (err) return err;
@@ -372,13 +405,12 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
     total += int64_t(tx) * ty;
     if (mode == 0 && memory == UGU_HIP_MEMORY_HOST) {
       const int64_t px = int64_t(cams[i].width) * cams[i].height;
-      const int64_t sz[5] = {3 * px, 4 * px, 12 * px, px, 4 * px};
-      const void* req[5] = {outs[i].color, outs[i].depth, outs[i].normal, outs[i].mask,
-                            outs[i].face_id};
-      for (int k = 0; k < 5; k++) {
+      void* req[kPlanes];
+      plane_ptrs(outs[i], gbufs ? &gbufs[i] : nullptr, req);
+      for (int k = 0; k < kPlanes; k++) {
         if (req[k]) {
-          stage_off[size_t(i) * 5 + k] = stage_bytes;
-          stage_bytes += (sz[k] + 255) / 256 * 256;
+          stage_off[size_t(i) * kPlanes + k] = stage_bytes;
+          stage_bytes += (kPlaneBytes[k] * px + 255) / 256 * 256;
         }
       }
     }
@@ -423,21 +455,14 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
     v.tile_base = c->h_tile_base[i];
     v.proj_offset = int64_t(i) * dm.V;
     if (mode == 0) {
-      if (memory == UGU_HIP_MEMORY_DEVICE) {
-        v.color = outs[i].color;
-        v.depth = outs[i].depth;
-        v.normal = outs[i].normal;
-        v.mask = outs[i].mask;
-        v.face_id = outs[i].face_id;
-      } else {
+      void* p[kPlanes];
+      plane_ptrs(outs[i], gbufs ? &gbufs[i] : nullptr, p);
+      if (memory == UGU_HIP_MEMORY_HOST) {
         uint8_t* sb = c->stage_out.as<uint8_t>();
-        const int64_t* so = &stage_off[size_t(i) * 5];
-        v.color = so[0] >= 0 ? sb + so[0] : nullptr;
-        v.depth = so[1] >= 0 ? reinterpret_cast<float*>(sb + so[1]) : nullptr;
-        v.normal = so[2] >= 0 ? reinterpret_cast<float*>(sb + so[2]) : nullptr;
-        v.mask = so[3] >= 0 ? sb + so[3] : nullptr;
-        v.face_id = so[4] >= 0 ? reinterpret_cast<int32_t*>(sb + so[4]) : nullptr;
+        const int64_t* so = &stage_off[size_t(i) * kPlanes];
+        for (int k = 0; k < kPlanes; k++) p[k] = so[k] >= 0 ? sb + so[k] : nullptr;
       }
+      set_view_planes(v, p);
     } else {
       v.image = images[i];
       v.face_id = fids ? fids[i] : nullptr;
@@ -558,12 +583,13 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
     const uint8_t* sb = c->stage_out.as<uint8_t>();
     for (int i = 0; i < n; i++) {
       const int64_t px = int64_t(cams[i].width) * cams[i].height;
-      const int64_t sz[5] = {3 * px, 4 * px, 12 * px, px, 4 * px};
-      void* dst[5] = {outs[i].color, outs[i].depth, outs[i].normal, outs[i].mask, outs[i].face_id};
-      for (int k = 0; k < 5; k++) {
-        const int64_t off = stage_off[size_t(i) * 5 + k];
+      void* dst[kPlanes];
+      plane_ptrs(outs[i], gbufs ? &gbufs[i] : nullptr, dst);
+      for (int k = 0; k < kPlanes; k++) {
+        const int64_t off = stage_off[size_t(i) * kPlanes + k];
         if (off >= 0)
-          UGU_HIP_CHECK(c, hipMemcpyAsync(dst[k], sb + off, size_t(sz[k]), hipMemcpyDeviceToHost, c->stream));
+          UGU_HIP_CHECK(c, hipMemcpyAsync(dst[k], sb + off, size_t(kPlaneBytes[k] * px),
+                                          hipMemcpyDeviceToHost, c->stream));
       }
     }
     UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -572,7 +598,8 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
 }
 
 int render_common(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions* o,
-                  const UguOutputs* outs, int memory, bool use_graph) {
+                  const UguOutputs* outs, int memory, bool use_graph,
+                  const UguGBuffer* gbufs = nullptr) {
   if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
   if (n < 1 || cams == nullptr || outs == nullptr)
     return fail(c, n < 1 ? UGU_HIP_ERR_INVALID_ARGUMENT : UGU_HIP_ERR_NO_CAMERA,
@@ -580,14 +607,18 @@ int render_common(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions
   if (memory != UGU_HIP_MEMORY_HOST && memory != UGU_HIP_MEMORY_DEVICE)
     return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "unknown memory kind");
   for (int i = 0; i < n; i++) {
-    const UguOutputs& q = outs[i];
-    int err = validate(c, &cams[i], o, q.color || q.depth || q.normal || q.mask || q.face_id);
+    void* p[kPlanes];
+    plane_ptrs(outs[i], gbufs ? &gbufs[i] : nullptr, p);
+    bool any = false;
+    for (int k = 0; k < kPlanes; k++) any = any || p[k] != nullptr;
+    int err = validate(c, &cams[i], o, any);
     if (err) return err;
   }
   UGU_HIP_CHECK(c, hipSetDevice(c->device));
   if (int oe = order_after_user(c)) return oe;
   DevMesh dm = dev_mesh(c);
-  return run_pipeline(c, cams, n, dm, dev_options(*o), 0, outs, memory, nullptr, nullptr, use_graph);
+  return run_pipeline(c, cams, n, dm, dev_options(*o), 0, outs, memory, nullptr, nullptr, use_graph,
+                      gbufs);
 }
 
 }  // namespace
@@ -793,6 +824,12 @@ int ugu_hip_render(ugu_hip_ctx* c, const UguCamera* cam, const UguOptions* o,
 int ugu_hip_render_batch(ugu_hip_ctx* c, const UguCamera* cams, int n,
                          const UguOptions* o, const UguOutputs* outs, int memory) {
   return render_common(c, cams, n, o, outs, memory, true);
+}
+
+int ugu_hip_render_gbuffer(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions* o,
+                           const UguOutputs* outs, const UguGBuffer* gbufs, int memory) {
+  if (c && !gbufs) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "gbuffers are required");
+  return render_common(c, cams, n, o, outs, memory, true, gbufs);
 }
 
 int ugu_hip_render_batch_eager(ugu_hip_ctx* c, const UguCamera* cams, int n,

@@ -280,6 +280,38 @@ class Rasterizer:
                        _abi.UGU_HIP_MEMORY_HOST))
         return res
 
+    GBUFFER_PLANES = ("bary", "uv", "pos_wld", "pos_cam", "normal_wld", "geo_id")
+
+    def render_gbuffer(self, cameras, outputs=("color", "depth", "normal", "mask", "face_id"),
+                       gbuffer=GBUFFER_PLANES):
+        """render_batch plus G-buffer planes (ugu_hip_render_gbuffer): the
+        fields of ugu::GBuffer the rasterizer's resolve fills (bary, uv,
+        pos_wld, pos_cam, normal_wld as H x W x 3 float32, geo_id H x W int32).
+        Returns a list of dicts holding both kinds of planes."""
+        res = []
+        outs = (_abi.UguOutputs * len(cameras))()
+        gbs = (_abi.UguGBuffer * len(cameras))()
+        shapes = {"color": (3, np.uint8), "depth": (0, np.float32), "normal": (3, np.float32),
+                  "mask": (0, np.uint8), "face_id": (0, np.int32), "bary": (3, np.float32),
+                  "uv": (3, np.float32), "pos_wld": (3, np.float32), "pos_cam": (3, np.float32),
+                  "normal_wld": (3, np.float32), "geo_id": (0, np.int32)}
+        for i, c in enumerate(cameras):
+            d = {}
+            for k in tuple(outputs) + tuple(gbuffer):
+                ch, dt = shapes[k]
+                d[k] = np.empty((c.height, c.width, ch) if ch else (c.height, c.width), dt)
+            outs[i] = _abi.UguOutputs(*[_ptr(d.get(k)) for k in
+                                        ("color", "depth", "normal", "mask", "face_id")])
+            gbs[i] = _abi.UguGBuffer(*[_ptr(d.get(k)) for k in self.GBUFFER_PLANES])
+            res.append(d)
+        cams = (_abi.UguCamera * len(cameras))(
+            *[c if isinstance(c, _abi.UguCamera) else c.to_abi() for c in cameras])
+        opt = options_to_abi(self.option_)
+        self._check(self._lib.ugu_hip_render_gbuffer(self._ctx, cams, len(cameras),
+                                                     ctypes.byref(opt), outs, gbs,
+                                                     _abi.UGU_HIP_MEMORY_HOST))
+        return res
+
     def render_device(self, cameras, buffers, graph=True, batched=True):
         """Render into device memory. `buffers` is a list (one per camera) of
         dicts of torch CUDA tensors (any of color/depth/normal/mask/face_id).
