@@ -633,8 +633,8 @@ UGU_DEV void ftex_coords(int tw, int th, float iu, float iv, int& x0, int& y0,
 // not the reference's float32 rounding of thin triangles).
 
 struct DiffEval {
-  double X[3], Y[3], Z[3], Qx[3], Qy[3];
-  double e[3], A, q[3], s, b0, b1, b2;
+  double X[3], Y[3], iZ[3], Qx[3], Qy[3];  // iZ = 1 / camera depth
+  double e[3], A, iA, q[3], s, is, b0, b1, b2;
   double c[3], t[3], dcdu[3], dcdv[3], dtdu[3], dtdv[3];
   double lu, lv;
   int tx0, ty0, tx1, ty1;
@@ -657,9 +657,9 @@ UGU_DEV void diff_eval(const DevView& v, const DevMesh& m, int fid, int px, int 
              double(v.w2c_R[r * 3 + 2]) * P[2] + double(v.w2c_t[r]);
     e.Qx[i] = Q[0];
     e.Qy[i] = Q[1];
-    e.Z[i] = Q[2];
-    e.X[i] = double(v.fx) * Q[0] / Q[2] + double(v.cx);
-    e.Y[i] = double(v.fy) * Q[1] / Q[2] + double(v.cy);
+    e.iZ[i] = 1.0 / Q[2];
+    e.X[i] = double(v.fx) * Q[0] * e.iZ[i] + double(v.cx);
+    e.Y[i] = double(v.fy) * Q[1] * e.iZ[i] + double(v.cy);
   }
   auto E = [](double ax, double ay, double bx, double by, double cx, double cy) {
     return (cx - ax) * (by - ay) - (cy - ay) * (bx - ax);
@@ -668,10 +668,14 @@ UGU_DEV void diff_eval(const DevView& v, const DevMesh& m, int fid, int px, int 
   e.e[0] = E(e.X[1], e.Y[1], e.X[2], e.Y[2], x, y);
   e.e[1] = E(e.X[2], e.Y[2], e.X[0], e.Y[0], x, y);
   e.e[2] = E(e.X[0], e.Y[0], e.X[1], e.Y[1], x, y);
-  for (int i = 0; i < 3; i++) e.q[i] = e.e[i] / e.A / e.Z[i];
+  // reciprocals once, then products: the model is evaluated in float64 and
+  // compared with a float64 reference, so a few ulps of rounding are free
+  e.iA = 1.0 / e.A;
+  for (int i = 0; i < 3; i++) e.q[i] = e.e[i] * e.iA * e.iZ[i];
   e.s = e.q[0] + e.q[1] + e.q[2];
-  e.b1 = e.q[1] / e.s;
-  e.b2 = e.q[2] / e.s;
+  e.is = 1.0 / e.s;
+  e.b1 = e.q[1] * e.is;
+  e.b2 = e.q[2] * e.is;
   e.b0 = 1.0 - e.b1 - e.b2;
   for (int k = 0; k < 3; k++) {
     e.c[k] = 1.0;
@@ -1353,13 +1357,13 @@ __global__ __launch_bounds__(256) void k_bwd_items(
     if (g.verts) {
       // b_i = q_i / s  =>  dL/dq_j = (dL/db_j - (gu b1 + gv b2)) / s
       const double S = gu * e.b1 + gv * e.b2;
-      const double gq[3] = {-S / e.s, (gu - S) / e.s, (gv - S) / e.s};
+      const double gq[3] = {-S * e.is, (gu - S) * e.is, (gv - S) * e.is};
       // q_j = e_j / (A Z_j)
       double ge[3], gZ[3], gA = 0.0;
       for (int j = 0; j < 3; j++) {
-        ge[j] = gq[j] / (e.A * e.Z[j]);
-        gZ[j] = -gq[j] * e.q[j] / e.Z[j];
-        gA -= gq[j] * e.q[j] / e.A;
+        ge[j] = gq[j] * e.iA * e.iZ[j];
+        gZ[j] = -gq[j] * e.q[j] * e.iZ[j];
+        gA -= gq[j] * e.q[j] * e.iA;
       }
       // E(a,b,c): dE/da = (c.y-b.y, b.x-c.x), dE/db = (a.y-c.y, c.x-a.x),
       //           dE/dc = (b.y-a.y, a.x-b.x)
@@ -1382,7 +1386,7 @@ __global__ __launch_bounds__(256) void k_bwd_items(
       gY[2] += gA * (e.X[0] - e.X[1]);
       // X = fx Qx / Qz + cx, Y = fy Qy / Qz + cy, Z = Qz; Q = R P + t
       for (int j = 0; j < 3; j++) {
-        const double iz = 1.0 / e.Z[j];
+        const double iz = e.iZ[j];
         const double gQx = gX[j] * v.fx * iz;
         const double gQy = gY[j] * v.fy * iz;
         const double gQz = gZ[j] - gX[j] * v.fx * e.Qx[j] * iz * iz -
