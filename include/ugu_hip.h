@@ -24,6 +24,9 @@
  *       (include/ugu/renderer/base.h:34-77, filled by the OpenGL renderer in
  *       src/renderer/gl/renderer.cc:501-647) that the rasterizer's resolve
  *       can produce, next to the Render outputs
+ *   ugu_hip_prepare_raytracer / ugu_hip_raytrace
+ *       Raytracer::PrepareMesh / Raytracer::Render
+ *       (src/renderer/cpu/raytracer.cc:88-108, :117-225)
  *   ugu_hip_depth_to_u16
  *       the ConvertTo step of Rasterizer::RenderW (src/renderer/cpu/rasterizer.cc:329-344,
  *       include/ugu/image_proc.h:453-520)
@@ -52,7 +55,7 @@
 extern "C" {
 #endif
 
-#define UGU_HIP_ABI_VERSION 2
+#define UGU_HIP_ABI_VERSION 3
 
 typedef struct ugu_hip_ctx ugu_hip_ctx;
 
@@ -109,6 +112,9 @@ typedef struct {
  *   w2c_R = camera->w2c().rotation().cast<float>()   (rasterizer.cc:118)
  *   w2c_t = camera->w2c().translation().cast<float>() (rasterizer.cc:119)
  *   c2w_R = c2w.matrix().block<3,3>(0,0).cast<float>() (camera.h:322)
+ *   c2w_t = c2w.translation().cast<float>(), the camera position: the ray
+ *           origin of the raytracer (camera.h:322, :431-436); unused by the
+ *           rasterizer
  * Matrices are row-major. (cx, cy) is the principal point, (fx, fy) the
  * focal length, both in pixels (camera.h:380-387). */
 typedef struct {
@@ -118,12 +124,14 @@ typedef struct {
   float w2c_R[9];
   float w2c_t[3];
   float c2w_R[9];
+  float c2w_t[3];
 } UguCamera;
 
 /* RendererCpuOption (include/ugu/renderer/cpu/renderer.h:17-38) with the
  * enum values of include/ugu/renderer/base.h:11-32 and common.h:50-53.
- * As in the reference, depth_scale and oren_nayar_sigma have no effect on
- * Render (RendererCpuOption::CopyTo never copies the sigma). */
+ * As in the reference, oren_nayar_sigma has no effect (RendererCpuOption::
+ * CopyTo never copies it) and depth_scale none on the rasterizer's Render;
+ * the raytracer multiplies its depth by depth_scale (raytracer.cc:185). */
 typedef struct {
   int32_t diffuse_color;   /* 0 kNone, 1 kTexture, 2 kVertex */
   int32_t interp;          /* 0 kNn, 1 kBilinear */
@@ -233,6 +241,21 @@ typedef struct {
 int ugu_hip_render_gbuffer(ugu_hip_ctx* ctx, const UguCamera* cameras, int n,
                            const UguOptions* options, const UguOutputs* outputs,
                            const UguGBuffer* gbuffers, int memory);
+
+/* ---- raytracer (the reference's second RendererCpu) --------------------
+ * ugu_hip_prepare_raytracer builds a bounding volume hierarchy over the
+ * prepared mesh (call after ugu_hip_prepare_mesh; the mesh must not change).
+ * ugu_hip_raytrace renders like Raytracer::Render: per pixel a ray from c2w_t
+ * along ray_w(x, y) finds its closest face with the reference's float64
+ * Moller-Trumbore test (src/util/geom_util.cc:60-115, epsilon 1e-8, standard
+ * barycentric conversion), closest = smallest float t, equal t -> lowest face
+ * id; then culling (face normal . ray > 0 -> background), face_id, mask,
+ * depth = z of the camera-space hit * depth_scale, normal and colour with
+ * the raytracer's (u, v) conventions (raytracer.cc:160-218). outputs[i]
+ * belongs to cameras[i]. */
+int ugu_hip_prepare_raytracer(ugu_hip_ctx* ctx);
+int ugu_hip_raytrace(ugu_hip_ctx* ctx, const UguCamera* cameras, int n,
+                     const UguOptions* options, const UguOutputs* outputs, int memory);
 
 /* Tiling control and the autotuner. ugu_hip_autotune renders the given views
  * with every tile size, sizes the bin capacity from the measured maximum bin

@@ -73,6 +73,10 @@ def lib():
                                             _P, ctypes.POINTER(OracleOptions), _P, _P, _P, _P, _P,
                                             ctypes.POINTER(OracleGBuf),
                                             ctypes.POINTER(ctypes.c_double)]
+        L.oracle_raytrace.restype = ctypes.c_int
+        L.oracle_raytrace.argtypes = [ctypes.POINTER(OracleMesh), ctypes.POINTER(OracleCamera), _P,
+                                      ctypes.POINTER(OracleOptions), _P, _P, _P, _P, _P,
+                                      ctypes.POINTER(ctypes.c_double)]
         L.oracle_depth_to_u16.restype = None
         L.oracle_depth_to_u16.argtypes = [_P, ctypes.c_int64, _P]
         L.oracle_edge_function.restype = ctypes.c_float
@@ -196,6 +200,24 @@ def render(mesh, cam, options, c2w_t=None, outputs=("color", "depth", "normal", 
     rc = lib().oracle_render(ctypes.byref(hold.m), ctypes.byref(cam), _p(ct), ctypes.byref(opt),
                              _p(out.get("color")), _p(out.get("depth")), _p(out.get("normal")),
                              _p(out.get("mask")), _p(out.get("face_id")), ctypes.byref(ms))
+    return rc, out, ms.value
+
+
+def raytrace(mesh, cam, options, c2w_t=None, outputs=("color", "depth", "normal", "mask", "face_id")):
+    """oracle_raytrace: Raytracer::Render with the exact closest hit (every
+    face tested) -> (rc, dict of numpy outputs, elapsed_ms)."""
+    h, w = cam.height, cam.width
+    shapes = {"color": ((h, w, 3), np.uint8), "depth": ((h, w), np.float32),
+              "normal": ((h, w, 3), np.float32), "mask": ((h, w), np.uint8),
+              "face_id": ((h, w), np.int32)}
+    out = {k: np.zeros(*shapes[k]) for k in outputs}
+    hold = _MeshHold(mesh)
+    opt = options_struct(options)
+    ct = None if c2w_t is None else np.ascontiguousarray(c2w_t, np.float32)
+    ms = ctypes.c_double(0.0)
+    rc = lib().oracle_raytrace(ctypes.byref(hold.m), ctypes.byref(cam), _p(ct), ctypes.byref(opt),
+                               _p(out.get("color")), _p(out.get("depth")), _p(out.get("normal")),
+                               _p(out.get("mask")), _p(out.get("face_id")), ctypes.byref(ms))
     return rc, out, ms.value
 
 

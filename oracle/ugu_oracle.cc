@@ -630,6 +630,147 @@ int oracle_render_gbuffer(const OracleMesh* mesh, const OracleCamera* cam,
   return kOk;
 }
 
+// ---------------------------------------------------------------------------
+// Raytracer::Impl::Render (raytracer.cc:117-225) with an exact closest hit:
+// every face is tested (no BVH) with the reference's float64 Moller-Trumbore
+// routine IntersectImpl (geom_util.cc:60-115) at the naive BVH's epsilon
+// (bvh_naive.h:121, 1e-8f) and its standard-barycentric conversion; the
+// closest hit is the smallest float t, equal t -> lowest face id.
+
+// Eigen's dot of two float64 3-vectors: one Packet2d for coefficients 0..1
+// and a scalar tail (redux_impl, LinearVectorizedTraversal): (x0 + x1) + x2.
+static inline double sum3d(double a, double b, double c) { return (a + b) + c; }
+
+// Eigen's cross for 3-vectors: (a1 b2 - a2 b1, a2 b0 - a0 b2, a0 b1 - a1 b0)
+static inline void cross_d(const double a[3], const double b[3], double r[3]) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline double dot_d(const double a[3], const double b[3]) {
+  return sum3d(a[0] * b[0], a[1] * b[1], a[2] * b[2]);
+}
+
+// returns true and (t, u, v) as floats on a hit
+static bool moller_trumbore(const double o[3], const double d[3], const double v0[3],
+                     const double v1[3], const double v2[3], double eps, float* t,
+                     float* u, float* v) {
+  double e1[3], e2[3], alpha[3], r[3], beta[3];
+  for (int k = 0; k < 3; k++) {
+    e1[k] = v1[k] - v0[k];
+    e2[k] = v2[k] - v0[k];
+  }
+  cross_d(d, e2, alpha);
+  const double det = dot_d(e1, alpha);
+  if (-eps < det && det < eps) return false;
+  const double inv_det = 1.0 / det;
+  for (int k = 0; k < 3; k++) r[k] = o[k] - v0[k];
+  const double uu = dot_d(alpha, r) * inv_det;
+  if (uu < 0.0f || uu > 1.0f) return false;
+  cross_d(r, e1, beta);
+  const double vv = dot_d(d, beta) * inv_det;
+  if (vv < 0.0f || uu + vv > 1.0f) return false;
+  const double tt = dot_d(e2, beta) * inv_det;
+  if (tt < 0.0f) return false;
+  // standard barycentric: u <- 1 - u - v, v <- u
+  const double w = 1 - uu - vv;
+  *t = static_cast<float>(tt);
+  *u = static_cast<float>(MUT(16) ? uu : w);
+  *v = static_cast<float>(MUT(16) ? w : uu);
+  return true;
+}
+
+int oracle_raytrace(const OracleMesh* mesh, const OracleCamera* cam, const float* c2w_t,
+                    const OracleOptions* opt, uint8_t* color, float* depth, float* normal,
+                    uint8_t* mask, int32_t* face_id, double* elapsed_ms) {
+  if (cam == nullptr) return kErrNoCamera;
+  if (mesh == nullptr) return kErrMeshNotInitialized;
+  const OracleMesh& m = *mesh;
+  const OracleOptions& o = *opt;
+  int err = Validate(m, o, color || depth || normal || mask || face_id);
+  if (err) return err;
+  const int W = cam->width, H = cam->height;
+  const size_t npx = static_cast<size_t>(W) * H;
+  if (color) std::memset(color, 0, npx * 3);
+  if (depth) std::fill(depth, depth + npx, 0.0f);
+  if (normal) std::fill(normal, normal + npx * 3, 0.0f);
+  if (mask) std::memset(mask, 0, npx);
+  if (face_id) std::fill(face_id, face_id + npx, -1);
+  auto t0 = std::chrono::steady_clock::now();
+  V3 ct{{c2w_t ? c2w_t[0] : 0.0f, c2w_t ? c2w_t[1] : 0.0f, c2w_t ? c2w_t[2] : 0.0f}};
+  RayTables rt;
+  InitRayTable(*cam, ct, &rt);
+  ShadeCtx sc(0.3f);  // Raytracer::set_option also goes through CopyTo (raytracer.cc:72-74)
+  sc.m = &m;
+  sc.o = &o;
+  for (int i = 0; i < m.num_materials; i++) sc.tex.push_back(Tex{m.tex_width[i], m.tex_height[i], m.tex_data[i]});
+  const double eps = static_cast<double>(1e-8f);
+  const int F = m.num_faces;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int y = 0; y < H; y++) {
+    for (int x = 0; x < W; x++) {
+      const size_t pi = static_cast<size_t>(y) * W + x;
+      const V3& ray_w = rt.ray_w[pi];
+      const V3 org = MUT(20) ? V3{{0.0f, 0.0f, 0.0f}} : rt.org_w[pi];
+      const double od[3] = {org[0], org[1], org[2]};
+      const double dd[3] = {ray_w[0], ray_w[1], ray_w[2]};
+      float bt = std::numeric_limits<float>::infinity(), bu = 0.0f, bv = 0.0f;
+      int fid = -1;
+      for (int f = 0; f < F; f++) {
+        const int32_t* fi = m.faces + static_cast<int64_t>(f) * 3;
+        double p[3][3];
+        bool finite = true;
+        for (int k = 0; k < 3; k++)
+          for (int c = 0; c < 3; c++) {
+            p[k][c] = m.vertices[static_cast<int64_t>(fi[k]) * 3 + c];
+            finite = finite && std::isfinite(p[k][c]);
+          }
+        // a face with a non-finite coordinate can give a NaN t, which the
+        // reference's std::sort cannot order: such faces are never hit here
+        if (!finite) continue;
+        float t, u, v;
+        if (moller_trumbore(od, dd, p[0], p[1], p[2], eps, &t, &u, &v) &&
+            (MUT(17) ? (fid < 0 || t > bt) : t < bt)) {
+          bt = t;
+          bu = u;
+          bv = v;
+          fid = f;
+        }
+      }
+      if (fid < 0) continue;
+      if (o.backface_culling && (MUT(18) ? dot3(load3(m.face_normals, fid), ray_w) < 0
+                                         : dot3(load3(m.face_normals, fid), ray_w) > 0))
+        continue;
+      if (face_id) face_id[pi] = fid;
+      if (mask) mask[pi] = 255;
+      if (depth) {
+        V3 hit;
+        for (int k = 0; k < 3; k++) hit[k] = org[k] + ray_w[k] * bt;
+        const V3 hc = matvec(cam->w2c_R, hit);
+        depth[pi] = MUT(19) ? bt * o.depth_scale : (hc[2] + cam->w2c_t[2]) * o.depth_scale;
+      }
+      V3 sn{{0.0f, 0.0f, 0.0f}};
+      if (o.shading_normal == 0) {
+        sn = load3(m.face_normals, fid);
+      } else {
+        const int32_t* ni = m.normal_indices + static_cast<int64_t>(fid) * 3;
+        const V3 na = load3(m.normals, ni[0]), nb = load3(m.normals, ni[1]),
+                 nc = load3(m.normals, ni[2]);
+        const float w = 1.0f - bu - bv;
+        for (int k = 0; k < 3; k++) sn[k] = w * na[k] + bu * nb[k] + bv * nc[k];
+      }
+      if (normal) {
+        const V3 snc = matvec(cam->w2c_R, sn);
+        for (int k = 0; k < 3; k++) normal[pi * 3 + k] = snc[k];
+      }
+      if (color) ShadePixel(sc, color + pi * 3, bu, bv, fid, ray_w, ray_w, sn);
+    }
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  return kOk;
+}
+
 // RenderW / RenderDepthW (rasterizer.cc:329-348) depth conversion:
 // ConvertTo(f_depth, depth) with scale 1 -> static_cast<uint16_t>(1.0 * (double)d)
 void oracle_depth_to_u16(const float* depth, int64_t n, uint16_t* out) {

@@ -34,8 +34,10 @@ def render(mesh, cam, opt):
 # float64 brute force
 
 
-def brute_force(mesh, cam, culling):
-    """Independent float64 rasterizer: returns (face_id, depth, ambiguous)."""
+def brute_force(mesh, cam, culling, all_edges_ambiguous=False):
+    """Independent float64 rasterizer: returns (face_id, depth, ambiguous).
+    all_edges_ambiguous: pixels within 1e-6 of any edge count as ambiguous,
+    covered or not (for checks of a different coverage rule)."""
     a = cam.to_abi()
     R = np.array(a.w2c_R[:], np.float64).reshape(3, 3)
     t = np.array(a.w2c_t[:], np.float64)
@@ -73,7 +75,7 @@ def brute_force(mesh, cam, culling):
         inbox = (xs >= bx0) & (xs <= bx1) & (ys >= by0) & (ys <= by1)
         cov &= inbox
         near_edge = inbox & (np.minimum(np.minimum(abs(e0), abs(e1)), abs(e2)) < 1e-6 * abs(A))
-        amb |= near_edge & ~cov
+        amb |= near_edge if all_edges_ambiguous else (near_edge & ~cov)
         with np.errstate(divide="ignore", invalid="ignore"):
             z = 1.0 / (e0 / A / Z[i0] + e1 / A / Z[i1] + e2 / A / Z[i2])
         for yy, xx in zip(*np.nonzero(cov)):
@@ -452,7 +454,143 @@ def pin_gbuffer():
     return True, ""
 
 
+# ---------------------------------------------------------------------------
+# raytracer (oracle_raytrace)
+
+
+def raytrace(mesh, cam, opt):
+    rc, out, _ = O.raytrace(mesh, O.camera_from_abi(cam.to_abi()), opt, c2w_t=cam.c2w[:3, 3])
+    assert rc == 0, rc
+    return out
+
+
+def pin_raytrace_brute_force(seeds=range(8)):
+    """Ray casting sees the same nearest face as the float64 projective brute
+    force (a pixel ray meets a face iff the projected pixel centre lies in the
+    projected face), away from edges and depth ties; depth is the camera z."""
+    for seed in seeds:
+        w, h = [(8, 8), (16, 12), (21, 17), (32, 32)][seed % 4]
+        tris = TS.random_triangles(8 + 2 * seed, w, h, 300 + seed)
+        mesh = TS.mesh_from_pixels(tris, seed=seed)
+        cam = TS.exact_camera(w, h)
+        for cull in (False, True):
+            opt = RendererCpuOption(diffuse_color=DiffuseColor.kVertex, backface_culling=cull)
+            out = raytrace(mesh, cam, opt)
+            fid, depth, amb = brute_force(mesh, cam, cull, all_edges_ambiguous=True)
+            ok = (out["face_id"] == fid) | amb
+            if not ok.all():
+                return False, f"raytrace seed {seed} cull {cull}: face_id differs at " \
+                              f"{np.argwhere(~ok)[:3].tolist()}"
+            sel = (fid >= 0) & ~amb
+            if not cull and sel.sum() < 10:
+                return False, f"raytrace seed {seed}: scene not covered"
+            if not np.allclose(out["depth"][sel], depth[sel], rtol=1e-5):
+                return False, f"raytrace seed {seed}: depth differs"
+            if not np.array_equal(out["mask"] == 255, out["face_id"] >= 0):
+                return False, f"raytrace seed {seed}: mask != covered"
+    return True, ""
+
+
+def pin_raytrace_plane():
+    """Tilted quad from an identity and a rotated, translated camera: face
+    from the side of the diagonal, depth = camera z of the ray-plane hit times
+    depth_scale, vertex colour with the raytracer's barycentric convention
+    (u = weight of corner 0, v = weight of corner 1 after the standard
+    conversion, colour = (1-u-v) c0 + u c1 + v c2), face 0 shaded."""
+    m, _ = _tilted_quad()
+    V = m.vertices.astype(np.float64)
+    C = m.vertex_colors.astype(np.float64)
+    th = 0.3
+    Ry = np.array([[math.cos(th), 0, math.sin(th)], [0, 1, 0], [-math.sin(th), 0, math.cos(th)]])
+    for pose in ("identity", "rotated"):
+        c2w = np.eye(4)
+        if pose == "rotated":
+            c2w[:3, :3] = Ry
+            c2w[:3, 3] = (-0.6, 0.1, 0.2)
+        cam = PinholeCamera(64, 48, c2w, (31.5, 23.5), (40.0, 40.0))
+        out = raytrace(m, cam, RendererCpuOption(diffuse_color=DiffuseColor.kVertex,
+                                                 depth_scale=2.5))
+        a = cam.to_abi()
+        R = np.array(a.w2c_R[:], np.float64).reshape(3, 3)
+        t = np.array(a.w2c_t[:], np.float64)
+        o = -R.T @ t
+        ys, xs = np.mgrid[0:48, 0:64].astype(np.float64)
+        d = np.stack([(xs - 31.5) / 40.0, (ys - 23.5) / 40.0, np.ones_like(xs)], -1) @ R
+        n = np.array([-0.8, 0.0, 1.0])
+        hit = o + d * ((2.0 - o @ n) / (d @ n))[..., None]
+        X, Y = hit[..., 0], hit[..., 1]
+        inner = (np.abs(X) < 0.97) & (np.abs(Y) < 0.77) & (np.abs(Y - 0.8 * X) > 2e-3)
+        if inner.sum() < 300:
+            return False, f"{pose}: quad not covered"
+        exp_f = np.where(Y < 0.8 * X, 0, 1)
+        if not np.array_equal(out["face_id"][inner], exp_f[inner]):
+            return False, f"{pose}: face id is not the side of the diagonal"
+        if not (out["mask"][inner] == 255).all():
+            return False, f"{pose}: mask"
+        zc = (hit @ R.T + t)[..., 2] * 2.5
+        if not np.allclose(out["depth"][inner], zc[inner], rtol=2e-5):
+            return False, f"{pose}: depth is not depth_scale * camera z of the hit"
+        for f in (0, 1):
+            sel = inner & (out["face_id"] == f)
+            i0, i1, i2 = m.vertex_indices[f]
+            M = np.stack([V[i1] - V[i0], V[i2] - V[i0]], 1)
+            l1, l2 = np.linalg.lstsq(M, (hit[sel] - V[i0]).T, rcond=None)[0]
+            l0 = 1 - l1 - l2
+            exp = l2[:, None] * C[i0] + l0[:, None] * C[i1] + l1[:, None] * C[i2]
+            err = np.abs(out["color"][sel].astype(np.float64) - np.floor(exp))
+            if err.max() > 1:
+                return False, f"{pose}: face {f} colour is not (1-u-v)c0 + u c1 + v c2 " \
+                              f"with the raytracer's (u, v) (max err {err.max()})"
+        nrm = m.normals[0].astype(np.float64)
+        if not np.allclose(out["normal"][inner], R @ nrm, atol=1e-5):
+            return False, f"{pose}: normal is not w2c_R n"
+    return True, ""
+
+
+def pin_raytrace_sphere():
+    """Outside a uv sphere (rotated camera): silhouette area of the closed
+    form, hits on the unit sphere, normal = w2c_R face normal. Inside it
+    (camera at the centre): every face is a back face, so culling leaves
+    nothing and without culling every ray meets the sphere at distance ~1."""
+    m, cam = _sphere_scene(rotated=True, nseg=96)
+    out = raytrace(m, cam, RendererCpuOption(diffuse_color=DiffuseColor.kVertex,
+                                             shading_normal=ShadingNormal.kFace))
+    hit, P, N, d, R = _sphere_hits(cam)
+    cov = out["face_id"] >= 0
+    rad = 125.0 / math.sqrt(9.0 - 1.0)
+    if abs(cov.sum() / (math.pi * rad * rad) - 1) > 0.02:
+        return False, f"raytraced silhouette area {cov.sum()}"
+    sel = cov & hit
+    # away from the silhouette (ray-normal cosine > 0.3) the 96-segment sphere
+    # is within 2.2e-3 of the unit sphere along the ray
+    cosang = -(d * (N / np.linalg.norm(N, axis=-1, keepdims=True))).sum(-1)
+    core = sel & (cosang > 0.3)
+    s_ = out["depth"][core] / d[core][:, 2]      # distance along the unit ray
+    dist = np.linalg.norm(P[core], axis=-1)
+    if core.sum() < 1000 or np.abs(s_ - dist).max() > 4e-3:
+        return False, "raytraced depth is not on the sphere"
+    fnc = m.face_normals[out["face_id"][sel]].astype(np.float64) @ R.T
+    if not np.allclose(out["normal"][sel], fnc, atol=1e-5):
+        return False, "raytraced normal is not w2c_R face_normal"
+    # inside
+    c2w = np.eye(4)
+    inside = PinholeCamera(48, 40, c2w, (23.5, 19.5), (30.0, 30.0))
+    o1 = raytrace(m, inside, RendererCpuOption(diffuse_color=DiffuseColor.kVertex))
+    if (o1["mask"] != 0).any() or (o1["face_id"] != -1).any():
+        return False, "inside the sphere every face is culled"
+    o2 = raytrace(m, inside, RendererCpuOption(diffuse_color=DiffuseColor.kVertex,
+                                               backface_culling=False))
+    if not (o2["mask"] == 255).all():
+        return False, "inside the sphere every ray hits"
+    ys, xs = np.mgrid[0:40, 0:48].astype(np.float64)
+    dz = 1.0 / np.sqrt(((xs - 23.5) / 30) ** 2 + ((ys - 19.5) / 30) ** 2 + 1)
+    s2 = o2["depth"] / dz
+    if s2.min() < 0.999 or s2.max() > 1.0001:
+        return False, f"inside distances {s2.min()}..{s2.max()} not ~1"
+    return True, ""
+
+
 ALL_PINS = [pin_brute_force, pin_ties_lowest_id, pin_face_zero_not_shaded,
             pin_perspective_correct, pin_texture_sampling, pin_sphere_closed_forms,
             pin_normals_rotated_camera, pin_lambertian, pin_oren_nayar, pin_depth_u16,
-            pin_gbuffer]
+            pin_gbuffer, pin_raytrace_brute_force, pin_raytrace_plane, pin_raytrace_sphere]

@@ -27,7 +27,7 @@ def test_python_binding_covers_header():
 
 
 def test_abi_version():
-    assert _abi.load_library().ugu_hip_abi_version() == 2
+    assert _abi.load_library().ugu_hip_abi_version() == 3
 
 
 def test_create_without_device_returns_null():
@@ -38,7 +38,7 @@ def test_create_without_device_returns_null():
 
 def test_struct_sizes_match_header_layout():
     import ctypes
-    assert ctypes.sizeof(_abi.UguCamera) == 4 * (2 + 4 + 9 + 3 + 9)
+    assert ctypes.sizeof(_abi.UguCamera) == 4 * (2 + 4 + 9 + 3 + 9 + 3)
     assert ctypes.sizeof(_abi.UguOptions) == 28
     assert ctypes.sizeof(_abi.UguOutputs) == 40
     assert ctypes.sizeof(_abi.UguTiling) == 16

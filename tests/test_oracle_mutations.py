@@ -32,6 +32,11 @@ MUTATIONS = {
     13: "back-face culling ignored at resolve",
     14: "G-buffer barycentrics u and v swapped",
     15: "G-buffer camera position rotated by R^T",
+    16: "raytracer barycentric conversion not applied (u, v swapped)",
+    17: "raytracer keeps the farthest hit",
+    18: "raytracer culling sign flipped",
+    19: "raytracer depth = ray parameter t instead of camera z",
+    20: "raytracer ray origin ignored",
 }
 
 

@@ -5,6 +5,7 @@ rebuilt for MI355X (gfx950) as hand-written HIP kernels behind a C ABI
 Python entry points:
   Rasterizer          mirror of ugu::Rasterizer (set_option/set_mesh/PrepareMesh/
                       set_camera/Render...), plus batched and device-memory renders
+  Raytracer           mirror of ugu::Raytracer (same interface, BVH ray casting)
   Mesh, PinholeCamera, RendererCpuOption, Image*   reference types
   diff.render_diff    differentiable rendering (torch.autograd.Function)
   scenes              seeded benchmark scenes
@@ -12,12 +13,12 @@ Python entry points:
 from .types import (ColorInterpolation, DiffuseColor, DiffuseShading, Image, Image1b,
                     Image1f, Image1i, Image1w, Image3b, Image3f, Mesh, PinholeCamera,
                     RendererCpuOption, ShadingNormal, look_at_c2w)
-from .renderer import Rasterizer
+from .renderer import Rasterizer, Raytracer
 from . import scenes
 
 Renderer = Rasterizer
 
-__all__ = ["Rasterizer", "Renderer", "Mesh", "PinholeCamera", "RendererCpuOption",
+__all__ = ["Rasterizer", "Raytracer", "Renderer", "Mesh", "PinholeCamera", "RendererCpuOption",
            "DiffuseColor", "DiffuseShading", "ShadingNormal", "ColorInterpolation",
            "Image", "Image1b", "Image1f", "Image1i", "Image1w", "Image3b", "Image3f",
            "look_at_c2w", "scenes"]

@@ -57,7 +57,7 @@ class UguCamera(ctypes.Structure):
     _fields_ = [
         ("width", _i), ("height", _i),
         ("fx", _f), ("fy", _f), ("cx", _f), ("cy", _f),
-        ("w2c_R", _f * 9), ("w2c_t", _f * 3), ("c2w_R", _f * 9),
+        ("w2c_R", _f * 9), ("w2c_t", _f * 3), ("c2w_R", _f * 9), ("c2w_t", _f * 3),
     ]
 
 
@@ -125,6 +125,10 @@ SIGNATURES = {
                                               ctypes.POINTER(UguOptions),
                                               ctypes.POINTER(UguOutputs),
                                               ctypes.POINTER(UguGBuffer), ctypes.c_int]),
+    "ugu_hip_prepare_raytracer": (ctypes.c_int, [_P]),
+    "ugu_hip_raytrace": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,
+                                        ctypes.POINTER(UguOptions), ctypes.POINTER(UguOutputs),
+                                        ctypes.c_int]),
     "ugu_hip_set_tiling": (ctypes.c_int, [_P, ctypes.POINTER(UguTiling)]),
     "ugu_hip_get_tiling": (ctypes.c_int, [_P, ctypes.POINTER(UguTiling)]),
     "ugu_hip_autotune": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,

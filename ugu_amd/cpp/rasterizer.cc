@@ -60,6 +60,8 @@ bool to_abi(const Camera& camera, UguCamera* c) {
     }
     c->w2c_t[r] = w2c_t[r];
   }
+  const Eigen::Vector3f c2w_t = p->c2w().translation().cast<float>();
+  for (int r = 0; r < 3; r++) c->c2w_t[r] = c2w_t[r];
   return true;
 }
 

@@ -25,73 +25,11 @@
 #include <float.h>
 #include <limits.h>
 
-#include "raster_types.h"
+#include "device_common.h"
 
 #pragma clang fp contract(off)
 
 namespace ugu_hip {
-
-#define UGU_DEV __device__ __forceinline__
-
-// ---------------------------------------------------------------------------
-// scalar helpers
-
-// Eigen's unrolled reduction of a fixed 3-vector: x0 + (x1 + x2)
-UGU_DEV float sum3(float a, float b, float c) { return a + (b + c); }
-
-// EdgeFunction(a, b, c) = (c0-a0)(b1-a1) - (c1-a1)(b0-a0)  (rasterizer.cc:25-28)
-UGU_DEV float edge_fn(float ax, float ay, float bx, float by, float cx, float cy) {
-  return (cx - ax) * (by - ay) - (cy - ay) * (bx - ax);
-}
-
-// float -> int32 as x86-64 cvttss2si (what the reference's static_cast
-// compiles to): truncation, INT32_MIN for NaN and out-of-range values.
-UGU_DEV int32_t f2i_x86(float f) {
-  return (f >= -2147483648.0f && f < 2147483648.0f) ? static_cast<int32_t>(f)
-                                                    : INT32_MIN;
-}
-// saturating float -> int32 for the pixel bbox upper bound
-UGU_DEV int32_t f2i_sat(float f) {
-  if (f >= 2147483648.0f) return INT32_MAX;
-  if (f >= -2147483648.0f) return static_cast<int32_t>(f);
-  return INT32_MIN;
-}
-UGU_DEV uint8_t f2u8_x86(float f) {
-  return static_cast<uint8_t>(static_cast<uint32_t>(f2i_x86(f)) & 0xffu);
-}
-
-// std::min({a,b,c}) / std::max({a,b,c}) including their NaN behaviour
-UGU_DEV float min3_std(float a, float b, float c) {
-  float m = a;
-  if (b < m) m = b;
-  if (c < m) m = c;
-  return m;
-}
-UGU_DEV float max3_std(float a, float b, float c) {
-  float m = a;
-  if (m < b) m = b;
-  if (m < c) m = c;
-  return m;
-}
-
-// PinholeCamera::ray_w(float x, float y) (camera.h:438-448):
-// d = ((x-cx)/fx, (y-cy)/fy, 1); d.normalize(); ray = c2w_R_f * d
-UGU_DEV void ray_w_at(const DevView& v, float x, float y, float& r0, float& r1,
-                      float& r2) {
-  float d0 = (x - v.cx) / v.fx;
-  float d1 = (y - v.cy) / v.fy;
-  float d2 = 1.0f;
-  float z = sum3(d0 * d0, d1 * d1, d2 * d2);
-  if (z > 0.0f) {
-    float s = sqrtf(z);
-    d0 = d0 / s;
-    d1 = d1 / s;
-    d2 = d2 / s;
-  }
-  r0 = sum3(v.c2w_R[0] * d0, v.c2w_R[1] * d1, v.c2w_R[2] * d2);
-  r1 = sum3(v.c2w_R[3] * d0, v.c2w_R[4] * d1, v.c2w_R[5] * d2);
-  r2 = sum3(v.c2w_R[6] * d0, v.c2w_R[7] * d1, v.c2w_R[8] * d2);
-}
 
 // w2c_R * p + w2c_t, then PinholeCamera::Project (camera.h:380-387):
 // x = fx / z * x + cx (divide first), y likewise, z kept.
@@ -168,22 +106,6 @@ UGU_DEV bool cover_test(float ax, float ay, float az, float bx, float by,
   return true;
 }
 
-// backface = face_normal . ray_w > 0 (rasterizer.cc:200)
-UGU_DEV bool backface_of(const float* fn, int64_t fid, float r0, float r1, float r2) {
-  if (fn == nullptr) return false;
-  const float* n = fn + fid * 3;
-  return sum3(n[0] * r0, n[1] * r1, n[2] * r2) > 0.0f;
-}
-
-// ImageBase::at on an RGB8 texture (image.h:416-427) with the linear index
-// clamped to the image. Device textures are stored RGBA8 (alpha unused) so a
-// texel is one aligned 32-bit load.
-UGU_DEV uchar4 tex_at(const uint8_t* base, int w, int h, int row, int col) {
-  int64_t lin = static_cast<int64_t>(row) * w + col;
-  int64_t n = static_cast<int64_t>(w) * h;
-  lin = lin < 0 ? 0 : (lin >= n ? n - 1 : lin);
-  return reinterpret_cast<const uchar4*>(base)[lin];
-}
 
 UGU_DEV int xcd_remap(int b, int nb) {
   // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -532,83 +454,6 @@ UGU_DEV void sequential_pixel(const DevView& v, const DevMesh& mesh,
   out_fid = dfid;
 }
 
-// Colour of one shaded pixel of the reference path: colourizer then shader
-// (pixel_shader.h:215-218 with the classes at :222-422).
-UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, float u, float vv,
-                      float r0, float r1, float r2, float n0, float n1, float n2,
-                      uint8_t c[3]) {
-  if (o.diffuse_color == 2) {
-    // DiffuseVertexColorColorizer (pixel_shader.h:234-256)
-    const int32_t* fi = m.faces + int64_t(fid) * 3;
-    const float* c0 = m.vcolors + int64_t(fi[0]) * 3;
-    const float* c1 = m.vcolors + int64_t(fi[1]) * 3;
-    const float* c2 = m.vcolors + int64_t(fi[2]) * 3;
-    const float w = 1.0f - u - vv;
-    for (int k = 0; k < 3; k++) c[k] = f2u8_x86(w * c0[k] + u * c1[k] + vv * c2[k]);
-  } else if (o.diffuse_color == 1) {
-    const TexInfo t = m.tex[m.matid[fid]];
-    const uint8_t* base = m.texdata + t.offset;
-    const int32_t* ui = m.uvidx + int64_t(fid) * 3;
-    const float* a = m.uv + int64_t(ui[0]) * 2;
-    const float* b = m.uv + int64_t(ui[1]) * 2;
-    const float* cc = m.uv + int64_t(ui[2]) * 2;
-    const float w = 1.0f - u - vv;
-    const float iu = w * a[0] + u * b[0] + vv * cc[0];
-    const float iv = w * a[1] + u * b[1] + vv * cc[1];
-    const float fx = iu * static_cast<float>(t.w - 1);
-    const float fy = (1.0f - iv) * static_cast<float>(t.h - 1);
-    if (o.interp == 0) {
-      // DiffuseTextureNnColorizer (pixel_shader.h:260-298)
-      const uchar4 dt = tex_at(base, t.w, t.h, f2i_x86(roundf(fy)), f2i_x86(roundf(fx)));
-      c[0] = dt.x;
-      c[1] = dt.y;
-      c[2] = dt.z;
-    } else {
-      // DiffuseTextureBilinearColorizer (pixel_shader.h:302-359)
-      const int x0 = f2i_x86(floorf(fx));
-      const int y0 = f2i_x86(floorf(fy));
-      const int x1 = x0 + 1, y1 = y0 + 1;
-      const float lu = fx - static_cast<float>(x0);
-      const float lv = fy - static_cast<float>(y0);
-      const uchar4 q00 = tex_at(base, t.w, t.h, y0, x0);
-      const uchar4 q01 = tex_at(base, t.w, t.h, y0, x1);
-      const uchar4 q10 = tex_at(base, t.w, t.h, y1, x0);
-      const uchar4 q11 = tex_at(base, t.w, t.h, y1, x1);
-      const uint8_t p00[3] = {q00.x, q00.y, q00.z}, p01[3] = {q01.x, q01.y, q01.z};
-      const uint8_t p10[3] = {q10.x, q10.y, q10.z}, p11[3] = {q11.x, q11.y, q11.z};
-      for (int k = 0; k < 3; k++) {
-        float ic = (1.0f - lu) * (1.0f - lv) * static_cast<float>(p00[k]) +
-                   lu * (1.0f - lv) * static_cast<float>(p01[k]) +
-                   (1.0f - lu) * lv * static_cast<float>(p10[k]) +
-                   lu * lv * static_cast<float>(p11[k]);
-        c[k] = f2u8_x86(ic);
-      }
-    }
-  } else {
-    // DiffuseDefaultColorizer (pixel_shader.h:222-230)
-    c[0] = c[1] = c[2] = 255;
-  }
-  if (o.diffuse_shading == 1) {
-    // DiffuseLambertianShader (pixel_shader.h:370-388); light_dir = ray_w
-    float coeff = -sum3(r0 * n0, r1 * n1, r2 * n2);
-    if (coeff < 0.0f) coeff = 0.0f;
-    for (int k = 0; k < 3; k++) c[k] = f2u8_x86(coeff * static_cast<float>(c[k]));
-  } else if (o.diffuse_shading == 2) {
-    // DiffuseOrenNayarShader (pixel_shader.h:392-422); light_dir = ray_w
-    const float dot_light = -sum3(r0 * n0, r1 * n1, r2 * n2);
-    const float theta_i = acosf(dot_light);
-    const float dot_ray = -sum3(r0 * n0, r1 * n1, r2 * n2);
-    const float theta_r = acosf(dot_ray);
-    const float bl0 = -n0 * dot_light - r0, bl1 = -n1 * dot_light - r1,
-                bl2 = -n2 * dot_light - r2;
-    const float phi_diff_cos = fmaxf(0.0f, sum3(bl0 * bl0, bl1 * bl1, bl2 * bl2));
-    const float alpha = fmaxf(theta_i, theta_r);
-    const float beta = fminf(theta_i, theta_r);
-    const float coeff = fmaxf(0.0f, dot_light) *
-                        (o.on_A + (o.on_B * phi_diff_cos * sinf(alpha) * tanf(beta)));
-    for (int k = 0; k < 3; k++) c[k] = f2u8_x86(coeff * static_cast<float>(c[k]));
-  }
-}
 
 // Bilinear sample of an f32 RGB texture with clamp-to-edge addressing
 // (differentiable path). Returns the top-left texel and the local weights.

@@ -14,6 +14,7 @@ struct DevView {
   float w2c_R[9];
   float w2c_t[3];
   float c2w_R[9];
+  float c2w_t[3];     // camera position: the raytracer's ray origin
   int32_t tiles_x, tiles_y;
   int32_t tile_base;  // first global tile of this view
   int32_t pad0;
@@ -71,6 +72,27 @@ struct DevOptions {
   int32_t diffuse_shading;  // 0 none, 1 lambertian, 2 oren-nayar
   int32_t culling;
   float on_A, on_B;         // Oren-Nayar constants for sigma 0.3
+  float depth_scale;        // raytracer depth factor (the rasterizer ignores it)
+};
+
+// Bounding volume hierarchy of the raytracer. Every inner node stores the
+// boxes of its two children; a child reference is an inner node index (>= 0,
+// always greater than the parent's), ~leaf for a leaf (a run of triangle
+// records) or kBvhEmpty. Boxes are widened outward so that the float slab test
+// never rejects a box that holds a float64 Moller-Trumbore hit.
+struct BvhNode {
+  float4 a_lo;  // child A box minimum, w = child A reference (int bits)
+  float4 a_hi;  // child A box maximum, w unused
+  float4 b_lo;  // child B
+  float4 b_hi;
+};
+constexpr int32_t kBvhEmpty = INT32_MIN;
+constexpr int kBvhStack = 48;  // traversal stack entries; the builder keeps depth below
+
+struct DevBvh {
+  const BvhNode* nodes;
+  const int2* leaves;   // (first triangle record, count)
+  const float4* tris;   // 3 per record: (v0, face id bits) (v1, 0) (v2, 0)
 };
 
 struct DevBins {

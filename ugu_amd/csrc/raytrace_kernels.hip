@@ -1,0 +1,214 @@
+// HIP kernel of the gfx950 raytracer: Raytracer::Impl::Render
+// (src/renderer/cpu/raytracer.cc:117-225) for a batch of views.
+//
+// One wave per 8x8 pixel tile, one lane per pixel. Each lane walks the BVH
+// (built on the host, nodes hold both children's boxes) with a stack in LDS,
+// nearer child first, and tests the triangles of every leaf it reaches with
+// the reference's float64 Moller-Trumbore routine (src/util/geom_util.cc:60-115,
+// epsilon double(1e-8f) of bvh_naive.h:121, standard barycentric conversion).
+// The closest hit is the smallest float t; equal t resolve to the lowest face
+// id. The box test is float32 with widened boxes and a relative slack on the
+// ray interval, so it only prunes boxes that cannot hold a hit the float64 test
+// would accept. Then the pixel is culled or shaded as in raytracer.cc:160-218.
+
+#include "device_common.h"
+
+#pragma clang fp contract(off)
+
+namespace ugu_hip {
+
+// Eigen's dot of two float64 3-vectors. With SSE2 Eigen reduces a 3-vector
+// of doubles with one Packet2d for coefficients 0..1 and a scalar tail
+// (redux_impl, LinearVectorizedTraversal): (x0 + x1) + x2.
+UGU_DEV double dot_d(double a0, double a1, double a2, double b0, double b1, double b2) {
+  return (a0 * b0 + a1 * b1) + a2 * b2;
+}
+
+// IntersectImpl (geom_util.cc:60-115) on float vertices promoted to double.
+// Returns true with the float casts of (t, u, v) on a hit.
+UGU_DEV bool mt_hit(double ox, double oy, double oz, double dx, double dy, double dz,
+                    const float4& p0, const float4& p1, const float4& p2, float& t,
+                    float& u, float& v) {
+  const double eps = static_cast<double>(1e-8f);
+  const double v0x = p0.x, v0y = p0.y, v0z = p0.z;
+  const double e1x = double(p1.x) - v0x, e1y = double(p1.y) - v0y, e1z = double(p1.z) - v0z;
+  const double e2x = double(p2.x) - v0x, e2y = double(p2.y) - v0y, e2z = double(p2.z) - v0z;
+  // alpha = ray x e2
+  const double ax = dy * e2z - dz * e2y;
+  const double ay = dz * e2x - dx * e2z;
+  const double az = dx * e2y - dy * e2x;
+  const double det = dot_d(e1x, e1y, e1z, ax, ay, az);
+  if (-eps < det && det < eps) return false;
+  const double inv_det = 1.0 / det;
+  const double rx = ox - v0x, ry = oy - v0y, rz = oz - v0z;
+  const double uu = dot_d(ax, ay, az, rx, ry, rz) * inv_det;
+  if (uu < 0.0 || uu > 1.0) return false;
+  // beta = r x e1
+  const double bx = ry * e1z - rz * e1y;
+  const double by = rz * e1x - rx * e1z;
+  const double bz = rx * e1y - ry * e1x;
+  const double vv = dot_d(dx, dy, dz, bx, by, bz) * inv_det;
+  if (vv < 0.0 || uu + vv > 1.0) return false;
+  const double tt = dot_d(e2x, e2y, e2z, bx, by, bz) * inv_det;
+  if (tt < 0.0) return false;
+  const double w = 1.0 - uu - vv;
+  t = static_cast<float>(tt);
+  u = static_cast<float>(w);
+  v = static_cast<float>(uu);
+  return true;
+}
+
+// Float slab test of a widened box. Returns the interval entry (>= 0) or
+// +inf when the box is missed or lies beyond tmax. The relative slack of
+// 2^-20 covers the float rounding of the slab arithmetic (a few ulp).
+UGU_DEV float box_entry(const float4& lo, const float4& hi, float ox, float oy, float oz,
+                        float ix, float iy, float iz, float tmax) {
+  const float tx0 = (lo.x - ox) * ix, tx1 = (hi.x - ox) * ix;
+  const float ty0 = (lo.y - oy) * iy, ty1 = (hi.y - oy) * iy;
+  const float tz0 = (lo.z - oz) * iz, tz1 = (hi.z - oz) * iz;
+  // fminf/fmaxf return the other operand for NaN ((b - o) * inf with b == o);
+  // that happens only where the ray runs along a widened face, which holds no
+  // geometry.
+  const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+  const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+  const float k = 9.5367431640625e-07f;  // 2^-20
+  const float tn_lo = tn - tn * k;
+  const float tf_hi = tf + fabsf(tf) * k;
+  return tn_lo <= tf_hi ? tn_lo : __builtin_huge_valf();
+}
+
+__global__ __launch_bounds__(64) void k_raytrace(const DevView* __restrict__ views,
+                                                 DevBvh bvh, DevMesh m, DevOptions o) {
+  __shared__ int32_t stack[kBvhStack * 64];
+  const DevView& v = views[blockIdx.y];
+  const int tiles_x = (v.W + 7) >> 3;
+  const int tile = blockIdx.x;
+  if (tile >= tiles_x * ((v.H + 7) >> 3)) return;
+  const int lane = threadIdx.x;
+  const int x = (tile % tiles_x) * 8 + (lane & 7);
+  const int y = (tile / tiles_x) * 8 + (lane >> 3);
+  if (x >= v.W || y >= v.H) return;
+  const int64_t pi = int64_t(y) * v.W + x;
+
+  // ray_w / org_ray_w tables (camera.h:482-513)
+  float r0, r1, r2;
+  ray_w_at(v, static_cast<float>(x), static_cast<float>(y), r0, r1, r2);
+  const float ox = v.c2w_t[0], oy = v.c2w_t[1], oz = v.c2w_t[2];
+  const float ix = 1.0f / r0, iy = 1.0f / r1, iz = 1.0f / r2;
+  const double dox = ox, doy = oy, doz = oz, ddx = r0, ddy = r1, ddz = r2;
+
+  float bt = __builtin_huge_valf(), bu = 0.0f, bv = 0.0f;
+  int bf = -1;
+  int sp = 0;
+  int ref = 0;  // the root is inner node 0
+  // Every inner reference points to a node of larger index and every node is
+  // pushed at most once, so the walk ends after at most (nodes + leaves) steps.
+  for (;;) {
+    if (ref >= 0) {
+      const BvhNode nd = bvh.nodes[ref];
+      const float tmax = bt + bt * 9.5367431640625e-07f;
+      const int ra = __float_as_int(nd.a_lo.w), rb = __float_as_int(nd.b_lo.w);
+      const float ta = box_entry(nd.a_lo, nd.a_hi, ox, oy, oz, ix, iy, iz, tmax);
+      const float tb = rb == kBvhEmpty ? __builtin_huge_valf()
+                                       : box_entry(nd.b_lo, nd.b_hi, ox, oy, oz, ix, iy, iz, tmax);
+      const bool ha = ta != __builtin_huge_valf(), hb = tb != __builtin_huge_valf();
+      if (ha && hb) {
+        const bool a_first = ta <= tb;
+        stack[sp * 64 + lane] = a_first ? rb : ra;
+        sp++;
+        ref = a_first ? ra : rb;
+        continue;
+      }
+      if (ha || hb) {
+        ref = ha ? ra : rb;
+        continue;
+      }
+    } else {
+      const int2 lf = bvh.leaves[~ref];
+      for (int k = 0; k < lf.y; k++) {
+        const float4* tr = bvh.tris + int64_t(lf.x + k) * 3;
+        const float4 p0 = tr[0], p1 = tr[1], p2 = tr[2];
+        float t, u, w;
+        if (mt_hit(dox, doy, doz, ddx, ddy, ddz, p0, p1, p2, t, u, w)) {
+          const int f = __float_as_int(p0.w);
+          if (t < bt || (t == bt && f < bf)) {
+            bt = t;
+            bu = u;
+            bv = w;
+            bf = f;
+          }
+        }
+      }
+    }
+    if (sp == 0) break;
+    sp--;
+    ref = stack[sp * 64 + lane];
+  }
+
+  bool hit = bf >= 0;
+  // back-face culling: face normal . ray_w > 0 -> background (raytracer.cc:166-172)
+  if (hit && o.culling && backface_of(m.fnormals, bf, r0, r1, r2)) hit = false;
+
+  if (!hit) {
+    if (v.color) {
+      v.color[pi * 3 + 0] = 0;
+      v.color[pi * 3 + 1] = 0;
+      v.color[pi * 3 + 2] = 0;
+    }
+    if (v.depth) v.depth[pi] = 0.0f;
+    if (v.normal) {
+      v.normal[pi * 3 + 0] = 0.0f;
+      v.normal[pi * 3 + 1] = 0.0f;
+      v.normal[pi * 3 + 2] = 0.0f;
+    }
+    if (v.mask) v.mask[pi] = 0;
+    if (v.face_id) v.face_id[pi] = -1;
+    return;
+  }
+  if (v.face_id) v.face_id[pi] = bf;
+  if (v.mask) v.mask[pi] = 255;
+  if (v.depth) {
+    // hit_pos_c = w2c_R * (org + ray * t) + w2c_t; depth = z * depth_scale
+    const float h0 = ox + r0 * bt, h1 = oy + r1 * bt, h2 = oz + r2 * bt;
+    const float zc = sum3(v.w2c_R[6] * h0, v.w2c_R[7] * h1, v.w2c_R[8] * h2) + v.w2c_t[2];
+    v.depth[pi] = zc * o.depth_scale;
+  }
+  float n0, n1, n2;
+  if (o.shading_normal == 0) {
+    const float* fn = m.fnormals + int64_t(bf) * 3;
+    n0 = fn[0];
+    n1 = fn[1];
+    n2 = fn[2];
+  } else {
+    // (1 - u - v) n[i0] + u n[i1] + v n[i2] with the raytracer's (u, v)
+    const int32_t* ni = m.nidx + int64_t(bf) * 3;
+    const float* na = m.normals + int64_t(ni[0]) * 3;
+    const float* nb = m.normals + int64_t(ni[1]) * 3;
+    const float* nc = m.normals + int64_t(ni[2]) * 3;
+    const float w = 1.0f - bu - bv;
+    n0 = w * na[0] + bu * nb[0] + bv * nc[0];
+    n1 = w * na[1] + bu * nb[1] + bv * nc[1];
+    n2 = w * na[2] + bu * nb[2] + bv * nc[2];
+  }
+  if (v.normal) {
+    v.normal[pi * 3 + 0] = sum3(v.w2c_R[0] * n0, v.w2c_R[1] * n1, v.w2c_R[2] * n2);
+    v.normal[pi * 3 + 1] = sum3(v.w2c_R[3] * n0, v.w2c_R[4] * n1, v.w2c_R[5] * n2);
+    v.normal[pi * 3 + 2] = sum3(v.w2c_R[6] * n0, v.w2c_R[7] * n1, v.w2c_R[8] * n2);
+  }
+  if (v.color) {
+    uint8_t c[3];
+    shade_u8(m, o, bf, bu, bv, r0, r1, r2, n0, n1, n2, c);
+    v.color[pi * 3 + 0] = c[0];
+    v.color[pi * 3 + 1] = c[1];
+    v.color[pi * 3 + 2] = c[2];
+  }
+}
+
+int launch_raytrace(hipStream_t st, const DevView* views, int n_views, int max_tiles,
+                    const DevBvh& bvh, const DevMesh& m, const DevOptions& o) {
+  if (n_views < 1 || max_tiles < 1) return 0;
+  hipLaunchKernelGGL(k_raytrace, dim3(max_tiles, n_views), dim3(64), 0, st, views, bvh, m, o);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace ugu_hip

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/ugu_hip.h"
+#include "bvh_build.h"
 #include "raster_types.h"
 
 namespace ugu_hip {
@@ -34,6 +35,8 @@ int launch_backward(hipStream_t st, const DevView* views, int n_views,
                     int max_pixels, int64_t total_pixels, const DevMesh& mesh,
                     const DevGrads& g, int4* items, int32_t* scratch);
 int launch_depth_to_u16(hipStream_t st, const float* d, int64_t n, uint16_t* out);
+int launch_raytrace(hipStream_t st, const DevView* views, int n_views, int max_tiles,
+                    const DevBvh& bvh, const DevMesh& m, const DevOptions& o);
 }  // namespace ugu_hip
 
 using namespace ugu_hip;
@@ -110,6 +113,9 @@ struct ugu_hip_ctx {
   DevBuf soup, diff_soup;  // 48-byte face records (prepared mesh / differentiable path)
   DevBuf blk_off, blk_vid, face_loc;           // setup blocks' vertex lists (prepared mesh)
   DevBuf blk_off_o, blk_vid_o, face_loc_o;     // the same for the original face array
+  DevBuf bvh_nodes, bvh_leaves, bvh_tris;      // raytracer hierarchy
+  bool rt_ready = false;
+  int64_t rt_mesh_gen = -1;  // mesh generation the hierarchy was built for
   int64_t buf_gen = 0;
   void* h_views = nullptr;
   size_t h_views_bytes = 0;
@@ -321,6 +327,7 @@ DevOptions dev_options(const UguOptions& o) {
   const float sigma_2 = sigma * sigma;
   d.on_A = 1.0f - (0.5f * sigma_2 / (sigma_2 + 0.33f));
   d.on_B = 0.45f * sigma_2 / (sigma_2 + 0.09f);
+  d.depth_scale = o.depth_scale;
   return d;
 }
 
@@ -450,6 +457,7 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
     std::memcpy(v.w2c_R, cams[i].w2c_R, sizeof(v.w2c_R));
     std::memcpy(v.w2c_t, cams[i].w2c_t, sizeof(v.w2c_t));
     std::memcpy(v.c2w_R, cams[i].c2w_R, sizeof(v.c2w_R));
+    std::memcpy(v.c2w_t, cams[i].c2w_t, sizeof(v.c2w_t));
     v.tiles_x = (v.W + tw - 1) / tw;
     v.tiles_y = (v.H + th - 1) / th;
     v.tile_base = c->h_tile_base[i];
@@ -621,6 +629,120 @@ int render_common(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions
                       gbufs);
 }
 
+// Raytracer::Render for n views (outputs[i] belongs to cams[i]).
+int raytrace_common(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions* o,
+                    const UguOutputs* outs, int memory) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  if (n < 1 || cams == nullptr || outs == nullptr)
+    return fail(c, n < 1 ? UGU_HIP_ERR_INVALID_ARGUMENT : UGU_HIP_ERR_NO_CAMERA,
+                cams == nullptr ? "camera has not been set\n" : "invalid view count or outputs");
+  if (memory != UGU_HIP_MEMORY_HOST && memory != UGU_HIP_MEMORY_DEVICE)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "unknown memory kind");
+  for (int i = 0; i < n; i++) {
+    void* p[kPlanes];
+    plane_ptrs(outs[i], nullptr, p);
+    bool any = false;
+    for (int k = 0; k < kPlanes; k++) any = any || p[k] != nullptr;
+    // mesh_initialized_ of the raytracer is set by its own PrepareMesh
+    if (c->prepared && !(c->rt_ready && c->rt_mesh_gen == c->mesh_gen) && cams != nullptr)
+      return fail(c, UGU_HIP_ERR_MESH_NOT_PREPARED, "mesh has not been initialized\n");
+    int err = validate(c, &cams[i], o, any);
+    if (err) return err;
+    if ((err = check_camera(c, cams[i]))) return err;
+  }
+  UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int oe = order_after_user(c)) return oe;
+
+  // output staging for host memory, view descriptors
+  std::vector<int64_t> stage_off(size_t(n) * kPlanes, -1);
+  int64_t stage_bytes = 0;
+  int max_tiles = 0;
+  for (int i = 0; i < n; i++) {
+    const int64_t px = int64_t(cams[i].width) * cams[i].height;
+    const int64_t tiles = int64_t((cams[i].width + 7) / 8) * ((cams[i].height + 7) / 8);
+    max_tiles = std::max<int>(max_tiles, static_cast<int>(tiles));
+    if (memory == UGU_HIP_MEMORY_HOST) {
+      void* req[kPlanes];
+      plane_ptrs(outs[i], nullptr, req);
+      for (int k = 0; k < kPlanes; k++) {
+        if (req[k]) {
+          stage_off[size_t(i) * kPlanes + k] = stage_bytes;
+          stage_bytes += (kPlaneBytes[k] * px + 255) / 256 * 256;
+        }
+      }
+    }
+  }
+  int err;
+  if ((err = ensure(c, c->views, size_t(n) * sizeof(DevView)))) return err;
+  if (stage_bytes > 0 && (err = ensure(c, c->stage_out, size_t(stage_bytes)))) return err;
+  const size_t view_bytes = size_t(n) * sizeof(DevView);
+  if (c->h_views_bytes < view_bytes) {
+    if (c->h_views) (void)hipHostFree(c->h_views);
+    c->h_views = nullptr;
+    c->h_views_bytes = 0;
+    UGU_HIP_CHECK(c, hipHostMalloc(&c->h_views, view_bytes, hipHostMallocDefault));
+    c->h_views_bytes = view_bytes;
+  }
+  DevView* hv = static_cast<DevView*>(c->h_views);
+  for (int i = 0; i < n; i++) {
+    DevView& v = hv[i];
+    std::memset(&v, 0, sizeof(v));
+    v.W = cams[i].width;
+    v.H = cams[i].height;
+    v.fx = cams[i].fx;
+    v.fy = cams[i].fy;
+    v.cx = cams[i].cx;
+    v.cy = cams[i].cy;
+    std::memcpy(v.w2c_R, cams[i].w2c_R, sizeof(v.w2c_R));
+    std::memcpy(v.w2c_t, cams[i].w2c_t, sizeof(v.w2c_t));
+    std::memcpy(v.c2w_R, cams[i].c2w_R, sizeof(v.c2w_R));
+    std::memcpy(v.c2w_t, cams[i].c2w_t, sizeof(v.c2w_t));
+    void* p[kPlanes];
+    plane_ptrs(outs[i], nullptr, p);
+    if (memory == UGU_HIP_MEMORY_HOST) {
+      uint8_t* sb = c->stage_out.as<uint8_t>();
+      const int64_t* so = &stage_off[size_t(i) * kPlanes];
+      for (int k = 0; k < kPlanes; k++) p[k] = so[k] >= 0 ? sb + so[k] : nullptr;
+    }
+    set_view_planes(v, p);
+  }
+  UGU_HIP_CHECK(c, hipMemcpyAsync(c->views.p, c->h_views, view_bytes, hipMemcpyHostToDevice,
+                                  c->stream));
+  DevBvh bvh;
+  bvh.nodes = c->bvh_nodes.as<BvhNode>();
+  bvh.leaves = c->bvh_leaves.as<int2>();
+  bvh.tris = c->bvh_tris.as<float4>();
+  const DevMesh dm = dev_mesh(c);
+  const DevOptions dopt = dev_options(*o);
+  c->tm = UguTimings{};
+  if (c->profile) (void)hipEventRecord(c->ev[0], c->stream);
+  if (launch_raytrace(c->stream, c->views.as<DevView>(), n, max_tiles, bvh, dm, dopt))
+    return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
+  if (c->profile) (void)hipEventRecord(c->ev[1], c->stream);
+  if (memory == UGU_HIP_MEMORY_HOST) {
+    const uint8_t* sb = c->stage_out.as<uint8_t>();
+    for (int i = 0; i < n; i++) {
+      const int64_t px = int64_t(cams[i].width) * cams[i].height;
+      void* dst[kPlanes];
+      plane_ptrs(outs[i], nullptr, dst);
+      for (int k = 0; k < kPlanes; k++) {
+        const int64_t off = stage_off[size_t(i) * kPlanes + k];
+        if (off >= 0)
+          UGU_HIP_CHECK(c, hipMemcpyAsync(dst[k], sb + off, size_t(kPlaneBytes[k] * px),
+                                          hipMemcpyDeviceToHost, c->stream));
+      }
+    }
+  }
+  UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  if (c->profile) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    c->tm.raster_ms = ms;
+    c->tm.total_ms = ms;
+  }
+  return UGU_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -664,7 +786,8 @@ void ugu_hip_destroy(ugu_hip_ctx* c) {
                     &c->texdata, &c->proj, &c->counts, &c->bins, &c->status,
                     &c->views, &c->tile_base, &c->stage_out, &c->diff_fn, &c->tile_view,
                     &c->bwd_items, &c->bwd_scratch, &c->soup, &c->diff_soup, &c->blk_off,
-                    &c->blk_vid, &c->face_loc, &c->blk_off_o, &c->blk_vid_o, &c->face_loc_o};
+                    &c->blk_vid, &c->face_loc, &c->blk_off_o, &c->blk_vid_o, &c->face_loc_o,
+                    &c->bvh_nodes, &c->bvh_leaves, &c->bvh_tris};
   for (DevBuf* b : bufs) b->release();
   if (c->h_views) (void)hipHostFree(c->h_views);
   if (c->h_status) (void)hipHostFree(c->h_status);
@@ -835,6 +958,41 @@ int ugu_hip_render_gbuffer(ugu_hip_ctx* c, const UguCamera* cams, int n, const U
 int ugu_hip_render_batch_eager(ugu_hip_ctx* c, const UguCamera* cams, int n,
                                const UguOptions* o, const UguOutputs* outs, int memory) {
   return render_common(c, cams, n, o, outs, memory, false);
+}
+
+int ugu_hip_prepare_raytracer(ugu_hip_ctx* c) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  c->rt_ready = false;
+  // Raytracer::Impl::PrepareMesh (raytracer.cc:90-108)
+  if (!c->prepared) return fail(c, UGU_HIP_ERR_NO_MESH, "mesh has not been set\n");
+  if (c->V == 0 || c->F == 0) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "mesh is empty\n");
+  UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int oe = order_after_user(c)) return oe;
+  // the face records (positions in face order) are already on the device
+  std::vector<float> rec(size_t(c->F) * 12);
+  UGU_HIP_CHECK(c, hipMemcpyAsync(rec.data(), c->soup.p, rec.size() * 4, hipMemcpyDeviceToHost,
+                                  c->stream));
+  UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  BvhBuild b;
+  // inner nodes at depth <= kBvhStack - 1: a walk holds at most one deferred
+  // child per level, so the stack never exceeds kBvhStack entries
+  const int br = build_bvh(rec.data(), c->F, kBvhStack - 1, &b);
+  if (br == -1)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "no face has finite vertex coordinates");
+  if (br != 0 || b.max_inner_depth + 1 > kBvhStack)
+    return fail(c, UGU_HIP_ERR_HIP, "hierarchy failed its structure checks");
+  int err;
+  if ((err = upload(c, c->bvh_nodes, b.nodes.data(), b.nodes.size() * 4))) return err;
+  if ((err = upload(c, c->bvh_leaves, b.leaves.data(), b.leaves.size() * 4))) return err;
+  if ((err = upload(c, c->bvh_tris, b.tris.data(), b.tris.size() * 4))) return err;
+  c->rt_mesh_gen = c->mesh_gen;
+  c->rt_ready = true;
+  return UGU_HIP_OK;
+}
+
+int ugu_hip_raytrace(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions* o,
+                     const UguOutputs* outs, int memory) {
+  return raytrace_common(c, cams, n, o, outs, memory);
 }
 
 int ugu_hip_set_tiling(ugu_hip_ctx* c, const UguTiling* t) {

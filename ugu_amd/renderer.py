@@ -170,8 +170,8 @@ class Rasterizer:
                                 ("color", "depth", "normal", "mask", "face_id")])
         cam = self.camera_.to_abi()
         opt = options_to_abi(self.option_)
-        rc = self._lib.ugu_hip_render(self._ctx, ctypes.byref(cam), ctypes.byref(opt),
-                                      ctypes.byref(out), _abi.UGU_HIP_MEMORY_HOST)
+        rc = self._render_views(ctypes.byref(cam), 1, ctypes.byref(opt), ctypes.byref(out),
+                                _abi.UGU_HIP_MEMORY_HOST, graph=False)
         if rc != 0:
             LOGE(self.last_error())
             return False
@@ -215,6 +215,12 @@ class Rasterizer:
         return self.RenderW(None, depth, None, None, None)
 
     # --- extensions ---------------------------------------------------------
+    def _render_views(self, cams, n, opt, outs, memory, graph=True):
+        if n == 1 and not graph:
+            return self._lib.ugu_hip_render(self._ctx, cams, opt, outs, memory)
+        fn = self._lib.ugu_hip_render_batch if graph else self._lib.ugu_hip_render_batch_eager
+        return fn(self._ctx, cams, n, opt, outs, memory)
+
     def last_error(self):
         return self._lib.ugu_hip_last_error(self._ctx).decode()
 
@@ -275,9 +281,8 @@ class Rasterizer:
         cams = (_abi.UguCamera * len(cameras))(
             *[c if isinstance(c, _abi.UguCamera) else c.to_abi() for c in cameras])
         opt = options_to_abi(self.option_)
-        fn = self._lib.ugu_hip_render_batch if graph else self._lib.ugu_hip_render_batch_eager
-        self._check(fn(self._ctx, cams, len(cameras), ctypes.byref(opt), outs,
-                       _abi.UGU_HIP_MEMORY_HOST))
+        self._check(self._render_views(cams, len(cameras), ctypes.byref(opt), outs,
+                                       _abi.UGU_HIP_MEMORY_HOST, graph=graph))
         return res
 
     GBUFFER_PLANES = ("bary", "uv", "pos_wld", "pos_cam", "normal_wld", "geo_id")
@@ -325,11 +330,47 @@ class Rasterizer:
         if batched:
             arr = (_abi.UguCamera * len(cams))(*cams)
             outs = (_abi.UguOutputs * len(cams))(*[outs_of(b) for b in buffers])
-            fn = self._lib.ugu_hip_render_batch if graph else self._lib.ugu_hip_render_batch_eager
-            self._check(fn(self._ctx, arr, len(cams), ctypes.byref(opt), outs,
-                           _abi.UGU_HIP_MEMORY_DEVICE))
+            self._check(self._render_views(arr, len(cams), ctypes.byref(opt), outs,
+                                           _abi.UGU_HIP_MEMORY_DEVICE, graph=graph))
         else:
             for c, b in zip(cams, buffers):
                 o = outs_of(b)
-                self._check(self._lib.ugu_hip_render(self._ctx, ctypes.byref(c), ctypes.byref(opt),
-                                                     ctypes.byref(o), _abi.UGU_HIP_MEMORY_DEVICE))
+                self._check(self._render_views(ctypes.byref(c), 1, ctypes.byref(opt),
+                                               ctypes.byref(o), _abi.UGU_HIP_MEMORY_DEVICE,
+                                               graph=False))
+
+
+class Raytracer(Rasterizer):
+    """ugu::Raytracer (src/renderer/cpu/raytracer.cc) on the gfx950 raytracer
+    kernel: PrepareMesh also builds the bounding volume hierarchy
+    (raytracer.cc:90-108), Render traces one ray per pixel from the camera
+    position (raytracer.cc:117-225). Every pixel gets the closest hit over all
+    faces (smallest float t, equal t -> lowest face id); the reference's
+    std::sort leaves equal t unordered. render_batch / render_device trace many
+    views in one launch; the rasterizer-only extensions (tiling, autotune,
+    G-buffer) are not available."""
+
+    def PrepareMesh(self):
+        """raytracer.cc:90-108: set_mesh is the rasterizer's (raytracer.cc:78-88)."""
+        if self.mesh_ is None:
+            LOGE("mesh has not been set\n")
+            return False
+        if len(self.mesh_.vertices) == 0 or len(self.mesh_.vertex_indices) == 0:
+            LOGE("mesh is empty\n")
+            return False
+        if not super().PrepareMesh():
+            return False
+        rc = self._lib.ugu_hip_prepare_raytracer(self._ctx)
+        if rc != 0:
+            LOGE(self.last_error())
+            self.mesh_initialized_ = False
+            return False
+        return True
+
+    def _render_views(self, cams, n, opt, outs, memory, graph=True):
+        return self._lib.ugu_hip_raytrace(self._ctx, cams, n, opt, outs, memory)
+
+    def _unsupported(self, *a, **k):
+        raise NotImplementedError("rasterizer-only extension")
+
+    set_tiling = get_tiling = autotune = render_gbuffer = _unsupported

@@ -310,7 +310,8 @@ class PinholeCamera:
         self.w2c = affine_inverse(self.c2w)
 
     def to_abi(self):
-        """The float reductions of rasterizer.cc:118-119 and camera.h:322."""
+        """The float reductions of rasterizer.cc:118-119 and camera.h:322
+        (c2w_t: the camera position, the raytracer's ray origin)."""
         from ._abi import UguCamera
         cam = UguCamera()
         cam.width, cam.height = self.width, self.height
@@ -322,8 +323,10 @@ class PinholeCamera:
         for i in range(9):
             cam.w2c_R[i] = float(w2c_R.reshape(-1)[i])
             cam.c2w_R[i] = float(c2w_R.reshape(-1)[i])
+        c2w_t = self.c2w[:3, 3].astype(np.float32)
         for i in range(3):
             cam.w2c_t[i] = float(w2c_t[i])
+            cam.c2w_t[i] = float(c2w_t[i])
         return cam
 
     def arrays(self):
