@@ -11,7 +11,8 @@ resident in HBM before timing starts.
 metric  = Mtri*views/s = faces * views / step time / 1e6
 extras  = Mpix/s, the per-view GPU loop (64 separate Render calls) and its
           ratio, the single 1080p view, the 32.5k-face UV sphere, per-kernel
-          HIP-event timings (roofline), and the CPU oracle on a bounded sample.
+          HIP-event timings (roofline), the raytracer on the same views, the
+          differentiable legs and the CPU oracle on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
@@ -215,6 +216,36 @@ def diff_benchmarks(torch, np, scenes, mesh, cams, args):
     return out
 
 
+def raytrace_benchmarks(torch, scenes, mesh, cams, cam1080, opt, args):
+    """Raytracer legs (ugu_hip_raytrace): the 64 ring views in one launch and
+    the single 1080p view of the same mesh, all five outputs in device
+    memory; plus the kernel time from HIP events and the host BVH build."""
+    from ugu_amd import Raytracer
+    rt = Raytracer(opt)
+    rt.set_mesh(mesh)
+    t0 = time.perf_counter()
+    assert rt.PrepareMesh()
+    prep = time.perf_counter() - t0
+    abi = [c.to_abi() for c in cams]
+    bufs = alloc_outputs(torch, cams)
+    px = sum(c.width * c.height for c in cams)
+    t64 = time_calls(torch, lambda: rt.render_device(abi, bufs), max(3, args.steps // 2), 1)
+    rt.enable_profiling(True)
+    rt.render_device(abi, bufs)
+    k64 = rt.timings()["raster_ms"]
+    rt.enable_profiling(False)
+    c1 = [cam1080.to_abi()]
+    b1 = alloc_outputs(torch, [cam1080])
+    t1 = time_calls(torch, lambda: rt.render_device(c1, b1), args.steps, args.warmup)
+    F = len(mesh.vertex_indices)
+    return {"raytrace": {
+        "views64_ms": t64 * 1e3, "views64_mrays_per_s": px / t64 / 1e6,
+        "views64_mtri_views_per_s": F * len(cams) / t64 / 1e6,
+        "views64_kernel_ms": k64,
+        "single_1080p_ms": t1 * 1e3, "single_1080p_mrays_per_s": 1920 * 1080 / t1 / 1e6,
+        "prepare_mesh_and_bvh_s": prep}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +259,8 @@ def main():
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable legs")
     ap.add_argument("--diff-only", action="store_true",
                     help="run only the differentiable legs (prints their JSON)")
+    ap.add_argument("--raytrace-only", action="store_true",
+                    help="run only the raytracer legs (prints their JSON)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (counter calibration)")
     args = ap.parse_args()
@@ -259,6 +292,9 @@ def main():
         return
     opt = RendererCpuOption(diffuse_color=DiffuseColor.kTexture,
                             interp=ColorInterpolation.kBilinear)
+    if args.raytrace_only:
+        print(json.dumps(raytrace_benchmarks(torch, scenes, mesh, cams, cam2, opt, args)))
+        return
     r = Rasterizer(opt)
     r.set_mesh(mesh)
     assert r.PrepareMesh()
@@ -335,6 +371,9 @@ def main():
                                       "mpix_per_s": 640 * 480 / t1 / 1e6}
         del r1
 
+    if not args.quick:
+        log("raytracer legs")
+        extra.update(raytrace_benchmarks(torch, scenes, mesh, cams, cam2, opt, args))
     if not args.quick and not args.no_diff:
         extra.update(diff_benchmarks(torch, np, scenes, mesh, cams, args))
 

@@ -1,4 +1,5 @@
-"""GPU: the C++ drop-in API (ugu::Rasterizer in ugu_amd/libugu_rasterizer.so)
+"""GPU: the C++ drop-in API (ugu::Rasterizer and ugu::Raytracer in
+ugu_amd/libugu_rasterizer.so)
 driven like UGU's own example by ugu_amd/cpp/test_rasterizer_api, compared
 with the oracle. Also checks the reference's error conventions through the
 C++ interface and that RenderBatch equals the per-view Render loop."""
@@ -132,3 +133,42 @@ def test_cpp_rasterizer_matches_oracle(gpu_available, tmp_path, variant):
                               got["face_id"])
         assert np.array_equal(np.fromfile(bpfx + "color.u8", np.uint8).reshape(h, w, 3),
                               got["color"])
+    # ugu::Raytracer through the same driver
+    for name in ("rt_prepare_without_mesh_fails", "rt_prepare_empty_mesh_fails",
+                 "rt_render_before_prepare_fails", "rt_prepare_ok", "rt_batch_equals_single"):
+        assert f"CHECK {name} ok" in res.stdout
+    from ugu_amd import Raytracer
+    rt = Raytracer(opt)
+    rt.set_mesh(mesh)
+    assert rt.PrepareMesh()
+    smallest = min(range(len(cams)), key=lambda i: cams[i].width * cams[i].height)
+    for i, c in enumerate(cams):
+        h, w = c.height, c.width
+        pfx = os.path.join(str(out), f"rt_view{i}_")
+        got = {
+            "face_id": np.fromfile(pfx + "face_id.i32", np.int32).reshape(h, w),
+            "mask": np.fromfile(pfx + "mask.u8", np.uint8).reshape(h, w),
+            "color": np.fromfile(pfx + "color.u8", np.uint8).reshape(h, w, 3),
+            "depth": np.fromfile(pfx + "depth.f32", np.float32).reshape(h, w),
+            "normal": np.fromfile(pfx + "normal.f32", np.float32).reshape(h, w, 3),
+        }
+        cam_f = np.fromfile(os.path.join(str(out), f"view{i}_camera.f32"), np.float32)
+        a2 = c.to_abi()
+        for k in range(9):
+            a2.w2c_R[k] = float(cam_f[k])
+        for k in range(3):
+            a2.w2c_t[k] = float(cam_f[9 + k])
+        py = rt.render_batch([a2])[0]
+        for k in got:
+            assert np.array_equal(got[k], py[k]), f"raytracer view {i} {k}: C++ != Python"
+        dw = np.fromfile(pfx + "depthw.u16", np.uint16).reshape(h, w)
+        assert np.array_equal(dw, O.depth_to_u16(got["depth"]))
+        if i == smallest:
+            oc = O.camera_from_abi(a2)
+            rc, ref, _ = O.raytrace(mesh, oc, opt, c2w_t=c.c2w[:3, 3])
+            assert rc == 0
+            assert np.array_equal(got["face_id"], ref["face_id"]), f"raytracer view {i} face_id"
+            assert np.array_equal(got["mask"], ref["mask"])
+            assert np.array_equal(got["color"], ref["color"])
+            np.testing.assert_allclose(got["depth"], ref["depth"], rtol=1e-5)
+            np.testing.assert_allclose(got["normal"], ref["normal"], rtol=1e-5, atol=1e-7)

@@ -11,6 +11,7 @@
 //            normals.f32, colors.f32, uv.f32, tex.u8, cams.f64 (ncam x
 //            (W H fx fy cx cy c2w[16 row-major]))
 
+#include <algorithm>
 #include <cstdio>
 #include <fstream>
 #include <memory>
@@ -19,6 +20,7 @@
 
 #include "ugu/log.h"
 #include "ugu/renderer/cpu/rasterizer.h"
+#include "ugu/renderer/cpu/raytracer.h"
 
 namespace {
 
@@ -207,6 +209,44 @@ int main(int argc, char** argv) {
     const std::string pfx = out + "/batch" + std::to_string(c) + "_";
     write_img(pfx + "face_id.i32", bfid[c]);
     write_img(pfx + "color.u8", bcol[c]);
+  }
+  // ---- ugu::Raytracer: same pattern (raytracer.cc:57-305)
+  {
+    ugu::set_log_level(ugu::LogLevel::kNone);
+    ugu::Raytracer rt(option);
+    check(!rt.PrepareMesh(), "rt_prepare_without_mesh_fails");
+    rt.set_mesh(ugu::Mesh::Create());
+    check(!rt.PrepareMesh(), "rt_prepare_empty_mesh_fails");
+    ugu::Image1i fid;
+    rt.set_camera(cameras[0]);
+    check(!rt.Render(nullptr, nullptr, nullptr, nullptr, &fid), "rt_render_before_prepare_fails");
+    ugu::set_log_level(ugu::LogLevel::kWarning);
+    rt.set_mesh(mesh);
+    check(rt.PrepareMesh(), "rt_prepare_ok");
+    std::vector<ugu::Image1i> single(ncam);
+    for (int c = 0; c < ncam; c++) {
+      rt.set_camera(cameras[c]);
+      ugu::Image3b color;
+      ugu::Image1f depth;
+      ugu::Image3f normal;
+      ugu::Image1b mask;
+      ugu::Image1w depthw;
+      check(rt.Render(&color, &depth, &normal, &mask, &single[c]), "rt_render");
+      check(rt.RenderDepthW(&depthw), "rt_render_depth_w");
+      const std::string pfx = out + "/rt_view" + std::to_string(c) + "_";
+      write_img(pfx + "color.u8", color);
+      write_img(pfx + "depth.f32", depth);
+      write_img(pfx + "normal.f32", normal);
+      write_img(pfx + "mask.u8", mask);
+      write_img(pfx + "face_id.i32", single[c]);
+      write_img(pfx + "depthw.u16", depthw);
+    }
+    std::vector<ugu::Image1i> bf;
+    check(rt.RenderBatch(all, nullptr, nullptr, nullptr, nullptr, &bf), "rt_render_batch");
+    bool same = bf.size() == size_t(ncam);
+    for (int c = 0; same && c < ncam; c++)
+      same = std::equal(bf[c].data, bf[c].data + bf[c].total(), single[c].data);
+    check(same, "rt_batch_equals_single");
   }
   std::printf("DONE failures=%d\n", g_fail);
   return g_fail == 0 ? 0 : 1;
