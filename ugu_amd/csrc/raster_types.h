@@ -87,7 +87,7 @@ struct BvhNode {
   float4 b_hi;
 };
 constexpr int32_t kBvhEmpty = INT32_MIN;
-constexpr int kBvhStack = 48;  // traversal stack entries; the builder keeps depth below
+constexpr int kBvhStack = 64;  // traversal stack entries; the builder keeps depth below
 
 struct DevBvh {
   const BvhNode* nodes;

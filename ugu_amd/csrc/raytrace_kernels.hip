@@ -1,11 +1,12 @@
 // HIP kernel of the gfx950 raytracer: Raytracer::Impl::Render
 // (src/renderer/cpu/raytracer.cc:117-225) for a batch of views.
 //
-// One wave per 8x8 pixel tile, one lane per pixel. Each lane walks the BVH
-// (built on the host, nodes hold both children's boxes) with a stack in LDS,
-// nearer child first, and tests the triangles of every leaf it reaches with
-// the reference's float64 Moller-Trumbore routine (src/util/geom_util.cc:60-115,
-// epsilon double(1e-8f) of bvh_naive.h:121, standard barycentric conversion).
+// One wave per 8x8 pixel tile, one lane per pixel. The wave walks the BVH
+// (built on the host, nodes hold both children's boxes) as one packet of 64
+// rays, nearer child first, and every lane tests the triangles of every leaf
+// reached with the reference's float64 Moller-Trumbore routine
+// (src/util/geom_util.cc:60-115, epsilon double(1e-8f) of bvh_naive.h:121,
+// standard barycentric conversion).
 // The closest hit is the smallest float t; equal t resolve to the lowest face
 // id. The box test is float32 with widened boxes and a relative slack on the
 // ray interval, so it only prunes boxes that cannot hold a hit the float64 test
@@ -77,44 +78,66 @@ UGU_DEV float box_entry(const float4& lo, const float4& hi, float ox, float oy, 
   return tn_lo <= tf_hi ? tn_lo : __builtin_huge_valf();
 }
 
-__global__ __launch_bounds__(64) void k_raytrace(const DevView* __restrict__ views,
-                                                 DevBvh bvh, DevMesh m, DevOptions o) {
-  __shared__ int32_t stack[kBvhStack * 64];
+// One wave traces the 64 rays of an 8x8 pixel tile through the hierarchy
+// together: the node and leaf indices are wave-uniform (fetched with scalar
+// loads), every lane tests its own ray against the node's two child boxes,
+// and a child is entered when any lane's ray may reach it (nearer child first
+// by a vote of the lanes). The stack of deferred children is one LDS word per
+// level for the whole wave. Each lane keeps its own closest hit, and the
+// closest hit does not depend on the visiting order, so the result is the
+// per-ray one; the traversal only decides which tests are made.
+// Four waves per workgroup, each on its own tile (no cross-wave sharing):
+// single-wave workgroups leave the CUs under-filled.
+constexpr int kRtWaves = 4;
+
+__global__ __launch_bounds__(64 * kRtWaves) void k_raytrace(const DevView* __restrict__ views,
+                                                            const BvhNode* __restrict__ nodes,
+                                                            const int2* __restrict__ leaves,
+                                                            const float4* __restrict__ tris,
+                                                            DevMesh m, DevOptions o) {
+  __shared__ int32_t stacks[kRtWaves][kBvhStack];
+  const int wave = threadIdx.x >> 6;
+  int32_t* stack = stacks[wave];
   const DevView& v = views[blockIdx.y];
   const int tiles_x = (v.W + 7) >> 3;
-  const int tile = blockIdx.x;
-  if (tile >= tiles_x * ((v.H + 7) >> 3)) return;
-  const int lane = threadIdx.x;
+  const int tile = blockIdx.x * kRtWaves + wave;
+  if (tile >= tiles_x * ((v.H + 7) >> 3)) return;  // uniform over the wave
+  const int lane = threadIdx.x & 63;
   const int x = (tile % tiles_x) * 8 + (lane & 7);
   const int y = (tile / tiles_x) * 8 + (lane >> 3);
-  if (x >= v.W || y >= v.H) return;
-  const int64_t pi = int64_t(y) * v.W + x;
+  const bool act = x < v.W && y < v.H;
 
   // ray_w / org_ray_w tables (camera.h:482-513)
   float r0, r1, r2;
-  ray_w_at(v, static_cast<float>(x), static_cast<float>(y), r0, r1, r2);
+  ray_w_at(v, static_cast<float>(act ? x : 0), static_cast<float>(act ? y : 0), r0, r1, r2);
   const float ox = v.c2w_t[0], oy = v.c2w_t[1], oz = v.c2w_t[2];
   const float ix = 1.0f / r0, iy = 1.0f / r1, iz = 1.0f / r2;
   const double dox = ox, doy = oy, doz = oz, ddx = r0, ddy = r1, ddz = r2;
+  const float kInf = __builtin_huge_valf();
 
-  float bt = __builtin_huge_valf(), bu = 0.0f, bv = 0.0f;
+  float bt = kInf, bu = 0.0f, bv = 0.0f;
   int bf = -1;
   int sp = 0;
   int ref = 0;  // the root is inner node 0
-  // Every inner reference points to a node of larger index and every node is
-  // pushed at most once, so the walk ends after at most (nodes + leaves) steps.
+  // Inner references point to larger node indices and every node is entered
+  // at most once, so the walk ends; the builder bounds the depth, and with it
+  // the stack, by kBvhStack.
   for (;;) {
     if (ref >= 0) {
-      const BvhNode nd = bvh.nodes[ref];
-      const float tmax = bt + bt * 9.5367431640625e-07f;
+      const BvhNode nd = nodes[ref];
       const int ra = __float_as_int(nd.a_lo.w), rb = __float_as_int(nd.b_lo.w);
-      const float ta = box_entry(nd.a_lo, nd.a_hi, ox, oy, oz, ix, iy, iz, tmax);
-      const float tb = rb == kBvhEmpty ? __builtin_huge_valf()
-                                       : box_entry(nd.b_lo, nd.b_hi, ox, oy, oz, ix, iy, iz, tmax);
-      const bool ha = ta != __builtin_huge_valf(), hb = tb != __builtin_huge_valf();
+      const float tmax = bt + bt * 9.5367431640625e-07f;
+      // both tests on every lane (no branch between the node's loads), then
+      // inactive lanes and the empty child are masked out
+      const float ea = box_entry(nd.a_lo, nd.a_hi, ox, oy, oz, ix, iy, iz, tmax);
+      const float eb = box_entry(nd.b_lo, nd.b_hi, ox, oy, oz, ix, iy, iz, tmax);
+      const float ta = act ? ea : kInf;
+      const float tb = (act && rb != kBvhEmpty) ? eb : kInf;
+      const bool ha = __any(ta != kInf), hb = __any(tb != kInf);
       if (ha && hb) {
-        const bool a_first = ta <= tb;
-        stack[sp * 64 + lane] = a_first ? rb : ra;
+        const int va = __popcll(__ballot(ta < tb)), vb = __popcll(__ballot(tb < ta));
+        const bool a_first = va >= vb;
+        stack[sp] = a_first ? rb : ra;
         sp++;
         ref = a_first ? ra : rb;
         continue;
@@ -124,12 +147,12 @@ __global__ __launch_bounds__(64) void k_raytrace(const DevView* __restrict__ vie
         continue;
       }
     } else {
-      const int2 lf = bvh.leaves[~ref];
+      const int2 lf = leaves[~ref];
       for (int k = 0; k < lf.y; k++) {
-        const float4* tr = bvh.tris + int64_t(lf.x + k) * 3;
+        const float4* tr = tris + int64_t(lf.x + k) * 3;
         const float4 p0 = tr[0], p1 = tr[1], p2 = tr[2];
         float t, u, w;
-        if (mt_hit(dox, doy, doz, ddx, ddy, ddz, p0, p1, p2, t, u, w)) {
+        if (act && mt_hit(dox, doy, doz, ddx, ddy, ddz, p0, p1, p2, t, u, w)) {
           const int f = __float_as_int(p0.w);
           if (t < bt || (t == bt && f < bf)) {
             bt = t;
@@ -142,8 +165,10 @@ __global__ __launch_bounds__(64) void k_raytrace(const DevView* __restrict__ vie
     }
     if (sp == 0) break;
     sp--;
-    ref = stack[sp * 64 + lane];
+    ref = __builtin_amdgcn_readfirstlane(stack[sp]);
   }
+  if (!act) return;
+  const int64_t pi = int64_t(y) * v.W + x;
 
   bool hit = bf >= 0;
   // back-face culling: face normal . ray_w > 0 -> background (raytracer.cc:166-172)
@@ -207,7 +232,9 @@ __global__ __launch_bounds__(64) void k_raytrace(const DevView* __restrict__ vie
 int launch_raytrace(hipStream_t st, const DevView* views, int n_views, int max_tiles,
                     const DevBvh& bvh, const DevMesh& m, const DevOptions& o) {
   if (n_views < 1 || max_tiles < 1) return 0;
-  hipLaunchKernelGGL(k_raytrace, dim3(max_tiles, n_views), dim3(64), 0, st, views, bvh, m, o);
+  const int blocks = (max_tiles + kRtWaves - 1) / kRtWaves;
+  hipLaunchKernelGGL(k_raytrace, dim3(blocks, n_views), dim3(64 * kRtWaves), 0, st, views,
+                     bvh.nodes, bvh.leaves, bvh.tris, m, o);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
