@@ -172,4 +172,13 @@ UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, float u, f
   }
 }
 
+UGU_DEV int xcd_remap(int b, int nb) {
+  // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous run of tiles so neighbouring tiles share its L2.
+  const int q = nb >> 3, r = nb & 7;
+  const int xcd = b & 7, idx = b >> 3;
+  const int start = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+  return start + idx;
+}
+
 }  // namespace ugu_hip

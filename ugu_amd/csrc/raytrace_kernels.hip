@@ -86,22 +86,37 @@ UGU_DEV float box_entry(const float4& lo, const float4& hi, float ox, float oy, 
 // level for the whole wave. Each lane keeps its own closest hit, and the
 // closest hit does not depend on the visiting order, so the result is the
 // per-ray one; the traversal only decides which tests are made.
-// Four waves per workgroup, each on its own tile (no cross-wave sharing):
-// single-wave workgroups leave the CUs under-filled.
+// Four waves per workgroup, each on its own tile (no cross-wave sharing).
 constexpr int kRtWaves = 4;
 
 __global__ __launch_bounds__(64 * kRtWaves) void k_raytrace(const DevView* __restrict__ views,
                                                             const BvhNode* __restrict__ nodes,
                                                             const int2* __restrict__ leaves,
                                                             const float4* __restrict__ tris,
-                                                            DevMesh m, DevOptions o) {
+                                                            int n_views, int total_tiles,
+                                                            int chunk, DevMesh m,
+                                                            DevOptions o) {
   __shared__ int32_t stacks[kRtWaves][kBvhStack];
   const int wave = threadIdx.x >> 6;
   int32_t* stack = stacks[wave];
-  const DevView& v = views[blockIdx.y];
+  // Tiles of all views are numbered consecutively (views[i].tile_base).
+  // Workgroups go round-robin over the 8 XCDs; each XCD here takes runs of
+  // `chunk` consecutive workgroups (neighbouring tiles share its L2), and the
+  // runs alternate over the XCDs. The grid is a multiple of 8 chunk
+  // workgroups, which makes the mapping a permutation.
+  const int i = blockIdx.x;
+  const int k = i >> 3;
+  const int lb = (((k / chunk) << 3) + (i & 7)) * chunk + (k % chunk);
+  const int g = lb * kRtWaves + wave;
+  if (g >= total_tiles) return;  // uniform over the wave
+  int lo = 0, hi = n_views - 1;  // last view with tile_base <= g
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (views[mid].tile_base <= g) lo = mid; else hi = mid - 1;
+  }
+  const DevView& v = views[lo];
   const int tiles_x = (v.W + 7) >> 3;
-  const int tile = blockIdx.x * kRtWaves + wave;
-  if (tile >= tiles_x * ((v.H + 7) >> 3)) return;  // uniform over the wave
+  const int tile = g - v.tile_base;
   const int lane = threadIdx.x & 63;
   const int x = (tile % tiles_x) * 8 + (lane & 7);
   const int y = (tile / tiles_x) * 8 + (lane >> 3);
@@ -229,12 +244,22 @@ __global__ __launch_bounds__(64 * kRtWaves) void k_raytrace(const DevView* __res
   }
 }
 
-int launch_raytrace(hipStream_t st, const DevView* views, int n_views, int max_tiles,
+// views[i].tile_base = first 8x8 tile of view i in the concatenated tile list
+int launch_raytrace(hipStream_t st, const DevView* views, int n_views, int total_tiles,
                     const DevBvh& bvh, const DevMesh& m, const DevOptions& o) {
-  if (n_views < 1 || max_tiles < 1) return 0;
-  const int blocks = (max_tiles + kRtWaves - 1) / kRtWaves;
-  hipLaunchKernelGGL(k_raytrace, dim3(blocks, n_views), dim3(64 * kRtWaves), 0, st, views,
-                     bvh.nodes, bvh.leaves, bvh.tris, m, o);
+  if (n_views < 1 || total_tiles < 1) return 0;
+  // One view: each XCD takes one contiguous band of tiles (measured 0.61 vs
+  // 0.75 ms at 1080p). Many views: runs of 16 workgroups alternate over the
+  // XCDs (views differ in cost per tile; one band per XCD would leave some
+  // XCDs with the expensive views: 13.8 vs 10.5 ms on the 64 ring views).
+  const int64_t need = (int64_t(total_tiles) + kRtWaves - 1) / kRtWaves;
+  const int64_t chunk = n_views == 1 ? (need + 7) / 8 : 16;
+  const int64_t per = 8 * chunk;
+  const int64_t blocks64 = (need + per - 1) / per * per;
+  if (blocks64 > INT32_MAX) return -1;
+  const int blocks = static_cast<int>(blocks64);
+  hipLaunchKernelGGL(k_raytrace, dim3(blocks), dim3(64 * kRtWaves), 0, st, views, bvh.nodes,
+                     bvh.leaves, bvh.tris, n_views, total_tiles, static_cast<int>(chunk), m, o);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

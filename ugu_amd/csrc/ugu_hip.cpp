@@ -35,7 +35,7 @@ int launch_backward(hipStream_t st, const DevView* views, int n_views,
                     int max_pixels, int64_t total_pixels, const DevMesh& mesh,
                     const DevGrads& g, int4* items, int32_t* scratch);
 int launch_depth_to_u16(hipStream_t st, const float* d, int64_t n, uint16_t* out);
-int launch_raytrace(hipStream_t st, const DevView* views, int n_views, int max_tiles,
+int launch_raytrace(hipStream_t st, const DevView* views, int n_views, int total_tiles,
                     const DevBvh& bvh, const DevMesh& m, const DevOptions& o);
 }  // namespace ugu_hip
 
@@ -655,12 +655,14 @@ int raytrace_common(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptio
 
   // output staging for host memory, view descriptors
   std::vector<int64_t> stage_off(size_t(n) * kPlanes, -1);
+  std::vector<int32_t> rt_tile_base(size_t(n) + 1, 0);
   int64_t stage_bytes = 0;
-  int max_tiles = 0;
   for (int i = 0; i < n; i++) {
     const int64_t px = int64_t(cams[i].width) * cams[i].height;
     const int64_t tiles = int64_t((cams[i].width + 7) / 8) * ((cams[i].height + 7) / 8);
-    max_tiles = std::max<int>(max_tiles, static_cast<int>(tiles));
+    const int64_t next = int64_t(rt_tile_base[size_t(i)]) + tiles;
+    if (next > INT32_MAX / 2) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "too many tiles");
+    rt_tile_base[size_t(i) + 1] = static_cast<int32_t>(next);
     if (memory == UGU_HIP_MEMORY_HOST) {
       void* req[kPlanes];
       plane_ptrs(outs[i], nullptr, req);
@@ -697,6 +699,7 @@ int raytrace_common(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptio
     std::memcpy(v.w2c_t, cams[i].w2c_t, sizeof(v.w2c_t));
     std::memcpy(v.c2w_R, cams[i].c2w_R, sizeof(v.c2w_R));
     std::memcpy(v.c2w_t, cams[i].c2w_t, sizeof(v.c2w_t));
+    v.tile_base = rt_tile_base[size_t(i)];
     void* p[kPlanes];
     plane_ptrs(outs[i], nullptr, p);
     if (memory == UGU_HIP_MEMORY_HOST) {
@@ -716,7 +719,7 @@ int raytrace_common(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptio
   const DevOptions dopt = dev_options(*o);
   c->tm = UguTimings{};
   if (c->profile) (void)hipEventRecord(c->ev[0], c->stream);
-  if (launch_raytrace(c->stream, c->views.as<DevView>(), n, max_tiles, bvh, dm, dopt))
+  if (launch_raytrace(c->stream, c->views.as<DevView>(), n, rt_tile_base[size_t(n)], bvh, dm, dopt))
     return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
   if (c->profile) (void)hipEventRecord(c->ev[1], c->stream);
   if (memory == UGU_HIP_MEMORY_HOST) {
