@@ -20,6 +20,7 @@ struct Vec_ {
 };
 
 using Vec3b = Vec_<uint8_t, 3>;
+using Vec4b = Vec_<uint8_t, 4>;
 using Vec3f = Vec_<float, 3>;
 
 template <typename T>
@@ -85,6 +86,7 @@ using Image1i = Image<int32_t>;   // face id
 using Image1f = Image<float>;     // depth
 using Image3f = Image<Vec3f>;     // normal, XYZ
 using Image3b = Image<Vec3b>;     // colour, RGB
+using Image4b = Image<Vec4b>;     // colour with alpha, RGBA
 
 // ugu::Init (image.h:964-985): reallocate when the size differs, then fill.
 template <typename T>
