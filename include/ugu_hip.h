@@ -27,6 +27,9 @@
  *   ugu_hip_prepare_raytracer / ugu_hip_raytrace
  *       Raytracer::PrepareMesh / Raytracer::Render
  *       (src/renderer/cpu/raytracer.cc:88-108, :117-225)
+ *   ugu_hip_visibility_test
+ *       VisibilityTester::TestVertex (src/texturing/visibility_tester.cc:390-530)
+ *       for a batch of vertices against one keyframe
  *   ugu_hip_depth_to_u16
  *       the ConvertTo step of Rasterizer::RenderW (src/renderer/cpu/rasterizer.cc:329-344,
  *       include/ugu/image_proc.h:453-520)
@@ -274,6 +277,59 @@ int ugu_hip_get_timings(const ugu_hip_ctx* ctx, UguTimings* timings);
  * truncating, as ConvertTo (image_proc.h:453-520). memory as above. */
 int ugu_hip_depth_to_u16(ugu_hip_ctx* ctx, const float* depth, int64_t n,
                          uint16_t* out, int memory);
+
+/* ---- visibility test (texturing consumer of the raytracer) -------------
+ * VisibilityTesterOption (include/ugu/texturing/visibility_tester.h:38-57),
+ * the fields TestVertex reads. interp: 0 kNn, 1 kBilinear. */
+typedef struct {
+  float backface_culling_angle_rad_th;
+  int32_t use_mask;
+  int32_t use_depth;
+  int32_t interp;
+  float max_viewing_angle;
+  float max_distance_from_camera_to_vertex;
+  float max_depth_difference;
+  float eps;
+} UguVisibilityOptions;
+
+/* A keyframe (visibility_tester.h:229-246): the camera (w2c_R / w2c_t = the
+ * float cast of w2c()'s linear part and translation; c2w_R, c2w_t, intrinsics
+ * as for Render), its colour image (width x height x 3 RGB8, the size the
+ * projection is bounded by) and optional depth (float) and mask (u8) images
+ * of the same size. Host pointers. */
+typedef struct {
+  UguCamera camera;
+  int32_t width, height;
+  const uint8_t* color;
+  const float* depth;
+  const uint8_t* mask;
+} UguKeyframe;
+
+/* TestVertex's result for one item (VertexInfoPerKeyframe, :61-77):
+ * visible = TestVertex's return value; the other fields as TestVertex sets
+ * them when visible (projected position, viewing angle in radians, distance
+ * = the ray parameter of the closest hit or FLT_MAX, colour as floats,
+ * intensity = Color2Gray(colour)). */
+typedef struct {
+  int32_t visible;
+  float projected[2];
+  float viewing_angle;
+  float distance;
+  float color[3];
+  float intensity;
+} UguVertexVisibility;
+
+/* TestVertex for n items against one keyframe, on the hierarchy built by
+ * ugu_hip_prepare_raytracer over the context's mesh (the tester's vertices
+ * and faces). Item i is the point item_pos[i] (= vertex + offset; the
+ * facewise test moves each corner 1e-3 towards its face centre), tested with
+ * the vertex position item_vertex[i] (occlusion distance) and the vertex
+ * normal item_normal[i]. Host arrays of n x 3 floats; out has n entries.
+ * Items are independent: the caller assembles VisibilityInfo from them. */
+int ugu_hip_visibility_test(ugu_hip_ctx* ctx, const UguKeyframe* keyframe,
+                            const UguVisibilityOptions* options, int n,
+                            const float* item_pos, const float* item_vertex,
+                            const float* item_normal, UguVertexVisibility* out);
 
 /* ---- differentiable rendering (no reference counterpart) ----------------
  * Uses the prepared mesh's topology (faces, uv, uv_indices) with parameters

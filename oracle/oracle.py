@@ -246,6 +246,41 @@ def render_gbuffer(mesh, cam, options, c2w_t=None):
     return rc, out, ms.value
 
 
+class OracleVisOptions(ctypes.Structure):
+    _fields_ = [("backface_culling_angle_rad_th", _f), ("use_mask", _i), ("use_depth", _i),
+                ("interp", _i), ("max_viewing_angle", _f),
+                ("max_distance_from_camera_to_vertex", _f), ("max_depth_difference", _f),
+                ("eps", _f)]
+
+
+VIS_DTYPE = np.dtype([("visible", np.int32), ("px", np.float32), ("py", np.float32),
+                      ("angle", np.float32), ("dist", np.float32), ("c0", np.float32),
+                      ("c1", np.float32), ("c2", np.float32), ("intensity", np.float32)])
+
+
+def visibility(mesh, cam, c2w_t, color, depth, mask, opt, item_pos, item_vertex, item_normal):
+    """oracle_visibility: TestVertex for every item (ugu_oracle.cc); returns a
+    structured array (VIS_DTYPE). `opt` is an OracleVisOptions."""
+    L = lib()
+    L.oracle_visibility.restype = ctypes.c_int
+    hold = _MeshHold(mesh)
+    h, w = color.shape[:2]
+    color = np.ascontiguousarray(color, np.uint8)
+    depth = None if depth is None else np.ascontiguousarray(depth, np.float32)
+    mask = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+    pos = np.ascontiguousarray(item_pos, np.float32)
+    vert = np.ascontiguousarray(item_vertex, np.float32)
+    nrm = np.ascontiguousarray(item_normal, np.float32)
+    ct = np.ascontiguousarray(c2w_t, np.float32)
+    out = np.zeros(len(pos), VIS_DTYPE)
+    rc = L.oracle_visibility(ctypes.byref(hold.m), ctypes.byref(cam), _p(ct), ctypes.c_int32(w),
+                             ctypes.c_int32(h), _p(color), _p(depth), _p(mask), ctypes.byref(opt),
+                             ctypes.c_int32(len(pos)), _p(pos), _p(vert), _p(nrm),
+                             out.ctypes.data_as(_P))
+    assert rc == 0
+    return out
+
+
 def depth_to_u16(depth):
     d = np.ascontiguousarray(depth, np.float32)
     out = np.empty(d.shape, np.uint16)

@@ -771,6 +771,122 @@ int oracle_raytrace(const OracleMesh* mesh, const OracleCamera* cam, const float
   return kOk;
 }
 
+// ---------------------------------------------------------------------------
+// VisibilityTester::TestVertex (src/texturing/visibility_tester.cc:390-530)
+// for n items against one keyframe; the closest hit of the vertex ray over all
+// faces (smallest float t) as in oracle_raytrace.
+
+struct OracleVisOptions {
+  float backface_culling_angle_rad_th;
+  int32_t use_mask, use_depth, interp;
+  float max_viewing_angle, max_distance_from_camera_to_vertex, max_depth_difference, eps;
+};
+
+struct OracleVisItem {
+  int32_t visible;
+  float px, py, angle, dist, c0, c1, c2, intensity;
+};
+
+int oracle_visibility(const OracleMesh* mesh, const OracleCamera* cam, const float* c2w_t,
+                      int32_t W, int32_t H, const uint8_t* color, const float* depth,
+                      const uint8_t* mask, const OracleVisOptions* opt, int32_t n,
+                      const float* item_pos, const float* item_vertex, const float* item_normal,
+                      OracleVisItem* out) {
+  const OracleMesh& m = *mesh;
+  const OracleVisOptions& o = *opt;
+  const double eps_mt = static_cast<double>(1e-8f);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int32_t i = 0; i < n; i++) {
+    OracleVisItem r{0, 0.0f, 0.0f, 999.9f, -999.9f, 0.0f, 0.0f, 0.0f, -1.0f};
+    out[i] = r;
+    const V3 wp = load3(item_pos, i);
+    const V3 cp = matvec(cam->w2c_R, wp);
+    const float qx = cp[0] + cam->w2c_t[0], qy = cp[1] + cam->w2c_t[1],
+                qz = cp[2] + cam->w2c_t[2];
+    const float ix = cam->fx / qz * qx + cam->cx;
+    const float iy = cam->fy / qz * qy + cam->cy;
+    if (qz < 0 || ix < 0 || static_cast<float>(W - 1) < ix || iy < 0 ||
+        static_cast<float>(H - 1) < iy)
+      continue;
+    if (!(ix >= 0.0f && iy >= 0.0f)) continue;  // NaN projection
+    const int nx = static_cast<int>(MUT(22) ? std::floor(ix) : std::round(ix));
+    const int ny = static_cast<int>(MUT(22) ? std::floor(iy) : std::round(iy));
+    const size_t pi = static_cast<size_t>(ny) * W + nx;
+    if (o.use_mask && mask[pi] != 255) continue;
+    if (o.use_depth && depth[pi] < std::numeric_limits<float>::epsilon()) continue;
+    const V3 ray = ray_w_at(*cam, ix, iy);
+    const V3 nrm = load3(item_normal, i);
+    const float dot = MUT(23) ? dot3(nrm, ray) : -dot3(nrm, ray);
+    const float angle = std::acos(dot);
+    if (o.backface_culling_angle_rad_th < angle) continue;
+    const double od[3] = {c2w_t[0], c2w_t[1], c2w_t[2]};
+    const double dd[3] = {ray[0], ray[1], ray[2]};
+    float bt = std::numeric_limits<float>::infinity();
+    for (int32_t f = 0; f < m.num_faces; f++) {
+      const int32_t* fi = m.faces + static_cast<int64_t>(f) * 3;
+      double p[3][3];
+      bool finite = true;
+      for (int k = 0; k < 3; k++)
+        for (int c = 0; c < 3; c++) {
+          p[k][c] = m.vertices[static_cast<int64_t>(fi[k]) * 3 + c];
+          finite = finite && std::isfinite(p[k][c]);
+        }
+      if (!finite) continue;
+      float t, u, v;
+      if (moller_trumbore(od, dd, p[0], p[1], p[2], eps_mt, &t, &u, &v) && t < bt) bt = t;
+    }
+    const V3 vert = load3(item_vertex, i);
+    V3 hit = vert;
+    const bool has_hit = bt < std::numeric_limits<float>::infinity();
+    const float distance = has_hit ? bt : std::numeric_limits<float>::max();
+    if (has_hit) {
+      for (int k = 0; k < 3; k++) hit[k] = c2w_t[k] + ray[k] * bt;
+      const float dx = hit[0] - vert[0], dy = hit[1] - vert[1], dz = hit[2] - vert[2];
+      if (!MUT(21) && std::sqrt(sum3(dx * dx, dy * dy, dz * dz)) > o.eps) continue;
+    }
+    if (o.use_depth) {
+      const V3 hc = matvec(cam->w2c_R, hit);
+      const float diff = std::fabs((hc[2] + cam->w2c_t[2]) - depth[pi]);
+      if (o.max_depth_difference < diff) continue;
+    }
+    r.px = ix;
+    r.py = iy;
+    r.angle = angle;
+    r.dist = distance;
+    out[i] = r;
+    if (distance > o.max_distance_from_camera_to_vertex || angle > o.max_viewing_angle) continue;
+    float c[3];
+    if (o.interp == 0) {
+      for (int k = 0; k < 3; k++) c[k] = color[pi * 3 + k];
+    } else {
+      int x0 = static_cast<int>(std::floor(ix)), y0 = static_cast<int>(std::floor(iy));
+      int x1 = x0 + 1, y1 = y0 + 1;
+      if (x0 < 0) x0 = 0;
+      if (y0 < 0) y0 = 0;
+      if (W <= x1) x1 = W - 1;
+      if (H <= y1) y1 = H - 1;
+      float lu = ix - x0, lv = iy - y0;
+      if (MUT(24)) std::swap(lu, lv);
+      const uint8_t* c00 = color + (static_cast<size_t>(y0) * W + x0) * 3;
+      const uint8_t* c01 = color + (static_cast<size_t>(y0) * W + x1) * 3;
+      const uint8_t* c10 = color + (static_cast<size_t>(y1) * W + x0) * 3;
+      const uint8_t* c11 = color + (static_cast<size_t>(y1) * W + x1) * 3;
+      for (int k = 0; k < 3; k++) {
+        const float f = (1.0f - lu) * (1.0f - lv) * c00[k] + lu * (1.0f - lv) * c01[k] +
+                        (1.0f - lu) * lv * c10[k] + lu * lv * c11[k];
+        c[k] = f2u8_x86(f);
+      }
+    }
+    r.c0 = c[0];
+    r.c1 = c[1];
+    r.c2 = c[2];
+    r.intensity = static_cast<float>(0.2989 * c[0] + 0.5870 * c[1] + 0.1140 * c[2]);
+    r.visible = 1;
+    out[i] = r;
+  }
+  return kOk;
+}
+
 // RenderW / RenderDepthW (rasterizer.cc:329-348) depth conversion:
 // ConvertTo(f_depth, depth) with scale 1 -> static_cast<uint16_t>(1.0 * (double)d)
 void oracle_depth_to_u16(const float* depth, int64_t n, uint16_t* out) {

@@ -590,7 +590,129 @@ def pin_raytrace_sphere():
     return True, ""
 
 
+
+
+
+# ---------------------------------------------------------------------------
+# visibility test (oracle_visibility)
+
+
+def visibility_scene(seed=0):
+    """Unit uv sphere (analytic vertex normals) with a square occluder at
+    z = -1.6 covering x in [-0.3, 0.9], y in [-0.35, 0.25]; camera at
+    (0, 0, -3) looking down +z; a random colour image; mask all 255; depth
+    from the oracle rasterizer (sphere and occluder)."""
+    sph = scenes.make_uv_sphere(40, 40)
+    n = sph.vertices / np.linalg.norm(sph.vertices, axis=1, keepdims=True)
+    sv = sph.vertices.astype(np.float32)
+    q = np.array([[-0.3, -0.35, -1.6], [0.9, -0.35, -1.6], [0.9, 0.25, -1.6], [-0.3, 0.25, -1.6]],
+                 np.float32)
+    verts = np.concatenate([sv, q], 0)
+    faces = np.concatenate([sph.vertex_indices, len(sv) + np.array([[0, 2, 1], [0, 3, 2]])], 0)
+    m = Mesh(verts, faces.astype(np.int32))
+    m.CalcFaceNormal()
+    normals = np.concatenate([n, np.tile([0.0, 0.0, -1.0], (4, 1))], 0).astype(np.float32)
+    m.set_normals(normals)
+    m.set_normal_indices(m.vertex_indices)
+    m.set_vertex_colors(np.full((len(verts), 3), 128.0, np.float32))
+    c2w = np.eye(4)
+    c2w[2, 3] = -3.0
+    cam = PinholeCamera(96, 80, c2w, (47.5, 39.5), (70.0, 70.0))
+    rng = np.random.default_rng(seed)
+    color = rng.integers(0, 256, (80, 96, 3), dtype=np.uint8)
+    out = render(m, cam, RendererCpuOption(diffuse_color=DiffuseColor.kVertex,
+                                           backface_culling=False))
+    return m, normals, cam, color, out["depth"], np.full((80, 96), 255, np.uint8)
+
+
+def vis_options(**kw):
+    o = O.OracleVisOptions()
+    o.backface_culling_angle_rad_th = float(np.float32(np.float32(90.0) * np.float32(math.pi) /
+                                                       np.float32(180.0)))
+    o.use_mask, o.use_depth, o.interp = 1, 1, 1
+    o.max_viewing_angle = o.backface_culling_angle_rad_th
+    o.max_distance_from_camera_to_vertex = float(np.finfo(np.float32).max)
+    o.max_depth_difference = float(np.finfo(np.float32).max)
+    o.eps = 0.1
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def visibility(m, normals, cam, color, depth, mask, opt):
+    a = cam.to_abi()
+    oc = O.camera_from_abi(a)
+    lin = cam.w2c[:3, :3].astype(np.float32)
+    for i in range(9):
+        oc.w2c_R[i] = float(lin.reshape(-1)[i])
+    return O.visibility(m, oc, cam.c2w[:3, 3], color, depth, mask, opt, m.vertices, m.vertices,
+                        normals)
+
+
+def pin_visibility():
+    """Projection, viewing angle, distance, occlusion by the square, colour
+    sampling (NN = rounded pixel, bilinear = numpy), intensity weights."""
+    m, normals, cam, color, depth, mask = visibility_scene()
+    nv = len(m.vertices) - 4
+    V = m.vertices[:nv].astype(np.float64)
+    C = np.array([0.0, 0.0, -3.0])
+    d = V - C
+    dist = np.linalg.norm(d, axis=1)
+    ray = d / dist[:, None]
+    cosang = -(ray * normals[:nv].astype(np.float64)).sum(1)
+    # occluded: the segment camera -> vertex crosses z = -1.6 inside the square
+    s = (-1.6 - C[2]) / d[:, 2]
+    xy = C[:2] + s[:, None] * d[:, :2]
+    occ = (s > 0) & (s < 1) & (xy[:, 0] > -0.3) & (xy[:, 0] < 0.9) & (xy[:, 1] > -0.35) & \
+        (xy[:, 1] < 0.25)
+    near_edge = (np.abs(xy[:, 0] + 0.3) < 0.02) | (np.abs(xy[:, 0] - 0.9) < 0.02) | \
+        (np.abs(xy[:, 1] + 0.35) < 0.02) | (np.abs(xy[:, 1] - 0.25) < 0.02)
+    for interp in (0, 1):
+        r = visibility(m, normals, cam, color, depth, mask, vis_options(interp=interp))[:nv]
+        vis = r["visible"] == 1
+        clear_front = (cosang > 0.2) & ~occ & ~near_edge
+        if not vis[clear_front].all():
+            return False, "a clearly visible vertex was rejected"
+        if vis[occ & ~near_edge].any():
+            return False, "a vertex behind the occluder was reported visible"
+        if vis[cosang < -0.05].any():
+            return False, "a back-facing vertex was reported visible"
+        fx = 70.0 * d[:, 0] / d[:, 2] + 47.5
+        fy = 70.0 * d[:, 1] / d[:, 2] + 39.5
+        if not (np.allclose(r["px"][vis], fx[vis], atol=1e-3) and
+                np.allclose(r["py"][vis], fy[vis], atol=1e-3)):
+            return False, "projected position"
+        if not np.allclose(np.cos(r["angle"][vis]), cosang[vis], atol=2e-5):
+            return False, "viewing angle is not acos(-n . ray)"
+        # a ray through a mesh vertex can slip between the faces around it
+        # (grazing rays then hit nothing): distance FLT_MAX, the vertex is
+        # kept as its own hit (visibility_tester.cc:478-485)
+        hit = vis & (r["dist"] < 1e30)
+        if hit.sum() < 0.8 * vis.sum() or \
+                not np.allclose(r["dist"][hit], dist[hit], rtol=2e-3):
+            return False, "distance is not the ray length to the surface"
+        px, py = r["px"][vis].astype(np.float64), r["py"][vis].astype(np.float64)
+        col = np.stack([r["c0"], r["c1"], r["c2"]], -1)[vis]
+        if interp == 0:
+            exp = color[np.floor(py + 0.5).astype(int), np.floor(px + 0.5).astype(int)]
+            if not np.array_equal(col, exp.astype(np.float32)):
+                return False, "nearest colour is not the rounded pixel"
+        else:
+            x0, y0 = np.floor(px).astype(int), np.floor(py).astype(int)
+            x1, y1 = np.minimum(x0 + 1, 95), np.minimum(y0 + 1, 79)
+            lu, lv = (px - x0)[:, None], (py - y0)[:, None]
+            exp = (1 - lu) * (1 - lv) * color[y0, x0] + lu * (1 - lv) * color[y0, x1] + \
+                (1 - lu) * lv * color[y1, x0] + lu * lv * color[y1, x1]
+            if np.abs(col - np.floor(exp)).max() > 1:
+                return False, "bilinear colour"
+        inten = 0.2989 * col[:, 0] + 0.5870 * col[:, 1] + 0.1140 * col[:, 2]
+        if not np.allclose(r["intensity"][vis], inten, atol=1e-3):
+            return False, "intensity weights"
+    return True, ""
+
+
 ALL_PINS = [pin_brute_force, pin_ties_lowest_id, pin_face_zero_not_shaded,
             pin_perspective_correct, pin_texture_sampling, pin_sphere_closed_forms,
             pin_normals_rotated_camera, pin_lambertian, pin_oren_nayar, pin_depth_u16,
-            pin_gbuffer, pin_raytrace_brute_force, pin_raytrace_plane, pin_raytrace_sphere]
+            pin_gbuffer, pin_raytrace_brute_force, pin_raytrace_plane, pin_raytrace_sphere,
+            pin_visibility]

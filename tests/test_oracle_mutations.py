@@ -37,6 +37,10 @@ MUTATIONS = {
     18: "raytracer culling sign flipped",
     19: "raytracer depth = ray parameter t instead of camera z",
     20: "raytracer ray origin ignored",
+    21: "visibility: occlusion check skipped",
+    22: "visibility: nearest pixel by floor instead of round",
+    23: "visibility: viewing angle from +n . ray",
+    24: "visibility: bilinear weights swapped",
 }
 
 

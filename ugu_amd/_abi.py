@@ -8,6 +8,8 @@ fallback for the rendering path.
 import ctypes
 import os
 
+import numpy as np
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libugu_hip.so")
 
@@ -85,6 +87,24 @@ class UguGBuffer(ctypes.Structure):
     ]
 
 
+class UguVisibilityOptions(ctypes.Structure):
+    _fields_ = [("backface_culling_angle_rad_th", _f), ("use_mask", _i), ("use_depth", _i),
+                ("interp", _i), ("max_viewing_angle", _f),
+                ("max_distance_from_camera_to_vertex", _f), ("max_depth_difference", _f),
+                ("eps", _f)]
+
+
+class UguKeyframe(ctypes.Structure):
+    _fields_ = [("camera", UguCamera), ("width", _i), ("height", _i),
+                ("color", ctypes.c_void_p), ("depth", ctypes.c_void_p), ("mask", ctypes.c_void_p)]
+
+
+# UguVertexVisibility as a numpy record
+VIS_DTYPE = np.dtype([("visible", np.int32), ("px", np.float32), ("py", np.float32),
+                      ("angle", np.float32), ("dist", np.float32), ("c0", np.float32),
+                      ("c1", np.float32), ("c2", np.float32), ("intensity", np.float32)])
+
+
 class UguTiling(ctypes.Structure):
     _fields_ = [("tile_w", _i), ("tile_h", _i), ("bin_capacity", _i), ("persistent", _i)]
 
@@ -129,6 +149,9 @@ SIGNATURES = {
     "ugu_hip_raytrace": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,
                                         ctypes.POINTER(UguOptions), ctypes.POINTER(UguOutputs),
                                         ctypes.c_int]),
+    "ugu_hip_visibility_test": (ctypes.c_int, [_P, ctypes.POINTER(UguKeyframe),
+                                               ctypes.POINTER(UguVisibilityOptions), ctypes.c_int,
+                                               _P, _P, _P, _P]),
     "ugu_hip_set_tiling": (ctypes.c_int, [_P, ctypes.POINTER(UguTiling)]),
     "ugu_hip_get_tiling": (ctypes.c_int, [_P, ctypes.POINTER(UguTiling)]),
     "ugu_hip_autotune": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,

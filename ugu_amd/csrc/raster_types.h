@@ -95,6 +95,13 @@ struct DevBvh {
   const float4* tris;   // 3 per record: (v0, face id bits) (v1, 0) (v2, 0)
 };
 
+// Visibility test options (UguVisibilityOptions)
+struct DevVisOpt {
+  float th;  // backface_culling_angle_rad_th
+  int32_t use_mask, use_depth, interp;
+  float max_angle, max_dist, max_ddiff, eps;
+};
+
 struct DevBins {
   int32_t* counts;   // per global tile: front / back segment counters, minimum
                      // skip depth of the back segment (float bits), pad

@@ -244,6 +244,168 @@ __global__ __launch_bounds__(64 * kRtWaves) void k_raytrace(const DevView* __res
   }
 }
 
+// ---------------------------------------------------------------------------
+// Visibility test: VisibilityTester::TestVertex (visibility_tester.cc:390-530)
+// for one item per lane. Items are scattered over the image, so every lane
+// walks the hierarchy on its own (stack of kBvhStack entries per lane in LDS).
+
+struct DevVisItem {
+  int32_t visible;
+  float px, py, angle, dist, c0, c1, c2, intensity;
+};
+
+// closest float t over all faces (+inf: no hit), per-lane traversal
+UGU_DEV float closest_t(const BvhNode* __restrict__ nodes, const int2* __restrict__ leaves,
+                        const float4* __restrict__ tris, int32_t* stack, int lane, float ox,
+                        float oy, float oz, float r0, float r1, float r2) {
+  const float ix = 1.0f / r0, iy = 1.0f / r1, iz = 1.0f / r2;
+  const double dox = ox, doy = oy, doz = oz;
+  float bt = __builtin_huge_valf();
+  int sp = 0, ref = 0;
+  for (;;) {
+    if (ref >= 0) {
+      const BvhNode nd = nodes[ref];
+      const int ra = __float_as_int(nd.a_lo.w), rb = __float_as_int(nd.b_lo.w);
+      const float tmax = bt + bt * 9.5367431640625e-07f;
+      const float ta = box_entry(nd.a_lo, nd.a_hi, ox, oy, oz, ix, iy, iz, tmax);
+      const float tb = rb == kBvhEmpty ? __builtin_huge_valf()
+                                       : box_entry(nd.b_lo, nd.b_hi, ox, oy, oz, ix, iy, iz, tmax);
+      const bool ha = ta != __builtin_huge_valf(), hb = tb != __builtin_huge_valf();
+      if (ha && hb) {
+        const bool a_first = ta <= tb;
+        stack[sp * 64 + lane] = a_first ? rb : ra;
+        sp++;
+        ref = a_first ? ra : rb;
+        continue;
+      }
+      if (ha || hb) {
+        ref = ha ? ra : rb;
+        continue;
+      }
+    } else {
+      const int2 lf = leaves[~ref];
+      for (int k = 0; k < lf.y; k++) {
+        const float4* tr = tris + int64_t(lf.x + k) * 3;
+        float t, u, w;
+        if (mt_hit(dox, doy, doz, r0, r1, r2, tr[0], tr[1], tr[2], t, u, w) && t < bt) bt = t;
+      }
+    }
+    if (sp == 0) break;
+    sp--;
+    ref = stack[sp * 64 + lane];
+  }
+  return bt;
+}
+
+__global__ __launch_bounds__(64) void k_visibility(DevView v, const uint8_t* __restrict__ color,
+                                                   const float* __restrict__ depth,
+                                                   const uint8_t* __restrict__ mask,
+                                                   DevVisOpt o, int n,
+                                                   const float* __restrict__ pos,
+                                                   const float* __restrict__ vert,
+                                                   const float* __restrict__ nrm,
+                                                   const BvhNode* __restrict__ nodes,
+                                                   const int2* __restrict__ leaves,
+                                                   const float4* __restrict__ tris,
+                                                   DevVisItem* __restrict__ out) {
+  __shared__ int32_t stack[kBvhStack * 64];
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x * 64 + lane;
+  if (i >= n) return;
+  DevVisItem r;
+  r.visible = 0;
+  r.px = r.py = r.c0 = r.c1 = r.c2 = 0.0f;
+  r.angle = 999.9f;
+  r.dist = -999.9f;
+  r.intensity = -1.0f;
+  const float px = pos[i * 3], py = pos[i * 3 + 1], pz = pos[i * 3 + 2];
+  // camera_p = w2c * world_p, then Project(camera_p, &image_p, &d)
+  const float qx = sum3(v.w2c_R[0] * px, v.w2c_R[1] * py, v.w2c_R[2] * pz) + v.w2c_t[0];
+  const float qy = sum3(v.w2c_R[3] * px, v.w2c_R[4] * py, v.w2c_R[5] * pz) + v.w2c_t[1];
+  const float qz = sum3(v.w2c_R[6] * px, v.w2c_R[7] * py, v.w2c_R[8] * pz) + v.w2c_t[2];
+  const float ix = v.fx / qz * qx + v.cx;
+  const float iy = v.fy / qz * qy + v.cy;
+  for (;;) {  // one pass; `break` = TestVertex returns false
+    if (qz < 0.0f || ix < 0.0f || static_cast<float>(v.W - 1) < ix || iy < 0.0f ||
+        static_cast<float>(v.H - 1) < iy)
+      break;
+    if (!(ix >= 0.0f && iy >= 0.0f)) break;  // NaN projections
+    const int nx = static_cast<int>(roundf(ix)), ny = static_cast<int>(roundf(iy));
+    const int64_t pi = int64_t(ny) * v.W + nx;
+    if (o.use_mask && mask[pi] != 255) break;
+    if (o.use_depth && depth[pi] < FLT_EPSILON) break;
+    float r0, r1, r2;
+    ray_w_at(v, ix, iy, r0, r1, r2);
+    const float dot = -sum3(nrm[i * 3] * r0, nrm[i * 3 + 1] * r1, nrm[i * 3 + 2] * r2);
+    const float angle = acosf(dot);
+    if (o.th < angle) break;
+    const float ox = v.c2w_t[0], oy = v.c2w_t[1], oz = v.c2w_t[2];
+    const float t = closest_t(nodes, leaves, tris, stack, lane, ox, oy, oz, r0, r1, r2);
+    const float vx = vert[i * 3], vy = vert[i * 3 + 1], vz = vert[i * 3 + 2];
+    float hx = vx, hy = vy, hz = vz;
+    const float distance = t < __builtin_huge_valf() ? t : FLT_MAX;
+    if (t < __builtin_huge_valf()) {
+      hx = ox + r0 * t;
+      hy = oy + r1 * t;
+      hz = oz + r2 * t;
+      const float dx = hx - vx, dy = hy - vy, dz = hz - vz;
+      if (sqrtf(sum3(dx * dx, dy * dy, dz * dz)) > o.eps) break;
+    }
+    if (o.use_depth) {
+      const float hcz = sum3(v.w2c_R[6] * hx, v.w2c_R[7] * hy, v.w2c_R[8] * hz) + v.w2c_t[2];
+      if (o.max_ddiff < fabsf(hcz - depth[pi])) break;
+    }
+    r.px = ix;
+    r.py = iy;
+    r.angle = angle;
+    r.dist = distance;
+    if (distance > o.max_dist || angle > o.max_angle) break;
+    float c[3];
+    if (o.interp == 0) {
+      for (int k = 0; k < 3; k++) c[k] = static_cast<float>(color[pi * 3 + k]);
+    } else {
+      // BilinearInterpolation (include/ugu/util/image_util.h:155-193)
+      int x0 = static_cast<int>(floorf(ix)), y0 = static_cast<int>(floorf(iy));
+      int x1 = x0 + 1, y1 = y0 + 1;
+      if (x0 < 0) x0 = 0;
+      if (y0 < 0) y0 = 0;
+      if (v.W <= x1) x1 = v.W - 1;
+      if (v.H <= y1) y1 = v.H - 1;
+      const float lu = ix - static_cast<float>(x0), lv = iy - static_cast<float>(y0);
+      const uint8_t* c00 = color + (int64_t(y0) * v.W + x0) * 3;
+      const uint8_t* c01 = color + (int64_t(y0) * v.W + x1) * 3;
+      const uint8_t* c10 = color + (int64_t(y1) * v.W + x0) * 3;
+      const uint8_t* c11 = color + (int64_t(y1) * v.W + x1) * 3;
+      for (int k = 0; k < 3; k++) {
+        const float f = (1.0f - lu) * (1.0f - lv) * static_cast<float>(c00[k]) +
+                        lu * (1.0f - lv) * static_cast<float>(c01[k]) +
+                        (1.0f - lu) * lv * static_cast<float>(c10[k]) +
+                        lu * lv * static_cast<float>(c11[k]);
+        c[k] = static_cast<float>(f2u8_x86(f));
+      }
+    }
+    r.c0 = c[0];
+    r.c1 = c[1];
+    r.c2 = c[2];
+    // Color2Gray (image_util.h:227-244): double arithmetic, cast to float
+    r.intensity = static_cast<float>(0.2989 * double(c[0]) + 0.5870 * double(c[1]) +
+                                     0.1140 * double(c[2]));
+    r.visible = 1;
+    break;
+  }
+  out[i] = r;
+}
+
+int launch_visibility(hipStream_t st, const DevView& v, const uint8_t* color, const float* depth,
+                      const uint8_t* mask, const DevVisOpt& o, int n, const float* pos,
+                      const float* vert, const float* nrm, const DevBvh& bvh, void* out) {
+  if (n < 1) return 0;
+  hipLaunchKernelGGL(k_visibility, dim3((n + 63) / 64), dim3(64), 0, st, v, color, depth, mask, o,
+                     n, pos, vert, nrm, bvh.nodes, bvh.leaves, bvh.tris,
+                     static_cast<DevVisItem*>(out));
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // views[i].tile_base = first 8x8 tile of view i in the concatenated tile list
 int launch_raytrace(hipStream_t st, const DevView* views, int n_views, int total_tiles,
                     const DevBvh& bvh, const DevMesh& m, const DevOptions& o) {

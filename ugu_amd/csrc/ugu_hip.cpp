@@ -37,6 +37,9 @@ int launch_backward(hipStream_t st, const DevView* views, int n_views,
 int launch_depth_to_u16(hipStream_t st, const float* d, int64_t n, uint16_t* out);
 int launch_raytrace(hipStream_t st, const DevView* views, int n_views, int total_tiles,
                     const DevBvh& bvh, const DevMesh& m, const DevOptions& o);
+int launch_visibility(hipStream_t st, const DevView& v, const uint8_t* color, const float* depth,
+                      const uint8_t* mask, const DevVisOpt& o, int n, const float* pos,
+                      const float* vert, const float* nrm, const DevBvh& bvh, void* out);
 }  // namespace ugu_hip
 
 using namespace ugu_hip;
@@ -114,6 +117,7 @@ struct ugu_hip_ctx {
   DevBuf blk_off, blk_vid, face_loc;           // setup blocks' vertex lists (prepared mesh)
   DevBuf blk_off_o, blk_vid_o, face_loc_o;     // the same for the original face array
   DevBuf bvh_nodes, bvh_leaves, bvh_tris;      // raytracer hierarchy
+  DevBuf vis_items, vis_images, vis_out;       // visibility test inputs / results
   bool rt_ready = false;
   int64_t rt_mesh_gen = -1;  // mesh generation the hierarchy was built for
   int64_t buf_gen = 0;
@@ -790,7 +794,8 @@ void ugu_hip_destroy(ugu_hip_ctx* c) {
                     &c->views, &c->tile_base, &c->stage_out, &c->diff_fn, &c->tile_view,
                     &c->bwd_items, &c->bwd_scratch, &c->soup, &c->diff_soup, &c->blk_off,
                     &c->blk_vid, &c->face_loc, &c->blk_off_o, &c->blk_vid_o, &c->face_loc_o,
-                    &c->bvh_nodes, &c->bvh_leaves, &c->bvh_tris};
+                    &c->bvh_nodes, &c->bvh_leaves, &c->bvh_tris, &c->vis_items,
+                    &c->vis_images, &c->vis_out};
   for (DevBuf* b : bufs) b->release();
   if (c->h_views) (void)hipHostFree(c->h_views);
   if (c->h_status) (void)hipHostFree(c->h_status);
@@ -996,6 +1001,85 @@ int ugu_hip_prepare_raytracer(ugu_hip_ctx* c) {
 int ugu_hip_raytrace(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions* o,
                      const UguOutputs* outs, int memory) {
   return raytrace_common(c, cams, n, o, outs, memory);
+}
+
+int ugu_hip_visibility_test(ugu_hip_ctx* c, const UguKeyframe* kf, const UguVisibilityOptions* o,
+                            int n, const float* item_pos, const float* item_vertex,
+                            const float* item_normal, UguVertexVisibility* out) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  if (kf == nullptr) return fail(c, UGU_HIP_ERR_NO_CAMERA, "keyframe has not been set\n");
+  if (!(c->prepared && c->rt_ready && c->rt_mesh_gen == c->mesh_gen))
+    return fail(c, UGU_HIP_ERR_MESH_NOT_PREPARED, "mesh has not been initialized\n");
+  if (o == nullptr || n < 0 || (n > 0 && (!item_pos || !item_vertex || !item_normal || !out)))
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "invalid options or items");
+  // ValidateAndInitBeforeTest (visibility_tester.cc:331-355)
+  if (kf->color == nullptr || kf->width < 1 || kf->height < 1)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "keyframe color is empty\n");
+  if (o->use_mask && kf->mask == nullptr)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "use_mask is true but keyframe mask is empty\n");
+  if (o->use_depth && kf->depth == nullptr)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "use_depth is true but keyframe depth is empty\n");
+  if (o->interp < 0 || o->interp > 1)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "option value out of range");
+  if (int err = check_camera(c, kf->camera)) return err;
+  if (n == 0) return UGU_HIP_OK;
+  UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int oe = order_after_user(c)) return oe;
+  const size_t px = size_t(kf->width) * size_t(kf->height);
+  const size_t item_bytes = size_t(n) * 12;
+  int err;
+  if ((err = ensure(c, c->vis_items, item_bytes * 3))) return err;
+  if ((err = ensure(c, c->vis_images, px * 8 + 512))) return err;  // colour, depth, mask
+  if ((err = ensure(c, c->vis_out, size_t(n) * sizeof(UguVertexVisibility)))) return err;
+  uint8_t* items = c->vis_items.as<uint8_t>();
+  uint8_t* img = c->vis_images.as<uint8_t>();
+  const size_t depth_off = (px * 3 + 255) / 256 * 256, mask_off = depth_off + px * 4;
+  UGU_HIP_CHECK(c, hipMemcpyAsync(items, item_pos, item_bytes, hipMemcpyHostToDevice, c->stream));
+  UGU_HIP_CHECK(c, hipMemcpyAsync(items + item_bytes, item_vertex, item_bytes,
+                                  hipMemcpyHostToDevice, c->stream));
+  UGU_HIP_CHECK(c, hipMemcpyAsync(items + 2 * item_bytes, item_normal, item_bytes,
+                                  hipMemcpyHostToDevice, c->stream));
+  UGU_HIP_CHECK(c, hipMemcpyAsync(img, kf->color, px * 3, hipMemcpyHostToDevice, c->stream));
+  if (o->use_depth)
+    UGU_HIP_CHECK(c, hipMemcpyAsync(img + depth_off, kf->depth, px * 4, hipMemcpyHostToDevice,
+                                    c->stream));
+  if (o->use_mask)
+    UGU_HIP_CHECK(c, hipMemcpyAsync(img + mask_off, kf->mask, px, hipMemcpyHostToDevice,
+                                    c->stream));
+  DevView v;
+  std::memset(&v, 0, sizeof(v));
+  v.W = kf->width;
+  v.H = kf->height;
+  v.fx = kf->camera.fx;
+  v.fy = kf->camera.fy;
+  v.cx = kf->camera.cx;
+  v.cy = kf->camera.cy;
+  std::memcpy(v.w2c_R, kf->camera.w2c_R, sizeof(v.w2c_R));
+  std::memcpy(v.w2c_t, kf->camera.w2c_t, sizeof(v.w2c_t));
+  std::memcpy(v.c2w_R, kf->camera.c2w_R, sizeof(v.c2w_R));
+  std::memcpy(v.c2w_t, kf->camera.c2w_t, sizeof(v.c2w_t));
+  DevVisOpt d;
+  d.th = o->backface_culling_angle_rad_th;
+  d.use_mask = o->use_mask ? 1 : 0;
+  d.use_depth = o->use_depth ? 1 : 0;
+  d.interp = o->interp;
+  d.max_angle = o->max_viewing_angle;
+  d.max_dist = o->max_distance_from_camera_to_vertex;
+  d.max_ddiff = o->max_depth_difference;
+  d.eps = o->eps;
+  DevBvh bvh;
+  bvh.nodes = c->bvh_nodes.as<BvhNode>();
+  bvh.leaves = c->bvh_leaves.as<int2>();
+  bvh.tris = c->bvh_tris.as<float4>();
+  const float* ip = reinterpret_cast<const float*>(items);
+  if (launch_visibility(c->stream, v, img, reinterpret_cast<const float*>(img + depth_off),
+                        img + mask_off, d, n, ip, ip + size_t(n) * 3, ip + size_t(n) * 6, bvh,
+                        c->vis_out.p))
+    return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
+  UGU_HIP_CHECK(c, hipMemcpyAsync(out, c->vis_out.p, size_t(n) * sizeof(UguVertexVisibility),
+                                  hipMemcpyDeviceToHost, c->stream));
+  UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return UGU_HIP_OK;
 }
 
 int ugu_hip_set_tiling(ugu_hip_ctx* c, const UguTiling* t) {
