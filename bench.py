@@ -246,6 +246,37 @@ def raytrace_benchmarks(torch, scenes, mesh, cams, cam1080, opt, args):
         "prepare_mesh_and_bvh_s": prep}}
 
 
+def visibility_benchmarks(torch, np, mesh, cam1080, opt):
+    """Visibility test (ugu_hip_visibility_test): every vertex of the mesh
+    against the 1080p keyframe (colour = the rasterizer's render, depth and
+    mask from it), vertex-wise and face-wise items; host arrays in and out,
+    so the time includes the PCIe copies."""
+    from ugu_amd import Rasterizer, visibility as V
+    r = Rasterizer(opt)
+    r.set_mesh(mesh)
+    assert r.PrepareMesh()
+    out = r.render_batch([cam1080])[0]
+    kf = V.Keyframe(0, out["color"], out["depth"], out["mask"], cam1080)
+    t = V.VisibilityTester()
+    t.set_data(mesh.vertices, mesh.normals, mesh.vertex_indices)
+    t0 = time.perf_counter()
+    assert t.PrepareData()
+    prep = time.perf_counter() - t0
+    t.set_keyframe(kf)
+    res = {"prepare_s": prep}
+    for mode, items in (("vertex", (t.vertices_, t.vertices_, t.normals_)),
+                        ("facewise", t.facewise_items())):
+        t.test_items(*items)  # warm-up
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            rec, _ = t.test_items(*items)
+        dt = (time.perf_counter() - t0) / reps
+        res[mode] = {"items": len(items[0]), "ms": dt * 1e3, "mitems_per_s": len(items[0]) / dt / 1e6,
+                     "visible": int(rec["visible"].sum())}
+    return {"visibility_1080p": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,7 +324,9 @@ def main():
     opt = RendererCpuOption(diffuse_color=DiffuseColor.kTexture,
                             interp=ColorInterpolation.kBilinear)
     if args.raytrace_only:
-        print(json.dumps(raytrace_benchmarks(torch, scenes, mesh, cams, cam2, opt, args)))
+        res = raytrace_benchmarks(torch, scenes, mesh, cams, cam2, opt, args)
+        res.update(visibility_benchmarks(torch, np, mesh, cam2, opt))
+        print(json.dumps(res))
         return
     r = Rasterizer(opt)
     r.set_mesh(mesh)
@@ -374,6 +407,8 @@ def main():
     if not args.quick:
         log("raytracer legs")
         extra.update(raytrace_benchmarks(torch, scenes, mesh, cams, cam2, opt, args))
+        log("visibility leg")
+        extra.update(visibility_benchmarks(torch, np, mesh, cam2, opt))
     if not args.quick and not args.no_diff:
         extra.update(diff_benchmarks(torch, np, scenes, mesh, cams, args))
 
