@@ -30,6 +30,9 @@
  *   ugu_hip_visibility_test
  *       VisibilityTester::TestVertex (src/texturing/visibility_tester.cc:390-530)
  *       for a batch of vertices against one keyframe
+ *   ugu_hip_texture_map
+ *       GenerateTextureOnOriginalUv + PaddingSimple of TextureMapping with
+ *       kUseOriginalMeshUv (src/texturing/texture_mapper.cc:148-201, :234-345)
  *   ugu_hip_depth_to_u16
  *       the ConvertTo step of Rasterizer::RenderW (src/renderer/cpu/rasterizer.cc:329-344,
  *       include/ugu/image_proc.h:453-520)
@@ -330,6 +333,27 @@ int ugu_hip_visibility_test(ugu_hip_ctx* ctx, const UguKeyframe* keyframe,
                             const UguVisibilityOptions* options, int n,
                             const float* item_pos, const float* item_vertex,
                             const float* item_normal, UguVertexVisibility* out);
+
+/* ---- texture mapping on the mesh's own uv (texturing consumer) --------
+ * One entry per face: keyframe = index into the colour images (-1: the face
+ * has no visible keyframe and is skipped), src = its projected triangle in
+ * that image (FaceInfoPerKeyframe::projected_tri), dst = its uv triangle in
+ * texel coordinates (u * tex_w - 0.5, (1 - v) * tex_h - 0.5). */
+typedef struct {
+  int32_t keyframe;
+  float src[6];
+  float dst[6];
+} UguTexFace;
+
+/* Rasterizes every face's dst triangle into a tex_w x tex_h RGB8 texture,
+ * sampling its src triangle of the keyframe colour bilinearly; where dst
+ * triangles overlap the face with the larger index wins (the reference's
+ * face loop order); then PaddingSimple with padding_kernel. texture
+ * (tex_h x tex_w x 3) and mask (tex_h x tex_w, 255 = set) are host outputs;
+ * images are RGB8 host images (UguTexture). */
+int ugu_hip_texture_map(ugu_hip_ctx* ctx, const UguTexture* images, int n_images,
+                        const UguTexFace* faces, int n_faces, int tex_w, int tex_h,
+                        int padding_kernel, uint8_t* texture, uint8_t* mask);
 
 /* ---- differentiable rendering (no reference counterpart) ----------------
  * Uses the prepared mesh's topology (faces, uv, uv_indices) with parameters

@@ -281,6 +281,28 @@ def visibility(mesh, cam, c2w_t, color, depth, mask, opt, item_pos, item_vertex,
     return out
 
 
+class OracleTexFace(ctypes.Structure):
+    _fields_ = [("keyframe", _i), ("src", _f * 6), ("dst", _f * 6)]
+
+
+def texture_map(images, faces, tex_w, tex_h, padding_kernel):
+    """oracle_texture_map: `images` = list of (H, W, 3) uint8, `faces` = a
+    ctypes array of OracleTexFace -> (texture, mask)."""
+    L = lib()
+    L.oracle_texture_map.restype = ctypes.c_int
+    imgs = [np.ascontiguousarray(im, np.uint8) for im in images]
+    iw = np.array([im.shape[1] for im in imgs] or [0], np.int32)
+    ih = np.array([im.shape[0] for im in imgs] or [0], np.int32)
+    ptrs = (ctypes.c_void_p * max(1, len(imgs)))(*[im.ctypes.data for im in imgs])
+    tex = np.zeros((tex_h, tex_w, 3), np.uint8)
+    mask = np.zeros((tex_h, tex_w), np.uint8)
+    rc = L.oracle_texture_map(_p(iw), _p(ih), ptrs, ctypes.c_int32(len(faces)), faces,
+                              ctypes.c_int32(tex_w), ctypes.c_int32(tex_h),
+                              ctypes.c_int32(padding_kernel), _p(tex), _p(mask))
+    assert rc == 0
+    return tex, mask
+
+
 def depth_to_u16(depth):
     d = np.ascontiguousarray(depth, np.float32)
     out = np.empty(d.shape, np.uint16)

@@ -887,6 +887,112 @@ int oracle_visibility(const OracleMesh* mesh, const OracleCamera* cam, const flo
   return kOk;
 }
 
+// ---------------------------------------------------------------------------
+// GenerateTextureOnOriginalUv's face loop with RasterizeTriangle and
+// BilinearInterpolation, then PaddingSimple (texture_mapper.cc:148-201,
+// 234-345; image_util.h:155-193), serially in face order.
+
+struct OracleTexFace {
+  int32_t keyframe;
+  float src[6];
+  float dst[6];
+};
+
+static float tm_edge(float ax, float ay, float bx, float by, float cx, float cy) {
+  return (cx - ax) * (by - ay) - (cy - ay) * (bx - ax);
+}
+
+int oracle_texture_map(const int32_t* img_w, const int32_t* img_h, const uint8_t* const* imgs,
+                       int32_t n_faces, const OracleTexFace* faces, int32_t W, int32_t H,
+                       int32_t padding_kernel, uint8_t* texture, uint8_t* mask) {
+  std::memset(texture, 0, size_t(W) * H * 3);
+  std::memset(mask, 0, size_t(W) * H);
+  std::vector<int32_t> written_by(size_t(W) * H, -1);
+  for (int32_t fi = 0; fi < n_faces; fi++) {
+    const OracleTexFace& f = faces[fi];
+    if (f.keyframe < 0) continue;
+    const int32_t iw = img_w[f.keyframe], ih = img_h[f.keyframe];
+    const uint8_t* img = imgs[f.keyframe];
+    const float* t = f.dst;
+    float area = tm_edge(t[0], t[1], t[2], t[3], t[4], t[5]);
+    if (std::fabs(area) < std::numeric_limits<float>::min())
+      area = area > 0 ? std::numeric_limits<float>::min() : -std::numeric_limits<float>::min();
+    const float inv_area = 1.0f / area;
+    int xmin = f2i_x86(std::min({t[0], t[2], t[4]}) - 1);
+    xmin = std::max(0, std::min(xmin, W - 1));
+    int xmax = f2i_x86(std::max({t[0], t[2], t[4]}) + 1);
+    xmax = std::max(0, std::min(xmax, W - 1));
+    int ymin = f2i_x86(std::min({t[1], t[3], t[5]}) - 1);
+    ymin = std::max(0, std::min(ymin, H - 1));
+    int ymax = f2i_x86(std::max({t[1], t[3], t[5]}) + 1);
+    ymax = std::max(0, std::min(ymax, H - 1));
+    for (int y = ymin; y <= ymax; y++) {
+      for (int x = xmin; x <= xmax; x++) {
+        const float px = static_cast<float>(x), py = static_cast<float>(y);
+        float w0 = tm_edge(t[2], t[3], t[4], t[5], px, py);
+        float w1 = tm_edge(t[4], t[5], t[0], t[1], px, py);
+        float w2 = tm_edge(t[0], t[1], t[2], t[3], px, py);
+        w0 *= inv_area;
+        w1 *= inv_area;
+        w2 *= inv_area;
+        if (w0 < 0 || w1 < 0 || w2 < 0) continue;
+        const size_t ti = size_t(y) * W + x;
+        if (MUT(25) && written_by[ti] >= 0) continue;  // planted: the first face wins
+        written_by[ti] = fi;
+        const float sx = w0 * f.src[0] + w1 * f.src[2] + w2 * f.src[4];
+        const float sy = w0 * f.src[1] + w1 * f.src[3] + w2 * f.src[5];
+        int x0 = static_cast<int>(std::floor(sx)), y0 = static_cast<int>(std::floor(sy));
+        int x1 = x0 + 1, y1 = y0 + 1;
+        if (x0 < 0) x0 = 0;
+        if (y0 < 0) y0 = 0;
+        if (iw <= x1) x1 = iw - 1;
+        if (ih <= y1) y1 = ih - 1;
+        float lu = sx - x0, lv = sy - y0;
+        if (MUT(27)) std::swap(lu, lv);
+        auto at = [&](int yy, int xx, int c) {
+          yy = std::min(std::max(yy, 0), ih - 1);  // in bounds for a visible face
+          xx = std::min(std::max(xx, 0), iw - 1);
+          return static_cast<float>(img[(size_t(yy) * iw + xx) * 3 + c]);
+        };
+        for (int c = 0; c < 3; c++) {
+          const float v = (1.0f - lu) * (1.0f - lv) * at(y0, x0, c) + lu * (1.0f - lv) * at(y0, x1, c) +
+                          (1.0f - lu) * lv * at(y1, x0, c) + lu * lv * at(y1, x1, c);
+          texture[ti * 3 + c] = f2u8_x86(v);
+        }
+        mask[ti] = 255;
+      }
+    }
+  }
+  // PaddingSimple on copies of the texture and mask
+  const std::vector<uint8_t> t0(texture, texture + size_t(W) * H * 3);
+  const std::vector<uint8_t> m0(mask, mask + size_t(W) * H);
+  const int hk = padding_kernel / 2;
+  for (int j = hk; j < H - hk; j++) {
+    for (int i = hk; i < W - hk; i++) {
+      // planted mistake 26: read the texture being padded instead of its copy
+      const uint8_t* mm = MUT(26) ? mask : m0.data();
+      if (m0[size_t(j) * W + i] == 255) continue;
+      float a[3] = {0, 0, 0};
+      size_t cnt = 0;
+      for (int jj = -hk; jj <= hk; jj++)
+        for (int ii = -hk; ii <= hk; ii++) {
+          const size_t k = size_t(j + jj) * W + (i + ii);
+          if (mm[k] == 255) {
+            for (int c = 0; c < 3; c++) a[c] += (MUT(26) ? texture : t0.data())[k * 3 + c];
+            cnt++;
+          }
+        }
+      if (cnt == 0) continue;
+      mask[size_t(j) * W + i] = 255;
+      for (int c = 0; c < 3; c++) {
+        a[c] /= cnt;
+        texture[(size_t(j) * W + i) * 3 + c] = f2u8_x86(a[c]);
+      }
+    }
+  }
+  return kOk;
+}
+
 // RenderW / RenderDepthW (rasterizer.cc:329-348) depth conversion:
 // ConvertTo(f_depth, depth) with scale 1 -> static_cast<uint16_t>(1.0 * (double)d)
 void oracle_depth_to_u16(const float* depth, int64_t n, uint16_t* out) {

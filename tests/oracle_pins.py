@@ -711,8 +711,81 @@ def pin_visibility():
     return True, ""
 
 
+
+
+
+# ---------------------------------------------------------------------------
+# texture mapping (oracle_texture_map)
+
+
+def _tex_face(kf, src, dst):
+    f = O.OracleTexFace()
+    f.keyframe = kf
+    for i in range(6):
+        f.src[i] = float(src[i])
+        f.dst[i] = float(dst[i])
+    return f
+
+
+def pin_texture_map():
+    """One face copying a linear-gradient image (bilinear reproduces a linear
+    function) into a 128^2 texture: coverage = the target triangle, values =
+    the gradient at the barycentric map; overlapping faces: the later one
+    wins; PaddingSimple = numpy 3x3 average of the unpadded texture."""
+    H, W = 48, 64
+    yy, xx = np.mgrid[0:H, 0:W]
+    img = np.stack([xx + 2 * yy, 3 * xx, np.full_like(xx, 100)], -1).astype(np.uint8)
+    src = np.array([5, 5, 50, 8, 20, 40], np.float32)
+    dst = np.array([10, 10, 100, 20, 40, 90], np.float32)
+    faces = (O.OracleTexFace * 1)(_tex_face(0, src, dst))
+    tex, mask = O.texture_map([img], faces, 128, 128, 0)
+    ty, tx = np.mgrid[0:128, 0:128].astype(np.float64)
+    d = dst.astype(np.float64).reshape(3, 2)
+
+    def e(a, b, px, py):
+        return (px - a[0]) * (b[1] - a[1]) - (py - a[1]) * (b[0] - a[0])
+    A = e(d[0], d[1], d[2][0], d[2][1])
+    w = [e(d[1], d[2], tx, ty) / A, e(d[2], d[0], tx, ty) / A, e(d[0], d[1], tx, ty) / A]
+    inside = (w[0] > 1e-4) & (w[1] > 1e-4) & (w[2] > 1e-4)
+    outside = (w[0] < -1e-4) | (w[1] < -1e-4) | (w[2] < -1e-4)
+    if not (mask[inside] == 255).all() or (mask[outside] == 255).any():
+        return False, "texture coverage is not the target triangle"
+    s = src.astype(np.float64).reshape(3, 2)
+    sx = w[0] * s[0, 0] + w[1] * s[1, 0] + w[2] * s[2, 0]
+    sy = w[0] * s[0, 1] + w[1] * s[1, 1] + w[2] * s[2, 1]
+    er = np.abs(tex[..., 0].astype(float) - np.floor(sx + 2 * sy))[inside]
+    eg = np.abs(tex[..., 1].astype(float) - np.floor(3 * sx))[inside]
+    # a constant 100 can truncate to 99: the four float weights need not sum to 1
+    if er.max() > 1 or eg.max() > 1 or not np.isin(tex[..., 2][inside], (99, 100)).all():
+        return False, f"texel values are not the bilinear samples (err {er.max()}, {eg.max()})"
+    # overlap: the later face wins
+    black = np.full((H, W, 3), 10, np.uint8)
+    white = np.full((H, W, 3), 200, np.uint8)
+    faces2 = (O.OracleTexFace * 2)(_tex_face(0, src, dst), _tex_face(1, src, dst))
+    t2, m2 = O.texture_map([black, white], faces2, 128, 128, 0)
+    if not np.isin(t2[m2 == 255], (199, 200)).all():
+        return False, "overlapping faces: the later face must win"
+    # padding
+    tp, mp = O.texture_map([img], faces, 128, 128, 3)
+    exp_t, exp_m = tex.copy(), mask.copy()
+    valid = mask == 255
+    for j in range(1, 127):
+        for i in range(1, 127):
+            if valid[j, i]:
+                continue
+            win = valid[j - 1:j + 2, i - 1:i + 2]
+            if not win.any():
+                continue
+            avg = tex[j - 1:j + 2, i - 1:i + 2][win].astype(np.float32).sum(0) / np.float32(win.sum())
+            exp_t[j, i] = np.floor(avg).astype(np.uint8)
+            exp_m[j, i] = 255
+    if not (np.array_equal(tp, exp_t) and np.array_equal(mp, exp_m)):
+        return False, "PaddingSimple"
+    return True, ""
+
+
 ALL_PINS = [pin_brute_force, pin_ties_lowest_id, pin_face_zero_not_shaded,
             pin_perspective_correct, pin_texture_sampling, pin_sphere_closed_forms,
             pin_normals_rotated_camera, pin_lambertian, pin_oren_nayar, pin_depth_u16,
             pin_gbuffer, pin_raytrace_brute_force, pin_raytrace_plane, pin_raytrace_sphere,
-            pin_visibility]
+            pin_visibility, pin_texture_map]

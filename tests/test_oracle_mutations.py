@@ -41,6 +41,9 @@ MUTATIONS = {
     22: "visibility: nearest pixel by floor instead of round",
     23: "visibility: viewing angle from +n . ray",
     24: "visibility: bilinear weights swapped",
+    25: "texture mapping: the first overlapping face wins",
+    26: "texture mapping: padding reads the texture being padded",
+    27: "texture mapping: bilinear weights swapped",
 }
 
 

@@ -105,6 +105,10 @@ VIS_DTYPE = np.dtype([("visible", np.int32), ("px", np.float32), ("py", np.float
                       ("c1", np.float32), ("c2", np.float32), ("intensity", np.float32)])
 
 
+class UguTexFace(ctypes.Structure):
+    _fields_ = [("keyframe", _i), ("src", _f * 6), ("dst", _f * 6)]
+
+
 class UguTiling(ctypes.Structure):
     _fields_ = [("tile_w", _i), ("tile_h", _i), ("bin_capacity", _i), ("persistent", _i)]
 
@@ -152,6 +156,9 @@ SIGNATURES = {
     "ugu_hip_visibility_test": (ctypes.c_int, [_P, ctypes.POINTER(UguKeyframe),
                                                ctypes.POINTER(UguVisibilityOptions), ctypes.c_int,
                                                _P, _P, _P, _P]),
+    "ugu_hip_texture_map": (ctypes.c_int, [_P, ctypes.POINTER(UguTexture), ctypes.c_int,
+                                           ctypes.POINTER(UguTexFace), ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, _P, _P]),
     "ugu_hip_set_tiling": (ctypes.c_int, [_P, ctypes.POINTER(UguTiling)]),
     "ugu_hip_get_tiling": (ctypes.c_int, [_P, ctypes.POINTER(UguTiling)]),
     "ugu_hip_autotune": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,

@@ -37,6 +37,9 @@ int launch_backward(hipStream_t st, const DevView* views, int n_views,
 int launch_depth_to_u16(hipStream_t st, const float* d, int64_t n, uint16_t* out);
 int launch_raytrace(hipStream_t st, const DevView* views, int n_views, int total_tiles,
                     const DevBvh& bvh, const DevMesh& m, const DevOptions& o);
+int launch_texture_map(hipStream_t st, const void* faces, int n, const void* images, int cols,
+                       int rows, int hk, int32_t* owner, uint8_t* tex0, uint8_t* mask0,
+                       uint8_t* tex, uint8_t* mask);
 int launch_visibility(hipStream_t st, const DevView& v, const uint8_t* color, const float* depth,
                       const uint8_t* mask, const DevVisOpt& o, int n, const float* pos,
                       const float* vert, const float* nrm, const DevBvh& bvh, void* out);
@@ -118,6 +121,7 @@ struct ugu_hip_ctx {
   DevBuf blk_off_o, blk_vid_o, face_loc_o;     // the same for the original face array
   DevBuf bvh_nodes, bvh_leaves, bvh_tris;      // raytracer hierarchy
   DevBuf vis_items, vis_images, vis_out;       // visibility test inputs / results
+  DevBuf tm_faces, tm_images, tm_out;          // texture mapping inputs / work / outputs
   bool rt_ready = false;
   int64_t rt_mesh_gen = -1;  // mesh generation the hierarchy was built for
   int64_t buf_gen = 0;
@@ -795,7 +799,7 @@ void ugu_hip_destroy(ugu_hip_ctx* c) {
                     &c->bwd_items, &c->bwd_scratch, &c->soup, &c->diff_soup, &c->blk_off,
                     &c->blk_vid, &c->face_loc, &c->blk_off_o, &c->blk_vid_o, &c->face_loc_o,
                     &c->bvh_nodes, &c->bvh_leaves, &c->bvh_tris, &c->vis_items,
-                    &c->vis_images, &c->vis_out};
+                    &c->vis_images, &c->vis_out, &c->tm_faces, &c->tm_images, &c->tm_out};
   for (DevBuf* b : bufs) b->release();
   if (c->h_views) (void)hipHostFree(c->h_views);
   if (c->h_status) (void)hipHostFree(c->h_status);
@@ -1078,6 +1082,66 @@ int ugu_hip_visibility_test(ugu_hip_ctx* c, const UguKeyframe* kf, const UguVisi
     return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
   UGU_HIP_CHECK(c, hipMemcpyAsync(out, c->vis_out.p, size_t(n) * sizeof(UguVertexVisibility),
                                   hipMemcpyDeviceToHost, c->stream));
+  UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return UGU_HIP_OK;
+}
+
+int ugu_hip_texture_map(ugu_hip_ctx* c, const UguTexture* images, int n_images,
+                        const UguTexFace* faces, int n_faces, int tex_w, int tex_h,
+                        int padding_kernel, uint8_t* texture, uint8_t* mask) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  if (tex_w < 1 || tex_h < 1 || tex_w > 32768 || tex_h > 32768 || padding_kernel < 0 ||
+      n_images < 0 || n_faces < 0 || (n_faces > 0 && faces == nullptr) ||
+      (n_images > 0 && images == nullptr) || texture == nullptr || mask == nullptr)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "invalid texture mapping arguments");
+  for (int i = 0; i < n_images; i++)
+    if (images[i].width < 1 || images[i].height < 1 || images[i].data == nullptr)
+      return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "keyframe color is empty\n");
+  for (int i = 0; i < n_faces; i++)
+    if (faces[i].keyframe < -1 || faces[i].keyframe >= n_images)
+      return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "face keyframe index out of range");
+  UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int oe = order_after_user(c)) return oe;
+  struct Img {
+    int32_t w, h;
+    const uint8_t* data;
+  };
+  std::vector<size_t> off(size_t(n_images) + 1, 0);
+  for (int i = 0; i < n_images; i++)
+    off[size_t(i) + 1] = off[size_t(i)] +
+                         (size_t(images[i].width) * size_t(images[i].height) * 3 + 255) / 256 * 256;
+  const size_t texels = size_t(tex_w) * size_t(tex_h);
+  const size_t desc_bytes = (size_t(n_images) * sizeof(Img) + 255) / 256 * 256;
+  int err;
+  if ((err = ensure(c, c->tm_faces, size_t(n_faces) * sizeof(UguTexFace) + 16))) return err;
+  if ((err = ensure(c, c->tm_images, desc_bytes + off[size_t(n_images)] + 16))) return err;
+  // owner (int32), unpadded texture + mask, padded texture + mask
+  const size_t o_tex0 = texels * 4, o_mask0 = o_tex0 + texels * 3,
+               o_tex = (o_mask0 + texels + 255) / 256 * 256, o_mask = o_tex + texels * 3;
+  if ((err = ensure(c, c->tm_out, o_mask + texels))) return err;
+  uint8_t* ib = c->tm_images.as<uint8_t>();
+  std::vector<Img> desc(static_cast<size_t>(n_images));
+  for (int i = 0; i < n_images; i++) {
+    desc[size_t(i)] = Img{images[i].width, images[i].height, ib + desc_bytes + off[size_t(i)]};
+    UGU_HIP_CHECK(c, hipMemcpyAsync(ib + desc_bytes + off[size_t(i)], images[i].data,
+                                    size_t(images[i].width) * size_t(images[i].height) * 3,
+                                    hipMemcpyHostToDevice, c->stream));
+  }
+  if (n_images > 0)
+    UGU_HIP_CHECK(c, hipMemcpyAsync(ib, desc.data(), desc.size() * sizeof(Img),
+                                    hipMemcpyHostToDevice, c->stream));
+  if (n_faces > 0)
+    UGU_HIP_CHECK(c, hipMemcpyAsync(c->tm_faces.p, faces, size_t(n_faces) * sizeof(UguTexFace),
+                                    hipMemcpyHostToDevice, c->stream));
+  uint8_t* ob = c->tm_out.as<uint8_t>();
+  UGU_HIP_CHECK(c, hipMemsetAsync(ob + o_tex0, 0, texels * 4, c->stream));  // texture0 + mask0
+  if (launch_texture_map(c->stream, c->tm_faces.p, n_faces, ib, tex_w, tex_h, padding_kernel / 2,
+                         reinterpret_cast<int32_t*>(ob), ob + o_tex0, ob + o_mask0, ob + o_tex,
+                         ob + o_mask))
+    return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
+  UGU_HIP_CHECK(c, hipMemcpyAsync(texture, ob + o_tex, texels * 3, hipMemcpyDeviceToHost,
+                                  c->stream));
+  UGU_HIP_CHECK(c, hipMemcpyAsync(mask, ob + o_mask, texels, hipMemcpyDeviceToHost, c->stream));
   UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return UGU_HIP_OK;
 }
