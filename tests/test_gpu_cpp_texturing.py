@@ -1,0 +1,135 @@
+"""GPU: the C++ texturing API (ugu::VisibilityTester, ugu::TextureMapping in
+ugu_amd/libugu_rasterizer.so) driven like UGU's texturing example by
+ugu_amd/cpp/test_texturing_api on the bundled Spot mesh -- LoadObj, keyframes
+rendered by the C++ Rasterizer, Test(keyframes, info) face-wise and
+vertex-wise, TextureMapping on the mesh uv -- against the Python pipeline
+run on the same keyframe images. Records, the selection statistics and the
+textures are bit-exact; the Python pipeline itself is checked against the
+oracle in test_gpu_visibility.py and test_gpu_texturing.py. Also runs the
+driver's checks of the reference's error conventions."""
+import copy
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from ugu_amd import PinholeCamera, io as uio, look_at_c2w, texturing as T, visibility as V
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DATA = os.path.join(ROOT, "tests", "golden", "data")
+DRIVER = os.path.join(ROOT, "ugu_amd", "cpp", "test_texturing_api")
+W, H, NCAM = 256, 192, 6
+
+
+def _cameras():
+    cams = []
+    for i in range(NCAM):
+        a = 2 * np.pi * i / NCAM
+        pos = np.array([2.0 * np.sin(a), 0.5, 0.35 - 2.0 * np.cos(a)])
+        cams.append(PinholeCamera(W, H, look_at_c2w(pos, (0, 0, 0.35), (0, 1, 0)), (127.5, 95.5),
+                                  (220.0, 220.0)))
+    return cams
+
+
+def _vertex_arrays(info):
+    rec, stat = [], []
+    for i, vi in enumerate(info.vertex_info_list):
+        for k in vi.visible_keyframes:
+            rec.append([i, k.kf_id, *k.color, k.intensity, *k.projected_pos, k.viewing_angle,
+                        k.distance])
+        if vi.visible_keyframes:
+            stat.append([*vi.mean_color, *vi.mean_distance_color, vi.min_viewing_angle_id,
+                         vi.min_distance_id])
+        else:
+            stat.append([0, 0, 0, 0, 0, 0, -1, -1])
+    return np.array(rec, np.float32).reshape(-1, 10), np.array(stat, np.float32)
+
+
+def _face_arrays(info):
+    rec, stat = [], []
+    for fi in info.face_info_list:
+        for k in fi.visible_keyframes:
+            rec.append([k.face_id, k.kf_id, k.area, k.viewing_angle, k.distance,
+                        *np.asarray(k.projected_tri, np.float32).reshape(-1)])
+        if fi.visible_keyframes:
+            stat.append([fi.max_area_index, fi.min_viewing_angle_index, fi.min_distance_index])
+        else:
+            stat.append([-1, -1, -1])
+    return np.array(rec, np.float32).reshape(-1, 11), np.array(stat, np.int32)
+
+
+def _same(a, b):
+    # bit-exact, with NaN == NaN and +inf == +inf
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def cpp_run(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("cpp_tex"))
+    for n in ("spot.mtl", "spot_texture.png"):
+        shutil.copy(os.path.join(DATA, n), d)
+    src = open(os.path.join(DATA, "spot_quadrangulated.obj")).read()
+    obj = os.path.join(d, "spot_q.obj")
+    open(obj, "w").write("mtllib spot.mtl\nusemtl material000\n" + src)
+    cams = _cameras()
+    rows = [[c.width, c.height, *c.focal_length, *c.principal_point, *c.c2w.reshape(-1)]
+            for c in cams]
+    np.array(rows, np.float64).tofile(os.path.join(d, "cams.f64"))
+    out = os.path.join(d, "out")
+    os.makedirs(out)
+    print(f"[cpp-texturing] driver on {obj}", flush=True)
+    res = subprocess.run([DRIVER, obj, os.path.join(d, "cams.f64"), str(NCAM), out],
+                         capture_output=True, text=True, timeout=120)
+    print(res.stdout[-2000:], res.stderr[-2000:], flush=True)
+    mesh, ok = uio.load_obj(obj)
+    assert ok
+    kfs = []
+    for i, cam in enumerate(cams):
+        p = os.path.join(out, f"kf{i}_")
+        kfs.append(V.Keyframe(i, np.fromfile(p + "color.u8", np.uint8).reshape(H, W, 3),
+                              np.fromfile(p + "depth.f32", np.float32).reshape(H, W),
+                              np.fromfile(p + "mask.u8", np.uint8).reshape(H, W), cam))
+    return res, out, mesh, kfs
+
+
+def test_driver_checks(gpu_available, cpp_run):
+    res = cpp_run[0]
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "FAILED" not in res.stdout
+    assert res.stdout.count("CHECK ") >= 20
+
+
+@pytest.mark.parametrize("facewise", [True, False])
+def test_cpp_matches_python_pipeline(gpu_available, cpp_run, facewise):
+    res, out, mesh, kfs = cpp_run
+    assert res.returncode == 0, res.stdout + res.stderr
+    vt = V.VisibilityTester()
+    vt.set_data(mesh.vertices, mesh.normals, mesh.vertex_indices)
+    assert vt.PrepareData()
+    info = V.VisibilityInfo(mesh)
+    assert vt.Test(info, facewise=facewise, keyframes=kfs)
+    p = os.path.join(out, "fw_" if facewise else "vw_")
+    vrec, vstat = _vertex_arrays(info)
+    frec, fstat = _face_arrays(info)
+    c_vrec = np.fromfile(p + "vrec.f32", np.float32).reshape(-1, 10)
+    c_frec = np.fromfile(p + "frec.f32", np.float32).reshape(-1, 11)
+    c_vstat = np.fromfile(p + "vstat.f32", np.float32).reshape(-1, 8)
+    c_fstat = np.fromfile(p + "fstat.i32", np.int32).reshape(-1, 3)
+    print(f"[cpp-texturing] facewise={facewise}: vertex records {len(vrec)} "
+          f"face records {len(frec)}", flush=True)
+    assert len(vrec) > len(mesh.vertices) and len(frec) > len(mesh.vertex_indices)
+    assert _same(c_vrec, vrec)
+    assert _same(c_frec, frec)
+    assert _same(c_vstat, vstat)
+    assert np.array_equal(c_fstat, fstat)
+    for c, crit in enumerate([T.ViewSelectionCriteria.kMaxArea,
+                              T.ViewSelectionCriteria.kMinViewingAngle]):
+        mm = copy.copy(mesh)  # TextureMapping replaces the materials
+        opt = T.TextureMappingOption(criteria=crit, tex_w=512, tex_h=512)
+        assert T.TextureMapping(kfs, info, mm, opt)
+        tex = np.fromfile(p + f"tex{c}.u8", np.uint8).reshape(512, 512, 3)
+        assert np.array_equal(tex, mm.materials[0]), crit.name

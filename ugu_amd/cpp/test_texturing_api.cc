@@ -1,0 +1,175 @@
+// Test driver of the C++ texturing API (built by ugu_amd/cpp/Makefile, run by
+// tests/test_gpu_cpp_texturing.py). Used like UGU's texturing example
+// (examples/ex12_texturing.cc:63-151): LoadObj, keyframes rendered by the
+// Rasterizer, VisibilityTester::Test(keyframes, info), TextureMapping on the
+// mesh's own uv. Writes the keyframe images, every visibility record, the
+// statistics and the textures as raw files for the Python side to compare.
+//
+// usage: test_texturing_api <obj_path> <cams.f64> <ncam> <out_dir>
+// cams.f64: ncam x (W H fx fy cx cy c2w[16 row-major])
+
+#include <cstdio>
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ugu/log.h"
+#include "ugu/renderer/cpu/rasterizer.h"
+#include "ugu/texturing/texture_mapper.h"
+#include "ugu/texturing/visibility_tester.h"
+
+namespace {
+
+int g_fail = 0;
+void check(bool ok, const char* name) {
+  std::printf("CHECK %s %s\n", name, ok ? "ok" : "FAILED");
+  if (!ok) g_fail++;
+}
+
+template <typename T>
+void write_vec(const std::string& path, const std::vector<T>& v) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(v.data()), std::streamsize(v.size() * sizeof(T)));
+}
+
+template <typename T>
+void write_img(const std::string& path, const ugu::Image<T>& img) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(img.data), std::streamsize(img.total() * sizeof(T)));
+}
+
+// vertex records: vid kf_id r g b intensity px py angle distance
+// face records: fid kf_id area angle distance tri[6]
+// face statistics: max_area_index min_viewing_angle_index min_distance_index
+// vertex statistics: mean rgb, mean_distance rgb, min_viewing_angle_id, min_distance_id
+void dump_info(const ugu::VisibilityInfo& info, const std::string& prefix) {
+  std::vector<float> vrec, frec, vstat;
+  std::vector<int32_t> fstat;
+  for (size_t i = 0; i < info.vertex_info_list.size(); i++) {
+    const ugu::VertexInfo& vi = info.vertex_info_list[i];
+    for (const auto& k : vi.visible_keyframes) {
+      const float r[10] = {float(i), float(k.kf_id), k.color[0], k.color[1], k.color[2],
+                           k.intensity, k.projected_pos[0], k.projected_pos[1],
+                           k.viewing_angle, k.distance};
+      vrec.insert(vrec.end(), r, r + 10);
+    }
+    const bool any = !vi.visible_keyframes.empty();
+    const float s[8] = {any ? vi.mean_color[0] : 0.f, any ? vi.mean_color[1] : 0.f,
+                        any ? vi.mean_color[2] : 0.f, any ? vi.mean_distance_color[0] : 0.f,
+                        any ? vi.mean_distance_color[1] : 0.f,
+                        any ? vi.mean_distance_color[2] : 0.f,
+                        float(vi.min_viewing_angle_id), float(vi.min_distance_id)};
+    vstat.insert(vstat.end(), s, s + 8);
+  }
+  for (size_t i = 0; i < info.face_info_list.size(); i++) {
+    const ugu::FaceInfo& fi = info.face_info_list[i];
+    for (const auto& k : fi.visible_keyframes) {
+      const float r[11] = {float(k.face_id), float(k.kf_id), k.area, k.viewing_angle, k.distance,
+                           k.projected_tri[0][0], k.projected_tri[0][1], k.projected_tri[1][0],
+                           k.projected_tri[1][1], k.projected_tri[2][0], k.projected_tri[2][1]};
+      frec.insert(frec.end(), r, r + 11);
+    }
+    fstat.push_back(fi.max_area_index);
+    fstat.push_back(fi.min_viewing_angle_index);
+    fstat.push_back(fi.min_distance_index);
+  }
+  write_vec(prefix + "vrec.f32", vrec);
+  write_vec(prefix + "frec.f32", frec);
+  write_vec(prefix + "vstat.f32", vstat);
+  write_vec(prefix + "fstat.i32", fstat);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 5) return 2;
+  const std::string obj = argv[1], cam_path = argv[2], out = argv[4];
+  const int ncam = std::atoi(argv[3]);
+  std::vector<double> cams(size_t(ncam) * 22);
+  {
+    std::ifstream f(cam_path, std::ios::binary);
+    f.read(reinterpret_cast<char*>(cams.data()), std::streamsize(cams.size() * sizeof(double)));
+    if (!f) return 3;
+  }
+  auto mesh = ugu::Mesh::Create();
+  check(mesh->LoadObj(obj), "load_obj");
+
+  ugu::RendererCpuOption ropt;
+  ropt.diffuse_color = ugu::DiffuseColor::kTexture;
+  ugu::Rasterizer raster(ropt);
+  raster.set_mesh(mesh);
+  check(raster.PrepareMesh(), "prepare_mesh");
+  std::vector<std::shared_ptr<ugu::Keyframe>> keyframes;
+  for (int c = 0; c < ncam; c++) {
+    const double* p = &cams[size_t(c) * 22];
+    Eigen::Affine3d c2w = Eigen::Affine3d::Identity();
+    for (int r = 0; r < 4; r++)
+      for (int k = 0; k < 4; k++) c2w.matrix()(r, k) = p[6 + r * 4 + k];
+    auto cam = std::make_shared<ugu::PinholeCamera>(
+        int(p[0]), int(p[1]), c2w, Eigen::Vector2f(float(p[4]), float(p[5])),
+        Eigen::Vector2f(float(p[2]), float(p[3])));
+    auto kf = ugu::Keyframe::Create();
+    kf->id = c;
+    kf->camera = cam;
+    raster.set_camera(cam);
+    check(raster.Render(&kf->color, &kf->depth, nullptr, &kf->mask, nullptr), "render_keyframe");
+    const std::string k = out + "/kf" + std::to_string(c) + "_";
+    write_img(k + "color.u8", kf->color);
+    write_img(k + "depth.f32", kf->depth);
+    write_img(k + "mask.u8", kf->mask);
+    keyframes.push_back(kf);
+  }
+
+  // ---- error conventions (visibility_tester.cc:331-389, texture_mapper.cc:1009-1017)
+  ugu::set_log_level(ugu::LogLevel::kNone);
+  {
+    ugu::VisibilityTester vt;
+    ugu::VisibilityInfo info(*mesh);
+    check(!vt.Test(&info), "test_without_keyframe_fails");
+    vt.set_keyframe(keyframes[0]);
+    check(!vt.Test(&info), "test_without_mesh_fails");
+    check(!vt.PrepareData(), "prepare_empty_mesh_fails");
+    vt.set_data(mesh->vertices(), mesh->normals(), mesh->vertex_indices());
+    check(vt.PrepareData(), "prepare_data");
+    ugu::VisibilityInfo wrong;
+    check(!vt.Test(&wrong), "test_info_size_mismatch_fails");
+    auto nomask = ugu::Keyframe::Create();
+    *nomask = *keyframes[0];
+    nomask->mask = ugu::Image1b();
+    vt.set_keyframe(nomask);
+    check(!vt.Test(&info), "test_use_mask_without_mask_fails");
+    ugu::TextureMappingOption topt;  // default uv type generates an atlas
+    ugu::Mesh m2 = *mesh;
+    check(!ugu::TextureMapping(keyframes, info, &m2, topt), "atlas_uv_type_not_provided");
+  }
+  ugu::set_log_level(ugu::LogLevel::kError);
+
+  for (int facewise = 0; facewise < 2; facewise++) {
+    ugu::VisibilityTester vt;
+    vt.set_data(mesh->vertices(), mesh->normals(), mesh->vertex_indices());
+    check(vt.PrepareData(), "prepare_data");
+    ugu::VisibilityInfo info(*mesh);
+    check(vt.Test(keyframes, &info, facewise != 0), "visibility_test");
+    check(info.has_vertex_stat && info.has_face_stat, "statistics_computed");
+    const std::string p = out + (facewise ? "/fw_" : "/vw_");
+    dump_info(info, p);
+    const ugu::ViewSelectionCriteria crit[2] = {ugu::ViewSelectionCriteria::kMaxArea,
+                                                ugu::ViewSelectionCriteria::kMinViewingAngle};
+    for (int c = 0; c < 2; c++) {
+      ugu::TextureMappingOption topt;
+      topt.uv_type = ugu::TexturingOutputUvType::kUseOriginalMeshUv;
+      topt.criteria = crit[c];
+      topt.tex_w = 512;
+      topt.tex_h = 512;
+      ugu::Mesh m2 = *mesh;
+      check(ugu::TextureMapping(keyframes, info, &m2, topt), "texture_mapping");
+      check(m2.materials().size() == 1 && m2.materials()[0].name == "ugutex" &&
+                m2.material_ids().size() == m2.vertex_indices().size(),
+            "texture_material_set");
+      write_img(p + "tex" + std::to_string(c) + ".u8", m2.materials()[0].diffuse_tex);
+    }
+  }
+  std::printf("FAILURES %d\n", g_fail);
+  return g_fail == 0 ? 0 : 1;
+}
