@@ -17,6 +17,17 @@ namespace ugu_hip {
 
 #define UGU_DEV __device__ __forceinline__
 
+// Output pointers read from descriptors in memory are generic to the
+// compiler, which then emits flat stores (counted against both the vector
+// memory and the LDS wait counters). Casting to the global address space
+// gives plain global stores.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+UGU_DEV gptr<T> to_global(T* p) {
+  return (gptr<T>)p;
+}
+
 // ---------------------------------------------------------------------------
 // scalar helpers
 

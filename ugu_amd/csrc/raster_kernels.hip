@@ -38,8 +38,10 @@ UGU_DEV float4 project_point(const DevView& v, float px, float py, float pz) {
   float qy = sum3(v.w2c_R[3] * px, v.w2c_R[4] * py, v.w2c_R[5] * pz) + v.w2c_t[1];
   float qz = sum3(v.w2c_R[6] * px, v.w2c_R[7] * py, v.w2c_R[8] * pz) + v.w2c_t[2];
   // fx / qz and fy / qz are the same correctly rounded quotient when fx == fy
+  // (a wave-uniform branch: the second divide is not executed then)
   const float sx = v.fx / qz;
-  const float sy = v.fy == v.fx ? sx : v.fy / qz;
+  float sy = sx;
+  if (__builtin_expect(v.fy != v.fx, 0)) sy = v.fy / qz;
   float4 r;
   r.x = sx * qx + v.cx;
   r.y = sy * qy + v.cy;
@@ -196,30 +198,46 @@ __global__ void k_face_soup(const float* __restrict__ verts, const int32_t* __re
 // ---------------------------------------------------------------------------
 // k_setup_bin: one thread per (face, view)
 
-// Appends `entry` to a bin segment for every lane with `has`. `seg` = 4 * tile
-// for the front segment (filled upwards from the bin's first slot) and
-// 4 * tile + 1 for the back segment (filled downwards from its last slot).
-// Lanes that target the same segment form a group (ballot + popcount rank);
-// the group leaders then issue their atomicAdds together, so a wavefront
-// waits for one atomic round trip per call however many segments it hits.
-// When the two segments together overflow the bin they overwrite each
-// other's slots; every slot the rasterizer reads still holds a valid entry,
-// and the host renders again with larger bins.
-UGU_DEV void wave_append(const DevBins& bins, int seg, int2 entry, bool has) {
-  uint64_t todo = __ballot(has);
-  if (!todo) return;
-  const int lane = __lane_id();
-  // Lower bound of the skip depths of this wavefront's back-segment entries
-  // (0 when one has no bound): the leaders of back groups fold it into their
-  // tile's back-segment minimum.
-  const bool back = has && (seg & 1);
+// Appends `entry` to a bin segment for every lane with `has`, in two halves.
+// `seg` = 4 * tile for the front segment (filled upwards from the bin's first
+// slot) and 4 * tile + 1 for the back segment (filled downwards from its last
+// slot). append_issue groups the lanes that target the same segment (ballot +
+// popcount rank) and the group leaders issue their atomicAdds together;
+// append_finish broadcasts each group's base and stores the entries. A caller
+// issues several appends before finishing any, so the wavefront waits for
+// one atomic round trip for all of them. Entries of one segment are in no
+// particular order (the rasterizer's result does not depend on it). When the
+// two segments together overflow the bin they overwrite each other's slots;
+// every slot the rasterizer reads still holds a valid entry, and the host
+// renders again with larger bins.
+struct AppendTicket {
+  int ret;   // the group's first slot, in the leader lane
+  int lr;    // leader lane | rank in the group << 8
+  int seg;   // segment, -1 for lanes without an entry
+};
+
+// Lower bound of the skip depths of a wavefront's back-face entries (bits of
+// a non-negative float; 0 when one has no bound): the leaders of back groups
+// fold it into their tile's back-segment minimum.
+UGU_DEV uint32_t back_zmin(bool back, float zskip) {
   uint32_t zmin = 0xffffffffu;
   if (__ballot(back)) {
-    const float zs = __int_as_float(entry.y);
-    zmin = back ? (zs >= 0.0f ? __float_as_uint(zs) : 0u) : 0xffffffffu;
+    zmin = back ? (zskip >= 0.0f ? __float_as_uint(zskip) : 0u) : 0xffffffffu;
     for (int off = 32; off > 0; off >>= 1)
       zmin = min(zmin, static_cast<uint32_t>(__shfl_xor(static_cast<int>(zmin), off)));
   }
+  return zmin;
+}
+
+UGU_DEV AppendTicket append_issue(const DevBins& bins, int seg, uint32_t zmin, bool has) {
+  AppendTicket tk;
+  tk.ret = 0;
+  tk.lr = 0;
+  tk.seg = has ? seg : -1;
+  uint64_t todo = __ballot(has);
+  if (!todo) return tk;
+  const int lane = __lane_id();
+  const bool back = has && (seg & 1);
   int leader = 0, rank = 0, size = 0;
   while (todo) {
     const int l = __ffsll(static_cast<unsigned long long>(todo)) - 1;
@@ -233,17 +251,27 @@ UGU_DEV void wave_append(const DevBins& bins, int seg, int2 entry, bool has) {
     if (lane == l) size = __popcll(grp);
     todo &= ~grp;
   }
-  int base = 0;
   if (has && lane == leader) {
-    base = atomicAdd(&bins.counts[seg], size);
+    tk.ret = atomicAdd(&bins.counts[seg], size);
     if (back) atomicMin(reinterpret_cast<unsigned int*>(bins.counts) + (seg + 1), zmin);
   }
-  base = __shfl(base, leader);
-  if (has) {
-    const int slot = base + rank;
-    if (slot < bins.cap) {
-      const int64_t row = int64_t(seg >> 2) * bins.cap;
-      bins.bins[row + ((seg & 1) ? bins.cap - 1 - slot : slot)] = entry;
+  tk.lr = leader | (rank << 8);
+  return tk;
+}
+
+// Finishes R tickets: every group base is broadcast before the first store,
+// so the wavefront waits once for all R atomics (a store issued before a
+// later wait would be waited for as well).
+template <int R>
+UGU_DEV void append_finish(const DevBins& bins, const AppendTicket* tk, int2 entry) {
+  int slot[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) slot[r] = __shfl(tk[r].ret, tk[r].lr & 0xff) + (tk[r].lr >> 8);
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    if (tk[r].seg >= 0 && slot[r] < bins.cap) {
+      const int64_t row = int64_t(tk[r].seg >> 2) * bins.cap;
+      bins.bins[row + ((tk[r].seg & 1) ? bins.cap - 1 - slot[r] : slot[r])] = entry;
     }
   }
 }
@@ -284,6 +312,11 @@ UGU_DEV bool skip_face(float zskip, float dmax) {
 // Views per thread: the face's indices are loaded once and reused for
 // kSetupViews consecutive views (grid.y = ceil(n_views / kSetupViews)).
 constexpr int kSetupViews = 8;
+
+// Bin appends a setup thread issues before waiting for their atomics: a face
+// whose pixel bbox spans up to this many tiles costs one atomic round trip
+// per view.
+constexpr int kAppendRounds = 4;
 
 // Chunks whose faces all have at most this many (face, pixel) pairs expand
 // them through an LDS pair list; otherwise by binary search over the prefix.
@@ -365,15 +398,32 @@ __global__ __launch_bounds__(256) void k_setup_bin(
         ty1 = y1 / TH;
       }
     }
+    // one append per tile of the face's pixel bbox: up to kAppendRounds
+    // rounds are issued before any is finished, further rounds (faces over
+    // more tiles) one at a time
+    const int2 entry = make_int2(f, __float_as_int(zskip));
+    // one bound for all rounds: the minimum over every back face of the
+    // wavefront is also below the skip depth of each back face of a round
+    const uint32_t zmin = back_zmin(has && back, zskip);
     int tx = tx0, ty = ty0;
+    AppendTicket tk[kAppendRounds];
+#pragma unroll
+    for (int r = 0; r < kAppendRounds; r++) {
+      const int seg = has ? 4 * (v.tile_base + ty * v.tiles_x + tx) + (back ? 1 : 0) : -1;
+      tk[r] = append_issue(bins, seg, zmin, has);
+      if (has && ++tx > tx1) {
+        tx = tx0;
+        if (++ty > ty1) has = false;
+      }
+    }
+    append_finish<kAppendRounds>(bins, tk, entry);
     while (__any(has)) {
       const int seg = has ? 4 * (v.tile_base + ty * v.tiles_x + tx) + (back ? 1 : 0) : -1;
-      wave_append(bins, seg, make_int2(f, __float_as_int(zskip)), has);
-      if (has) {
-        if (++tx > tx1) {
-          tx = tx0;
-          if (++ty > ty1) has = false;
-        }
+      const AppendTicket t1 = append_issue(bins, seg, zmin, has);
+      append_finish<1>(bins, &t1, entry);
+      if (has && ++tx > tx1) {
+        tx = tx0;
+        if (++ty > ty1) has = false;
       }
     }
   }
@@ -599,26 +649,26 @@ UGU_DEV void write_gbuffer(const DevView& v, int64_t pidx, bool covered, float u
     n0 = n1 = n2 = 0.0f;
   }
   if (v.bary) {
-    float* o = v.bary + pidx * 3;
+    gptr<float> o = to_global(v.bary + pidx * 3);
     o[0] = b[0]; o[1] = b[1]; o[2] = b[2];
   }
   if (v.uvo) {
-    float* o = v.uvo + pidx * 3;
+    gptr<float> o = to_global(v.uvo + pidx * 3);
     o[0] = t[0]; o[1] = t[1]; o[2] = t[2];
   }
   if (v.pos_wld) {
-    float* o = v.pos_wld + pidx * 3;
+    gptr<float> o = to_global(v.pos_wld + pidx * 3);
     o[0] = pw[0]; o[1] = pw[1]; o[2] = pw[2];
   }
   if (v.pos_cam) {
-    float* o = v.pos_cam + pidx * 3;
+    gptr<float> o = to_global(v.pos_cam + pidx * 3);
     o[0] = pc[0]; o[1] = pc[1]; o[2] = pc[2];
   }
   if (v.nrm_wld) {
-    float* o = v.nrm_wld + pidx * 3;
+    gptr<float> o = to_global(v.nrm_wld + pidx * 3);
     o[0] = n0; o[1] = n1; o[2] = n2;
   }
-  if (v.geo_id) v.geo_id[pidx] = covered ? 0 : -1;
+  if (v.geo_id) to_global(v.geo_id)[pidx] = covered ? 0 : -1;
 }
 
 template <int TW, int TH, int MODE>
@@ -647,21 +697,21 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
       const int64_t pidx = int64_t(gy) * v.W + gx;
       if (MODE == 0) {
         if (v.color) {
-          uint8_t* cp = v.color + pidx * 3;
+          gptr<uint8_t> cp = to_global(v.color + pidx * 3);
           cp[0] = cp[1] = cp[2] = 0;
         }
-        if (v.depth) v.depth[pidx] = 0.0f;
+        if (v.depth) to_global(v.depth)[pidx] = 0.0f;
         if (v.normal) {
-          float* np = v.normal + pidx * 3;
+          gptr<float> np = to_global(v.normal + pidx * 3);
           np[0] = np[1] = np[2] = 0.0f;
         }
-        if (v.mask) v.mask[pidx] = 0;
+        if (v.mask) to_global(v.mask)[pidx] = 0;
         write_gbuffer(v, pidx, false, 0.0f, 0.0f, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
       } else {
-        float* ip = v.image + pidx * 3;
+        gptr<float> ip = to_global(v.image + pidx * 3);
         ip[0] = ip[1] = ip[2] = 0.0f;
       }
-      if (v.face_id) v.face_id[pidx] = -1;
+      if (v.face_id) to_global(v.face_id)[pidx] = -1;
     }
     return;
   }
@@ -947,20 +997,20 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
       write_gbuffer(v, pidx, false, 0.0f, 0.0f, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
     }
     if (v.color) {
-      uint8_t* cp = v.color + pidx * 3;
+      gptr<uint8_t> cp = to_global(v.color + pidx * 3);
       cp[0] = col[0];
       cp[1] = col[1];
       cp[2] = col[2];
     }
-    if (v.depth) v.depth[pidx] = z;
+    if (v.depth) to_global(v.depth)[pidx] = z;
     if (v.normal) {
-      float* np = v.normal + pidx * 3;
+      gptr<float> np = to_global(v.normal + pidx * 3);
       np[0] = nrm[0];
       np[1] = nrm[1];
       np[2] = nrm[2];
     }
-    if (v.mask) v.mask[pidx] = msk;
-    if (v.face_id) v.face_id[pidx] = fid;
+    if (v.mask) to_global(v.mask)[pidx] = msk;
+    if (v.face_id) to_global(v.face_id)[pidx] = fid;
   } else {
     // differentiable forward: image = colour term * texture term (float64)
     float img[3] = {0.0f, 0.0f, 0.0f};
@@ -969,11 +1019,11 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
       diff_eval(v, mesh, fid, gx, gy, mesh.faces + int64_t(fid) * 3, e);
       for (int k = 0; k < 3; k++) img[k] = static_cast<float>(e.c[k] * e.t[k]);
     }
-    float* ip = v.image + pidx * 3;
+    gptr<float> ip = to_global(v.image + pidx * 3);
     ip[0] = img[0];
     ip[1] = img[1];
     ip[2] = img[2];
-    if (v.face_id) v.face_id[pidx] = fid;
+    if (v.face_id) to_global(v.face_id)[pidx] = fid;
   }
 }
 
