@@ -149,6 +149,20 @@ def test_multiview_batched_equals_per_view_and_oracle(gpu_available, ico_scene):
         for i in (0, 31, 63):
             assert np.array_equal(bufs[i]["face_id"].cpu().numpy(), batched[i]["face_id"])
             assert np.array_equal(bufs[i]["depth"].cpu().numpy(), batched[i]["depth"])
+    # the bound entry: same bits; a pose changed on the binding is rendered
+    binding = r.bind_device(cams, bufs)
+    for b in bufs:
+        b["face_id"].fill_(-7)
+    r.render_bound(binding)
+    for i in (0, 31, 63):
+        assert np.array_equal(bufs[i]["face_id"].cpu().numpy(), batched[i]["face_id"])
+    assert (cams[5].width, cams[5].height) == (cams[10].width, cams[10].height)
+    binding.set_camera(5, cams[10])
+    r.render_bound(binding)
+    assert np.array_equal(bufs[5]["face_id"].cpu().numpy(), batched[10]["face_id"])
+    assert np.array_equal(bufs[5]["depth"].cpu().numpy(), batched[10]["depth"])
+    with pytest.raises(ValueError):
+        binding.set_camera(5, cams[1])  # another image size than the bound tensors
 
 
 @pytest.mark.parametrize("tile,cull", [((16, 16), True), ((8, 8), False), ((32, 8), True)])

@@ -340,6 +340,52 @@ class Rasterizer:
                                                graph=False))
 
 
+    def bind_device(self, cameras, buffers):
+        """Marshal `cameras` and the device `buffers` of render_device once,
+        for a loop that renders into the same tensors again and again (a
+        serving loop): render_bound(binding) then only crosses into the
+        library. set_camera(i, camera) on the binding changes a pose; the
+        tensors must stay alive and unmoved while the binding is used."""
+        return DeviceBinding(self, cameras, buffers)
+
+    def render_bound(self, binding, graph=True):
+        """render_device for a DeviceBinding (same results, no per-call
+        marshalling of arguments)."""
+        self._check(self._render_views(binding.cams, binding.n, ctypes.byref(binding.opt),
+                                       binding.outs, _abi.UGU_HIP_MEMORY_DEVICE, graph=graph))
+
+
+class DeviceBinding:
+    """Camera, option and output-pointer arrays of one render_device call,
+    built once (Rasterizer.bind_device). The option struct is taken when the
+    binding is made."""
+
+    def __init__(self, renderer, cameras, buffers):
+        if len(cameras) != len(buffers) or not cameras:
+            raise ValueError("one buffer dict per camera, at least one camera")
+        self.n = len(cameras)
+        self.cams = (_abi.UguCamera * self.n)(
+            *[c if isinstance(c, _abi.UguCamera) else c.to_abi() for c in cameras])
+        self.sizes = [(c.width, c.height) for c in self.cams]
+        for (w, h), b in zip(self.sizes, buffers):
+            for k, t in b.items():
+                if t is not None and tuple(t.shape[:2]) != (h, w):
+                    raise ValueError(f"buffer {k} has shape {tuple(t.shape)} for a {w}x{h} view")
+        self.opt = options_to_abi(renderer.option_)
+        self.outs = (_abi.UguOutputs * self.n)(*[
+            _abi.UguOutputs(*[(b[k].data_ptr() if k in b and b[k] is not None else None)
+                              for k in ("color", "depth", "normal", "mask", "face_id")])
+            for b in buffers])
+        self._buffers = buffers  # keeps the tensors alive
+
+    def set_camera(self, i, camera):
+        c = camera if isinstance(camera, _abi.UguCamera) else camera.to_abi()
+        if (c.width, c.height) != self.sizes[i]:
+            raise ValueError(f"view {i} is bound to {self.sizes[i][0]}x{self.sizes[i][1]} "
+                             f"buffers, camera is {c.width}x{c.height}")
+        self.cams[i] = c
+
+
 class Raytracer(Rasterizer):
     """ugu::Raytracer (src/renderer/cpu/raytracer.cc) on the gfx950 raytracer
     kernel: PrepareMesh also builds the bounding volume hierarchy
