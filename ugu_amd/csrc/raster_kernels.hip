@@ -114,21 +114,20 @@ UGU_DEV bool cover_test(float ax, float ay, float az, float bx, float by,
 // k_clear
 
 // Four words per tile: front and back segment counters and the minimum skip
-// depth of the back segment. Also fills the tile -> view table (largest v with tile_base[v] <= tile),
-// so the rasterizer finds its view with one load instead of a search.
+// depth of the back segment. The fourth word holds the tile's view (largest v
+// with tile_base[v] <= tile), so the rasterizer reads its counters and finds
+// its view with one load.
 __global__ void k_clear(int32_t* counts, int32_t n, int32_t* status,
-                        const int32_t* __restrict__ tile_base, int32_t n_views,
-                        int32_t* __restrict__ tile_view) {
+                        const int32_t* __restrict__ tile_base, int32_t n_views) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    // front count, back count, back-segment skip-depth minimum (bits; +inf)
-    reinterpret_cast<int4*>(counts)[i] = make_int4(0, 0, 0x7f800000, 0);
     int lo = 0, hi = n_views - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (tile_base[mid] <= i) lo = mid; else hi = mid - 1;
     }
-    tile_view[i] = lo;
+    // front count, back count, back-segment skip-depth minimum (bits; +inf), view
+    reinterpret_cast<int4*>(counts)[i] = make_int4(0, 0, 0x7f800000, lo);
   }
   if (i < 4) status[i] = 0;
 }
@@ -678,9 +677,10 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   constexpr int NT = TW * TH;
   const int t = threadIdx.x;
 
+  // counters and view in one load
   const int4 seg = reinterpret_cast<const int4*>(bins.counts)[gtile];
   const int count = seg.x + seg.y;
-  const DevView& v = views[bins.tile_view[gtile]];
+  const DevView& v = views[seg.w];
   const int lt = gtile - v.tile_base;
   const int tyi = lt / v.tiles_x;
   const int txi = lt - tyi * v.tiles_x;
@@ -1043,16 +1043,26 @@ __global__ __launch_bounds__(TW * TH, 6) void k_raster(
   }
   // persistent: pull runs of kQueueRun consecutive (view, tile) items until
   // the queue is drained; every block exits once the head passes total_tiles.
+  // The next run is claimed when the last tile of the current one starts, so
+  // the atomic's round trip overlaps that tile. The barrier before reading
+  // s.item publishes thread 0's write (an empty tile returns from raster_tile
+  // without a barrier); the one after keeps thread 0 from overwriting it
+  // before every thread has read it. `item` is the same in every thread, so
+  // every thread leaves the loop together.
   constexpr int kQueueRun = 4;
-  for (;;) {
-    if (threadIdx.x == 0) s.item = atomicAdd(&bins.status[1], kQueueRun);
-    __syncthreads();
-    const int item = s.item;
-    __syncthreads();
-    if (item >= bins.total_tiles) break;
+  if (threadIdx.x == 0) s.item = atomicAdd(&bins.status[1], kQueueRun);
+  __syncthreads();
+  int item = s.item;
+  __syncthreads();
+  while (item < bins.total_tiles) {
     const int end = min(item + kQueueRun, bins.total_tiles);
-    for (int it = item; it < end; it++)
+    for (int it = item; it < end; it++) {
+      if (it == end - 1 && threadIdx.x == 0) s.item = atomicAdd(&bins.status[1], kQueueRun);
       raster_tile<TW, TH, MODE>(s, it, views, mesh, proj, bins, opt);
+    }
+    __syncthreads();
+    item = s.item;
+    __syncthreads();
   }
 }
 
@@ -1356,10 +1366,10 @@ __global__ void k_depth_to_u16(const float* __restrict__ d, int64_t n,
 // host-side launchers (called from ugu_hip.cpp)
 
 int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status,
-                 const int32_t* tile_base, int32_t n_views, int32_t* tile_view) {
+                 const int32_t* tile_base, int32_t n_views) {
   int m = n > 4 ? n : 4;
   hipLaunchKernelGGL(k_clear, dim3((m + 255) / 256), dim3(256), 0, st, counts, n, status,
-                     tile_base, n_views, tile_view);
+                     tile_base, n_views);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -104,7 +104,7 @@ struct DevVisOpt {
 
 struct DevBins {
   int32_t* counts;   // per global tile: front / back segment counters, minimum
-                     // skip depth of the back segment (float bits), pad
+                     // skip depth of the back segment (float bits), view index
   int2* bins;        // global tile t owns bins[t*cap .. t*cap+cap): front
                      // segment from the start, back segment from the end;
                      // entry = (face id, bits of its skip depth)
@@ -112,7 +112,6 @@ struct DevBins {
   int32_t total_tiles;
   int32_t* status;   // [0] max bin count seen, [1] work-queue head
   const int32_t* tile_base;  // n_views + 1 prefix of tiles per view
-  int32_t* tile_view;        // view of every global tile (filled by k_clear)
   int32_t n_views;
 };
 

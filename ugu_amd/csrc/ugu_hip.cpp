@@ -19,7 +19,7 @@
 
 namespace ugu_hip {
 int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status,
-                 const int32_t* tile_base, int32_t n_views, int32_t* tile_view);
+                 const int32_t* tile_base, int32_t n_views);
 int launch_face_soup(hipStream_t st, const float* verts, const int32_t* faces,
                      const float* fnormals, int32_t F, int compute_normals, float4* soup,
                      float* fn_out);
@@ -113,7 +113,7 @@ struct ugu_hip_ctx {
       texinfo, texdata;
 
   // work buffers
-  DevBuf proj, counts, tile_view, bins, status, views, tile_base, stage_out, diff_fn;
+  DevBuf proj, counts, bins, status, views, tile_base, stage_out, diff_fn;
   DevBuf bwd_items;  // (face, view, pixel) items of the backward pass, 16 B each
   DevBuf bwd_scratch;  // per-face counts and offsets of the backward pass
   DevBuf soup, diff_soup;  // 48-byte face records (prepared mesh / differentiable path)
@@ -437,7 +437,6 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
   int err;
   if ((err = ensure(c, c->proj, 16))) return err;
   if ((err = ensure(c, c->counts, size_t(total_tiles) * 16))) return err;  // 4 per tile
-  if ((err = ensure(c, c->tile_view, size_t(total_tiles) * 4))) return err;
   if ((err = ensure(c, c->status, 64))) return err;
   if ((err = ensure(c, c->views, size_t(n) * sizeof(DevView)))) return err;
   if ((err = ensure(c, c->tile_base, size_t(n + 1) * 4))) return err;
@@ -504,7 +503,6 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
     b.total_tiles = total_tiles;
     b.status = c->status.as<int32_t>();
     b.tile_base = c->tile_base.as<int32_t>();
-    b.tile_view = c->tile_view.as<int32_t>();
     b.n_views = n;
     const DevView* dv = c->views.as<DevView>();
     float4* proj = c->proj.as<float4>();
@@ -517,7 +515,7 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
 
     auto launch_all = [&](bool timed) -> int {
       if (timed) (void)hipEventRecord(c->ev[0], c->stream);
-      if (launch_clear(c->stream, b.counts, total_tiles, b.status, b.tile_base, n, b.tile_view))
+      if (launch_clear(c->stream, b.counts, total_tiles, b.status, b.tile_base, n))
         return -1;
       if (timed) (void)hipEventRecord(c->ev[1], c->stream);
       if (launch_setup_bin(c->stream, tw, th, dv, n, dm, proj, b)) return -1;
@@ -795,7 +793,7 @@ void ugu_hip_destroy(ugu_hip_ctx* c) {
   DevBuf* bufs[] = {&c->verts, &c->faces, &c->faces_orig, &c->fnormals, &c->normals, &c->nidx,
                     &c->vcolors, &c->uv, &c->uvidx, &c->matid, &c->texinfo,
                     &c->texdata, &c->proj, &c->counts, &c->bins, &c->status,
-                    &c->views, &c->tile_base, &c->stage_out, &c->diff_fn, &c->tile_view,
+                    &c->views, &c->tile_base, &c->stage_out, &c->diff_fn,
                     &c->bwd_items, &c->bwd_scratch, &c->soup, &c->diff_soup, &c->blk_off,
                     &c->blk_vid, &c->face_loc, &c->blk_off_o, &c->blk_vid_o, &c->face_loc_o,
                     &c->bvh_nodes, &c->bvh_leaves, &c->bvh_tris, &c->vis_items,
