@@ -105,11 +105,57 @@ UGU_DEV uchar4 tex_at(const uint8_t* base, int w, int h, int row, int col) {
   return reinterpret_cast<const uchar4*>(base)[lin];
 }
 
+// Shading inputs of one face: its corners' vertex normals and uvs and its
+// material. With the mesh's shading records (one 64-byte record per face,
+// built at PrepareMesh) they are one load; otherwise gathered through the
+// index arrays (normal_indices -> normals, uv_indices -> uv, material_ids).
+struct FaceShade {
+  float na[3], nb[3], nc[3];
+  float ua[2], ub[2], uc[2];
+  int mat;
+};
+
+UGU_DEV void face_shade(const DevMesh& m, int fid, bool need_n, bool need_uv, FaceShade& s) {
+  if (m.shade) {
+    const float4* r = m.shade + int64_t(fid) * 4;
+    const float4 a = r[0], b = r[1], c = r[2], d = r[3];
+    s.ua[0] = a.x; s.ua[1] = a.y; s.ub[0] = a.z; s.ub[1] = a.w;
+    s.uc[0] = b.x; s.uc[1] = b.y;
+    s.na[0] = b.z; s.na[1] = b.w; s.na[2] = c.x;
+    s.nb[0] = c.y; s.nb[1] = c.z; s.nb[2] = c.w;
+    s.nc[0] = d.x; s.nc[1] = d.y; s.nc[2] = d.z;
+    s.mat = __float_as_int(d.w);
+    return;
+  }
+  if (need_n) {
+    const int32_t* ni = m.nidx + int64_t(fid) * 3;
+    const float* na = m.normals + int64_t(ni[0]) * 3;
+    const float* nb = m.normals + int64_t(ni[1]) * 3;
+    const float* nc = m.normals + int64_t(ni[2]) * 3;
+    for (int k = 0; k < 3; k++) {
+      s.na[k] = na[k];
+      s.nb[k] = nb[k];
+      s.nc[k] = nc[k];
+    }
+  }
+  if (need_uv) {
+    const int32_t* ui = m.uvidx + int64_t(fid) * 3;
+    const float* a = m.uv + int64_t(ui[0]) * 2;
+    const float* b = m.uv + int64_t(ui[1]) * 2;
+    const float* c = m.uv + int64_t(ui[2]) * 2;
+    s.ua[0] = a[0]; s.ua[1] = a[1];
+    s.ub[0] = b[0]; s.ub[1] = b[1];
+    s.uc[0] = c[0]; s.uc[1] = c[1];
+    s.mat = m.matid[fid];
+  }
+}
+
 // Colour of one shaded pixel of the reference path: colourizer then shader
-// (pixel_shader.h:215-218 with the classes at :222-422).
-UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, float u, float vv,
-                      float r0, float r1, float r2, float n0, float n1, float n2,
-                      uint8_t c[3]) {
+// (pixel_shader.h:215-218 with the classes at :222-422). `fs` holds the face's
+// uvs and material when the diffuse colour is the texture.
+UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, const FaceShade& fs,
+                      float u, float vv, float r0, float r1, float r2, float n0, float n1,
+                      float n2, uint8_t c[3]) {
   if (o.diffuse_color == 2) {
     // DiffuseVertexColorColorizer (pixel_shader.h:234-256)
     const int32_t* fi = m.faces + int64_t(fid) * 3;
@@ -119,12 +165,11 @@ UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, float u, f
     const float w = 1.0f - u - vv;
     for (int k = 0; k < 3; k++) c[k] = f2u8_x86(w * c0[k] + u * c1[k] + vv * c2[k]);
   } else if (o.diffuse_color == 1) {
-    const TexInfo t = m.tex[m.matid[fid]];
+    const TexInfo t = m.tex[fs.mat];
     const uint8_t* base = m.texdata + t.offset;
-    const int32_t* ui = m.uvidx + int64_t(fid) * 3;
-    const float* a = m.uv + int64_t(ui[0]) * 2;
-    const float* b = m.uv + int64_t(ui[1]) * 2;
-    const float* cc = m.uv + int64_t(ui[2]) * 2;
+    const float* a = fs.ua;
+    const float* b = fs.ub;
+    const float* cc = fs.uc;
     const float w = 1.0f - u - vv;
     const float iu = w * a[0] + u * b[0] + vv * cc[0];
     const float iv = w * a[1] + u * b[1] + vv * cc[1];

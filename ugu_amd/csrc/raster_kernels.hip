@@ -148,6 +148,36 @@ UGU_DEV void normalize3(float& x, float& y, float& z) {
 
 
 // ---------------------------------------------------------------------------
+// k_shade_rec: one 64-byte shading record per face (DevMesh::shade): the
+// corners' uvs (uv[uv_indices]) and vertex normals (normals[normal_indices])
+// and the material id, copied bit for bit (zero where the mesh has none), so
+// the resolve fetches a winner's shading inputs with one load.
+__global__ void k_shade_rec(DevMesh m, float4* __restrict__ out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.F) return;
+  float u[6] = {0, 0, 0, 0, 0, 0}, n[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (m.uv && m.uvidx)
+    for (int k = 0; k < 3; k++) {
+      const float* q = m.uv + int64_t(m.uvidx[int64_t(f) * 3 + k]) * 2;
+      u[2 * k] = q[0];
+      u[2 * k + 1] = q[1];
+    }
+  if (m.normals && m.nidx)
+    for (int k = 0; k < 3; k++) {
+      const float* q = m.normals + int64_t(m.nidx[int64_t(f) * 3 + k]) * 3;
+      n[3 * k] = q[0];
+      n[3 * k + 1] = q[1];
+      n[3 * k + 2] = q[2];
+    }
+  const int mat = m.matid ? m.matid[f] : 0;
+  float4* r = out + int64_t(f) * 4;
+  r[0] = make_float4(u[0], u[1], u[2], u[3]);
+  r[1] = make_float4(u[4], u[5], n[0], n[1]);
+  r[2] = make_float4(n[2], n[3], n[4], n[5]);
+  r[3] = make_float4(n[6], n[7], n[8], __int_as_float(mat));
+}
+
+// ---------------------------------------------------------------------------
 // k_face_soup: one 48-byte record per face (three vertex positions, then the
 // face normal) so setup and raster fetch a face with one dependent load
 // instead of an index load and three vertex gathers. compute_normals:
@@ -968,14 +998,15 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
     if (fid > 0) {
       msk = 255;
       float n0, n1, n2;
+      FaceShade fs;
+      face_shade(mesh, fid, opt.shading_normal != 0, v.color && opt.diffuse_color == 1, fs);
       if (opt.shading_normal == 0) {
         const float* fn = mesh.fnormals + int64_t(fid) * 3;
         n0 = fn[0]; n1 = fn[1]; n2 = fn[2];
       } else {
-        const int32_t* ni = mesh.nidx + int64_t(fid) * 3;
-        const float* na = mesh.normals + int64_t(ni[0]) * 3;
-        const float* nb = mesh.normals + int64_t(ni[1]) * 3;
-        const float* nc = mesh.normals + int64_t(ni[2]) * 3;
+        const float* na = fs.na;
+        const float* nb = fs.nb;
+        const float* nc = fs.nc;
         n0 = w0 * na[0] + w1 * nb[0] + w2 * nc[0];
         n1 = w0 * na[1] + w1 * nb[1] + w2 * nc[1];
         n2 = w0 * na[2] + w1 * nb[2] + w2 * nc[2];
@@ -985,7 +1016,7 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
         nrm[1] = sum3(v.w2c_R[3] * n0, v.w2c_R[4] * n1, v.w2c_R[5] * n2);
         nrm[2] = sum3(v.w2c_R[6] * n0, v.w2c_R[7] * n1, v.w2c_R[8] * n2);
       }
-      if (v.color) shade_u8(mesh, opt, fid, w1, w2, r0, r1, r2, n0, n1, n2, col);
+      if (v.color) shade_u8(mesh, opt, fid, fs, w1, w2, r0, r1, r2, n0, n1, n2, col);
       if (v.bary || v.uvo || v.pos_wld || v.pos_cam || v.nrm_wld || v.geo_id) {
         float P[9], N[3];
         const bool pos = v.pos_wld || v.pos_cam;
@@ -1370,6 +1401,12 @@ int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status,
   int m = n > 4 ? n : 4;
   hipLaunchKernelGGL(k_clear, dim3((m + 255) / 256), dim3(256), 0, st, counts, n, status,
                      tile_base, n_views);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_shade_rec(hipStream_t st, const DevMesh& m, float4* out) {
+  if (m.F <= 0) return 0;
+  hipLaunchKernelGGL(k_shade_rec, dim3((m.F + 255) / 256), dim3(256), 0, st, m, out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -61,6 +61,10 @@ struct DevMesh {
   const int32_t* blk_vid;   // distinct vertex ids of each block
   const uint32_t* face_loc; // per face: its corners' indices in the block's
                             // list, 10 bits each
+  const float4* shade;      // F x 4 or null: per-face shading record, the corners'
+                            // uvs and vertex normals and the material id, packed
+                            // (u0 v0 u1 v1)(u2 v2 n0x n0y)(n0z n1x n1y n1z)
+                            // (n2x n2y n2z material-id bits)
   const float* ftex;        // differentiable path: f32 RGB texture or null
   int32_t ftex_w, ftex_h;
 };

@@ -156,6 +156,8 @@ __global__ __launch_bounds__(64 * kRtWaves) void k_raytrace(const DevView* __res
     v.depth[pi] = zc * o.depth_scale;
   }
   float n0, n1, n2;
+  FaceShade fs;
+  face_shade(m, bf, o.shading_normal != 0, v.color && o.diffuse_color == 1, fs);
   if (o.shading_normal == 0) {
     const float* fn = m.fnormals + int64_t(bf) * 3;
     n0 = fn[0];
@@ -163,10 +165,9 @@ __global__ __launch_bounds__(64 * kRtWaves) void k_raytrace(const DevView* __res
     n2 = fn[2];
   } else {
     // (1 - u - v) n[i0] + u n[i1] + v n[i2] with the raytracer's (u, v)
-    const int32_t* ni = m.nidx + int64_t(bf) * 3;
-    const float* na = m.normals + int64_t(ni[0]) * 3;
-    const float* nb = m.normals + int64_t(ni[1]) * 3;
-    const float* nc = m.normals + int64_t(ni[2]) * 3;
+    const float* na = fs.na;
+    const float* nb = fs.nb;
+    const float* nc = fs.nc;
     const float w = 1.0f - bu - bv;
     n0 = w * na[0] + bu * nb[0] + bv * nc[0];
     n1 = w * na[1] + bu * nb[1] + bv * nc[1];
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(64 * kRtWaves) void k_raytrace(const DevView* __res
   }
   if (v.color) {
     uint8_t c[3];
-    shade_u8(m, o, bf, bu, bv, r0, r1, r2, n0, n1, n2, c);
+    shade_u8(m, o, bf, fs, bu, bv, r0, r1, r2, n0, n1, n2, c);
     v.color[pi * 3 + 0] = c[0];
     v.color[pi * 3 + 1] = c[1];
     v.color[pi * 3 + 2] = c[2];

@@ -20,6 +20,7 @@
 namespace ugu_hip {
 int launch_clear(hipStream_t st, int32_t* counts, int32_t n, int32_t* status,
                  const int32_t* tile_base, int32_t n_views);
+int launch_shade_rec(hipStream_t st, const DevMesh& m, float4* out);
 int launch_face_soup(hipStream_t st, const float* verts, const int32_t* faces,
                      const float* fnormals, int32_t F, int compute_normals, float4* soup,
                      float* fn_out);
@@ -117,6 +118,7 @@ struct ugu_hip_ctx {
   DevBuf bwd_items;  // (face, view, pixel) items of the backward pass, 16 B each
   DevBuf bwd_scratch;  // per-face counts and offsets of the backward pass
   DevBuf soup, diff_soup;  // 48-byte face records (prepared mesh / differentiable path)
+  DevBuf shade;            // 64-byte per-face shading records (DevMesh::shade)
   DevBuf blk_off, blk_vid, face_loc;           // setup blocks' vertex lists (prepared mesh)
   DevBuf blk_off_o, blk_vid_o, face_loc_o;     // the same for the original face array
   DevBuf bvh_nodes, bvh_leaves, bvh_tris;      // raytracer hierarchy
@@ -347,6 +349,7 @@ DevMesh dev_mesh(const ugu_hip_ctx* c) {
   m.faces = c->faces.as<int32_t>();
   m.fnormals = c->has_fn ? c->fnormals.as<float>() : nullptr;
   m.soup = c->soup.as<float4>();
+  m.shade = c->F > 0 ? c->shade.as<float4>() : nullptr;
   m.blk_off = c->F > 0 ? c->blk_off.as<int32_t>() : nullptr;
   m.blk_vid = c->blk_vid.as<int32_t>();
   m.face_loc = c->face_loc.as<uint32_t>();
@@ -794,7 +797,7 @@ void ugu_hip_destroy(ugu_hip_ctx* c) {
                     &c->vcolors, &c->uv, &c->uvidx, &c->matid, &c->texinfo,
                     &c->texdata, &c->proj, &c->counts, &c->bins, &c->status,
                     &c->views, &c->tile_base, &c->stage_out, &c->diff_fn,
-                    &c->bwd_items, &c->bwd_scratch, &c->soup, &c->diff_soup, &c->blk_off,
+                    &c->bwd_items, &c->bwd_scratch, &c->soup, &c->diff_soup, &c->shade, &c->blk_off,
                     &c->blk_vid, &c->face_loc, &c->blk_off_o, &c->blk_vid_o, &c->face_loc_o,
                     &c->bvh_nodes, &c->bvh_leaves, &c->bvh_tris, &c->vis_items,
                     &c->vis_images, &c->vis_out, &c->tm_faces, &c->tm_images, &c->tm_out};
@@ -940,6 +943,11 @@ int ugu_hip_prepare_mesh(ugu_hip_ctx* c, const UguMeshDesc* m) {
     if (launch_face_soup(c->stream, c->verts.as<float>(), c->faces.as<int32_t>(),
                          c->has_fn ? c->fnormals.as<float>() : nullptr, F, 0,
                          c->soup.as<float4>(), nullptr))
+      return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
+    if ((err = ensure(c, c->shade, size_t(F) * 64))) return err;
+    DevMesh dm = dev_mesh(c);
+    dm.shade = nullptr;
+    if (launch_shade_rec(c->stream, dm, c->shade.as<float4>()))
       return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
     UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   }
