@@ -165,7 +165,7 @@ UGU_DEV void shade_u8(const DevMesh& m, const DevOptions& o, int fid, const Face
     const float w = 1.0f - u - vv;
     for (int k = 0; k < 3; k++) c[k] = f2u8_x86(w * c0[k] + u * c1[k] + vv * c2[k]);
   } else if (o.diffuse_color == 1) {
-    const TexInfo t = m.tex[fs.mat];
+    const TexInfo t = m.n_tex == 1 ? m.tex0 : m.tex[fs.mat];
     const uint8_t* base = m.texdata + t.offset;
     const float* a = fs.ua;
     const float* b = fs.ub;

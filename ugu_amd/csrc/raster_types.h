@@ -52,6 +52,8 @@ struct DevMesh {
   const int32_t* uvidx;     // F x 3 or null
   const int32_t* matid;     // F or null
   const TexInfo* tex;       // per material
+  TexInfo tex0;             // the first material's entry (by value: a one-material
+  int32_t n_tex;            // mesh needs no per-pixel load of it); materials
   const uint8_t* texdata;   // u8 RGB texture pool
   const float4* soup;       // F x 3: the face's three vertex positions and its
                             // face normal (zero when the mesh has none),

@@ -119,6 +119,7 @@ struct ugu_hip_ctx {
   DevBuf bwd_scratch;  // per-face counts and offsets of the backward pass
   DevBuf soup, diff_soup;  // 48-byte face records (prepared mesh / differentiable path)
   DevBuf shade;            // 64-byte per-face shading records (DevMesh::shade)
+  TexInfo tex0{};          // host copy of the first material's TexInfo
   DevBuf blk_off, blk_vid, face_loc;           // setup blocks' vertex lists (prepared mesh)
   DevBuf blk_off_o, blk_vid_o, face_loc_o;     // the same for the original face array
   DevBuf bvh_nodes, bvh_leaves, bvh_tris;      // raytracer hierarchy
@@ -360,6 +361,8 @@ DevMesh dev_mesh(const ugu_hip_ctx* c) {
   m.uvidx = c->has_uvidx ? c->uvidx.as<int32_t>() : nullptr;
   m.matid = c->has_matid ? c->matid.as<int32_t>() : nullptr;
   m.tex = c->M > 0 ? c->texinfo.as<TexInfo>() : nullptr;
+  m.tex0 = c->tex0;
+  m.n_tex = c->M;
   m.texdata = c->M > 0 ? c->texdata.as<uint8_t>() : nullptr;
   m.ftex = nullptr;
   m.ftex_w = m.ftex_h = 0;
@@ -904,6 +907,7 @@ int ugu_hip_prepare_mesh(ugu_hip_ctx* c, const UguMeshDesc* m) {
     ti[i].h = empty ? 1 : t.height;
     pool += empty ? 256 : (int64_t(t.width) * t.height * 4 + 255) / 256 * 256;
   }
+  c->tex0 = c->M > 0 ? ti[0] : TexInfo{};
   if (c->M > 0) {
     if ((err = upload(c, c->texinfo, ti.data(), ti.size() * sizeof(TexInfo)))) return err;
     if (c->texdata.ensure(size_t(pool)) < 0) return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, "device allocation failed");
