@@ -771,9 +771,9 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   // and projection of the winning face.
   static_assert(NT <= 256, "slot keys hold 8-bit LDS slots");
   const bool slot_key = mesh.F <= (1 << 24);
-  unsigned long long seen = ~0ull;  // key whose record the registers hold
-  float4 wa = make_float4(0, 0, 0, 0), wb = wa, wc = wa;
-  float warea = 0.0f, wn0 = 0.0f, wn1 = 0.0f, wn2 = 0.0f;
+  unsigned long long seen = ~0ull;  // key whose coverage the registers hold
+  float wz = 0.0f, ww0 = 0.0f, ww1 = 0.0f, ww2 = 0.0f;
+  bool wbf = false;
   // overflowing tiles report their true occupancy; the host re-renders with
   // bins that large (one same-address atomic per overflowing tile only)
   if (t == 0 && count > bins.cap) atomicMax(&bins.status[0], count);
@@ -928,18 +928,16 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
       for (int off = 32; off > 0; off >>= 1) db = max(db, static_cast<uint32_t>(__shfl_xor(static_cast<int>(db), off)));
       if ((t & 63) == 0) s.wmax[t >> 6] = db;
     }
-    if (slot_key) {
-      if (k != seen) {
-        const int sl = static_cast<int>(k & 0xffull);
-        wa = make_float4(s.fv[0][sl], s.fv[1][sl], s.fv[2][sl], 0.0f);
-        wb = make_float4(s.fv[3][sl], s.fv[4][sl], s.fv[5][sl], 0.0f);
-        wc = make_float4(s.fv[6][sl], s.fv[7][sl], s.fv[8][sl], 0.0f);
-        warea = s.fv[9][sl];
-        wn0 = s.fv[10][sl];
-        wn1 = s.fv[11][sl];
-        wn2 = s.fv[12][sl];
-        seen = k;
-      }
+    if (slot_key && k != seen) {
+      // the pixel's winner changed: evaluate its coverage from the staged
+      // record with the pair test's arithmetic (same bits), so the resolve
+      // needs neither the record nor a second test
+      const int sl = static_cast<int>(k & 0xffull);
+      wbf = has_fn && sum3(s.fv[10][sl] * r0, s.fv[11][sl] * r1, s.fv[12][sl] * r2) > 0.0f;
+      cover_test(s.fv[0][sl], s.fv[1][sl], s.fv[2][sl], s.fv[3][sl], s.fv[4][sl], s.fv[5][sl],
+                 s.fv[6][sl], s.fv[7][sl], s.fv[8][sl], s.fv[9][sl], wbf,
+                 static_cast<float>(gx), static_cast<float>(gy), wz, ww0, ww1, ww2);
+      seen = k;
     }
     if (more) {
       // the next chunk's staging overwrites the records read above
@@ -951,8 +949,8 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   }
 
   // ---- resolve (rasterizer.cc:243-301) for this thread's pixel; the
-  // winner's projected vertices, area, face normal and vertex ids are in
-  // registers (slot keys) or gathered again below
+  // winner's depth, weights and back-face flag are in registers (slot keys)
+  // or computed again below
   int fid = -1;
   bool have = false;
   if (s.any_flag && ((s.flag[t >> 5] >> (t & 31)) & 1u)) {
@@ -970,20 +968,23 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
   float z = 0.0f, w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
   bool bf = false;
   if (fid >= 0) {
-    if (!have) {
+    if (have) {
+      z = wz;
+      w0 = ww0;
+      w1 = ww1;
+      w2 = ww2;
+      bf = wbf;
+    } else {
       float P[9], N[3];
       load_face_rec(mesh.soup, fid, P, N);
-      wa = project_point(v, P[0], P[1], P[2]);
-      wb = project_point(v, P[3], P[4], P[5]);
-      wc = project_point(v, P[6], P[7], P[8]);
-      warea = edge_fn(wa.x, wa.y, wb.x, wb.y, wc.x, wc.y);
-      wn0 = N[0];
-      wn1 = N[1];
-      wn2 = N[2];
+      const float4 a = project_point(v, P[0], P[1], P[2]);
+      const float4 b = project_point(v, P[3], P[4], P[5]);
+      const float4 c = project_point(v, P[6], P[7], P[8]);
+      const float area = edge_fn(a.x, a.y, b.x, b.y, c.x, c.y);
+      bf = mesh.fnormals != nullptr && sum3(N[0] * r0, N[1] * r1, N[2] * r2) > 0.0f;
+      cover_test(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, area, bf,
+                 static_cast<float>(gx), static_cast<float>(gy), z, w0, w1, w2);
     }
-    bf = mesh.fnormals != nullptr && sum3(wn0 * r0, wn1 * r1, wn2 * r2) > 0.0f;
-    cover_test(wa.x, wa.y, wa.z, wb.x, wb.y, wb.z, wc.x, wc.y, wc.z, warea, bf,
-               static_cast<float>(gx), static_cast<float>(gy), z, w0, w1, w2);
     if (opt.culling && bf) {
       fid = -1;
       z = 0.0f;
