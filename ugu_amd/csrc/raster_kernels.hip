@@ -477,6 +477,7 @@ struct RasterLds {
   uint32_t wmax[NT / 64];       // per-wave max of the pixels' depth bits
   int32_t any_flag;
   int32_t item;
+  int4 runseg[4];              // counters of the persistent block's current run
 };
 
 // Exact sequential z-buffer for one pixel (slow path): visits the covering
@@ -700,15 +701,15 @@ UGU_DEV void write_gbuffer(const DevView& v, int64_t pidx, bool covered, float u
   if (v.geo_id) to_global(v.geo_id)[pidx] = covered ? 0 : -1;
 }
 
+// One screen tile; `seg` is the tile's counter record (front count, back
+// count, back-segment minimum skip depth, view index).
 template <int TW, int TH, int MODE>
-UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const DevView* views,
+UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const DevView* views,
                          const DevMesh& mesh, const float4* proj,
                          const DevBins& bins, const DevOptions& opt) {
   constexpr int NT = TW * TH;
   const int t = threadIdx.x;
 
-  // counters and view in one load
-  const int4 seg = reinterpret_cast<const int4*>(bins.counts)[gtile];
   const int count = seg.x + seg.y;
   const DevView& v = views[seg.w];
   const int lt = gtile - v.tile_base;
@@ -1069,8 +1070,9 @@ __global__ __launch_bounds__(TW * TH, 6) void k_raster(
     int persistent) {
   __shared__ RasterLds<TW * TH> s;
   if (!persistent) {
-    raster_tile<TW, TH, MODE>(s, xcd_remap(blockIdx.x, gridDim.x), views, mesh,
-                              proj, bins, opt);
+    const int gtile = xcd_remap(blockIdx.x, gridDim.x);
+    raster_tile<TW, TH, MODE>(s, gtile, reinterpret_cast<const int4*>(bins.counts)[gtile], views,
+                              mesh, proj, bins, opt);
     return;
   }
   // persistent: pull runs of kQueueRun consecutive (view, tile) items until
@@ -1081,16 +1083,22 @@ __global__ __launch_bounds__(TW * TH, 6) void k_raster(
   // without a barrier); the one after keeps thread 0 from overwriting it
   // before every thread has read it. `item` is the same in every thread, so
   // every thread leaves the loop together.
+  // The counter records of a run's tiles are read together into LDS, one
+  // round trip per run (most tiles of a batch are empty and need nothing else).
   constexpr int kQueueRun = 4;
+  static_assert(kQueueRun <= 4, "RasterLds::runseg holds one run");
   if (threadIdx.x == 0) s.item = atomicAdd(&bins.status[1], kQueueRun);
   __syncthreads();
   int item = s.item;
   __syncthreads();
   while (item < bins.total_tiles) {
     const int end = min(item + kQueueRun, bins.total_tiles);
+    if (threadIdx.x < end - item)
+      s.runseg[threadIdx.x] = reinterpret_cast<const int4*>(bins.counts)[item + threadIdx.x];
+    __syncthreads();
     for (int it = item; it < end; it++) {
       if (it == end - 1 && threadIdx.x == 0) s.item = atomicAdd(&bins.status[1], kQueueRun);
-      raster_tile<TW, TH, MODE>(s, it, views, mesh, proj, bins, opt);
+      raster_tile<TW, TH, MODE>(s, it, s.runseg[it - item], views, mesh, proj, bins, opt);
     }
     __syncthreads();
     item = s.item;
