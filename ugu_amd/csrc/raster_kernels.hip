@@ -354,8 +354,7 @@ constexpr int kScatterMaxPairs = 32;
 template <int TW, int TH>
 __global__ __launch_bounds__(256) void k_setup_bin(
     const DevView* __restrict__ views, int n_views, DevMesh mesh,
-    const float4* __restrict__ proj, DevBins bins) {
-  (void)proj;
+    DevBins bins) {
   // The block's 256 faces share their vertices (about 150 distinct ones on a
   // regular mesh): every distinct vertex is projected once per view into LDS
   // and the faces read their corners from there, about 0.6 projections per
@@ -705,7 +704,7 @@ UGU_DEV void write_gbuffer(const DevView& v, int64_t pidx, bool covered, float u
 // count, back-segment minimum skip depth, view index).
 template <int TW, int TH, int MODE>
 UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const DevView* views,
-                         const DevMesh& mesh, const float4* proj,
+                         const DevMesh& mesh,
                          const DevBins& bins, const DevOptions& opt) {
   constexpr int NT = TW * TH;
   const int t = threadIdx.x;
@@ -716,7 +715,6 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const
   const int tyi = lt / v.tiles_x;
   const int txi = lt - tyi * v.tiles_x;
   const int px0 = txi * TW, py0 = tyi * TH;
-  (void)proj;
 
   // this thread's pixel
   const int lx = t % TW, ly = t / TW;
@@ -1066,13 +1064,13 @@ template <int TW, int TH, int MODE>
 // compiler then keeps in scratch.
 __global__ __launch_bounds__(TW * TH, 6) void k_raster(
     const DevView* __restrict__ views, DevMesh mesh,
-    const float4* __restrict__ proj, DevBins bins, DevOptions opt,
+    DevBins bins, DevOptions opt,
     int persistent) {
   __shared__ RasterLds<TW * TH> s;
   if (!persistent) {
     const int gtile = xcd_remap(blockIdx.x, gridDim.x);
     raster_tile<TW, TH, MODE>(s, gtile, reinterpret_cast<const int4*>(bins.counts)[gtile], views,
-                              mesh, proj, bins, opt);
+                              mesh, bins, opt);
     return;
   }
   // persistent: pull runs of kQueueRun consecutive (view, tile) items until
@@ -1098,7 +1096,7 @@ __global__ __launch_bounds__(TW * TH, 6) void k_raster(
     __syncthreads();
     for (int it = item; it < end; it++) {
       if (it == end - 1 && threadIdx.x == 0) s.item = atomicAdd(&bins.status[1], kQueueRun);
-      raster_tile<TW, TH, MODE>(s, it, s.runseg[it - item], views, mesh, proj, bins, opt);
+      raster_tile<TW, TH, MODE>(s, it, s.runseg[it - item], views, mesh, bins, opt);
     }
     __syncthreads();
     item = s.item;
@@ -1429,16 +1427,16 @@ int launch_face_soup(hipStream_t st, const float* verts, const int32_t* faces,
 }
 
 int launch_setup_bin(hipStream_t st, int tw, int th, const DevView* views,
-                     int n_views, const DevMesh& mesh, const float4* proj,
+                     int n_views, const DevMesh& mesh,
                      const DevBins& bins) {
   if (mesh.F <= 0) return 0;
   dim3 grid((mesh.F + 255) / 256, (n_views + kSetupViews - 1) / kSetupViews), block(256);
   if (tw == 8 && th == 8)
-    hipLaunchKernelGGL((k_setup_bin<8, 8>), grid, block, 0, st, views, n_views, mesh, proj, bins);
+    hipLaunchKernelGGL((k_setup_bin<8, 8>), grid, block, 0, st, views, n_views, mesh, bins);
   else if (tw == 16 && th == 16)
-    hipLaunchKernelGGL((k_setup_bin<16, 16>), grid, block, 0, st, views, n_views, mesh, proj, bins);
+    hipLaunchKernelGGL((k_setup_bin<16, 16>), grid, block, 0, st, views, n_views, mesh, bins);
   else if (tw == 32 && th == 8)
-    hipLaunchKernelGGL((k_setup_bin<32, 8>), grid, block, 0, st, views, n_views, mesh, proj, bins);
+    hipLaunchKernelGGL((k_setup_bin<32, 8>), grid, block, 0, st, views, n_views, mesh, bins);
   else
     return -2;
   return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -1446,24 +1444,24 @@ int launch_setup_bin(hipStream_t st, int tw, int th, const DevView* views,
 
 template <int TW, int TH, int MODE>
 static int launch_raster_t(hipStream_t st, const DevView* views,
-                           const DevMesh& mesh, const float4* proj,
+                           const DevMesh& mesh,
                            const DevBins& bins, const DevOptions& opt,
                            int persistent_blocks) {
   if (bins.total_tiles <= 0) return 0;
   const int grid = persistent_blocks > 0 ? persistent_blocks : bins.total_tiles;
   hipLaunchKernelGGL((k_raster<TW, TH, MODE>), dim3(grid), dim3(TW * TH), 0, st,
-                     views, mesh, proj, bins, opt, persistent_blocks > 0 ? 1 : 0);
+                     views, mesh, bins, opt, persistent_blocks > 0 ? 1 : 0);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_raster(hipStream_t st, int tw, int th, int mode, const DevView* views,
-                  const DevMesh& mesh, const float4* proj, const DevBins& bins,
+                  const DevMesh& mesh, const DevBins& bins,
                   const DevOptions& opt, int persistent_blocks) {
 #define UGU_RASTER_CASE(TWV, THV)                                                  \
   if (tw == TWV && th == THV)                                                      \
-    return mode == 0 ? launch_raster_t<TWV, THV, 0>(st, views, mesh, proj, bins,   \
+    return mode == 0 ? launch_raster_t<TWV, THV, 0>(st, views, mesh, bins,   \
                                                     opt, persistent_blocks)        \
-                     : launch_raster_t<TWV, THV, 1>(st, views, mesh, proj, bins,   \
+                     : launch_raster_t<TWV, THV, 1>(st, views, mesh, bins,   \
                                                     opt, persistent_blocks);
   UGU_RASTER_CASE(8, 8)
   UGU_RASTER_CASE(16, 16)

@@ -18,7 +18,6 @@ struct DevView {
   int32_t tiles_x, tiles_y;
   int32_t tile_base;  // first global tile of this view
   int32_t pad0;
-  int64_t proj_offset;  // first projected vertex (float4 units) of this view
   uint8_t* color;
   float* depth;
   float* normal;

@@ -25,10 +25,10 @@ int launch_face_soup(hipStream_t st, const float* verts, const int32_t* faces,
                      const float* fnormals, int32_t F, int compute_normals, float4* soup,
                      float* fn_out);
 int launch_setup_bin(hipStream_t st, int tw, int th, const DevView* views,
-                     int n_views, const DevMesh& mesh, const float4* proj,
+                     int n_views, const DevMesh& mesh,
                      const DevBins& bins);
 int launch_raster(hipStream_t st, int tw, int th, int mode, const DevView* views,
-                  const DevMesh& mesh, const float4* proj, const DevBins& bins,
+                  const DevMesh& mesh, const DevBins& bins,
                   const DevOptions& opt, int persistent_blocks);
 int raster_blocks_per_cu(int tw, int th, int mode);
 int64_t backward_scratch_ints(int32_t F);
@@ -114,7 +114,7 @@ struct ugu_hip_ctx {
       texinfo, texdata;
 
   // work buffers
-  DevBuf proj, counts, bins, status, views, tile_base, stage_out, diff_fn;
+  DevBuf counts, bins, status, views, tile_base, stage_out, diff_fn;
   DevBuf bwd_items;  // (face, view, pixel) items of the backward pass, 16 B each
   DevBuf bwd_scratch;  // per-face counts and offsets of the backward pass
   DevBuf soup, diff_soup;  // 48-byte face records (prepared mesh / differentiable path)
@@ -441,7 +441,6 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
   const int total_tiles = static_cast<int>(total);
 
   int err;
-  if ((err = ensure(c, c->proj, 16))) return err;
   if ((err = ensure(c, c->counts, size_t(total_tiles) * 16))) return err;  // 4 per tile
   if ((err = ensure(c, c->status, 64))) return err;
   if ((err = ensure(c, c->views, size_t(n) * sizeof(DevView)))) return err;
@@ -474,7 +473,6 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
     v.tiles_x = (v.W + tw - 1) / tw;
     v.tiles_y = (v.H + th - 1) / th;
     v.tile_base = c->h_tile_base[i];
-    v.proj_offset = int64_t(i) * dm.V;
     if (mode == 0) {
       void* p[kPlanes];
       plane_ptrs(outs[i], gbufs ? &gbufs[i] : nullptr, p);
@@ -511,7 +509,6 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
     b.tile_base = c->tile_base.as<int32_t>();
     b.n_views = n;
     const DevView* dv = c->views.as<DevView>();
-    float4* proj = c->proj.as<float4>();
 
     UGU_HIP_CHECK(c, hipMemcpyAsync(c->views.p, c->h_views, size_t(n) * sizeof(DevView),
                                     hipMemcpyHostToDevice, c->stream));
@@ -524,9 +521,9 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
       if (launch_clear(c->stream, b.counts, total_tiles, b.status, b.tile_base, n))
         return -1;
       if (timed) (void)hipEventRecord(c->ev[1], c->stream);
-      if (launch_setup_bin(c->stream, tw, th, dv, n, dm, proj, b)) return -1;
+      if (launch_setup_bin(c->stream, tw, th, dv, n, dm, b)) return -1;
       if (timed) (void)hipEventRecord(c->ev[2], c->stream);
-      if (launch_raster(c->stream, tw, th, mode, dv, dm, proj, b, dopt, persistent_blocks)) return -1;
+      if (launch_raster(c->stream, tw, th, mode, dv, dm, b, dopt, persistent_blocks)) return -1;
       if (timed) (void)hipEventRecord(c->ev[3], c->stream);
       return 0;
     };
@@ -798,7 +795,7 @@ void ugu_hip_destroy(ugu_hip_ctx* c) {
   drop_graphs(c);
   DevBuf* bufs[] = {&c->verts, &c->faces, &c->faces_orig, &c->fnormals, &c->normals, &c->nidx,
                     &c->vcolors, &c->uv, &c->uvidx, &c->matid, &c->texinfo,
-                    &c->texdata, &c->proj, &c->counts, &c->bins, &c->status,
+                    &c->texdata, &c->counts, &c->bins, &c->status,
                     &c->views, &c->tile_base, &c->stage_out, &c->diff_fn,
                     &c->bwd_items, &c->bwd_scratch, &c->soup, &c->diff_soup, &c->shade, &c->blk_off,
                     &c->blk_vid, &c->face_loc, &c->blk_off_o, &c->blk_vid_o, &c->face_loc_o,
@@ -1366,7 +1363,6 @@ int ugu_hip_diff_backward(ugu_hip_ctx* c, const UguCamera* cams, int n,
     std::memcpy(v.w2c_R, cams[i].w2c_R, sizeof(v.w2c_R));
     std::memcpy(v.w2c_t, cams[i].w2c_t, sizeof(v.w2c_t));
     std::memcpy(v.c2w_R, cams[i].c2w_R, sizeof(v.c2w_R));
-    v.proj_offset = int64_t(i) * dm.V;
     v.face_id = const_cast<int32_t*>(face_ids[i]);
     v.grad = grad_images[i];
     max_px = std::max(max_px, v.W * v.H);
