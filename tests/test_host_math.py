@@ -90,3 +90,37 @@ def test_geodesic_is_watertight():
     assert (counts == 2).all()
     # Euler characteristic of a sphere
     assert len(v) - len(counts) + len(f) == 2
+
+
+def test_device_binding_checks_buffer_shapes_and_camera_sizes():
+    """DeviceBinding (Rasterizer.bind_device) marshals the arguments once; it
+    rejects tensors whose shape is not the camera's image and a camera of
+    another size than the bound tensors (the library would write past them)."""
+    import pytest
+    import torch
+    from ugu_amd import RendererCpuOption
+    from ugu_amd.renderer import DeviceBinding
+
+    class _R:
+        option_ = RendererCpuOption()
+
+    c2w = look_at_c2w((0, 0, -3), (0, 0, 0), (0, 1, 0))
+    small = PinholeCamera(64, 48, c2w, (31.5, 23.5), (60.0, 60.0))
+    big = PinholeCamera(96, 64, c2w, (47.5, 31.5), (60.0, 60.0))
+    bufs = [{"face_id": torch.empty((48, 64), dtype=torch.int32),
+             "color": torch.empty((48, 64, 3), dtype=torch.uint8)},
+            {"depth": torch.empty((64, 96), dtype=torch.float32)}]
+    b = DeviceBinding(_R(), [small, big], bufs)
+    assert b.n == 2 and b.sizes == [(64, 48), (96, 64)]
+    assert b.outs[0].face_id == bufs[0]["face_id"].data_ptr()
+    assert b.outs[0].depth is None and b.outs[1].depth == bufs[1]["depth"].data_ptr()
+    moved = PinholeCamera(64, 48, look_at_c2w((1, 0, -3), (0, 0, 0), (0, 1, 0)), (31.5, 23.5),
+                          (60.0, 60.0))
+    b.set_camera(0, moved)
+    assert b.cams[0].w2c_t[0] == moved.to_abi().w2c_t[0]
+    with pytest.raises(ValueError):
+        b.set_camera(0, big)
+    with pytest.raises(ValueError):
+        DeviceBinding(_R(), [big, small], bufs)
+    with pytest.raises(ValueError):
+        DeviceBinding(_R(), [small], bufs)
