@@ -296,8 +296,9 @@ def main():
                     help="run a 1 GiB device-to-device copy first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
-              "(views shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the views "
+              "would shard across GPUs with no exchange; one GPU's share is what is "
+              "measured)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

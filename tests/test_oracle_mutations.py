@@ -4,7 +4,7 @@ The oracle is rebuilt with -DORACLE_MUTATION=k (oracle/ugu_oracle.cc) into a
 temporary directory; the pins of oracle_pins.py run against that build and at
 least one must fail. Mistakes that only move results by float rounding (the
 Eigen reduction order, divide-then-multiply in Project) cannot be caught this
-way; they are listed as parity-unpinned in DESIGN.md.
+way; for those the oracle's parity is unpinned.
 """
 import os
 import subprocess
