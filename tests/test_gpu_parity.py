@@ -9,8 +9,8 @@ import pytest
 
 import oracle as O
 import tiny_scenes as TS
-from ugu_amd import (ColorInterpolation, DiffuseColor, DiffuseShading, Rasterizer,
-                     RendererCpuOption, ShadingNormal, scenes)
+from ugu_amd import (ColorInterpolation, DiffuseColor, DiffuseShading, PinholeCamera,
+                     Rasterizer, RendererCpuOption, ShadingNormal, look_at_c2w, scenes)
 
 pytestmark = pytest.mark.gpu
 
@@ -282,7 +282,10 @@ def test_multi_material_textures(gpu_available, interp, shading):
     mesh.set_material_ids(rng.integers(0, 3, len(mesh.vertex_indices)))
     opt = RendererCpuOption(diffuse_color=DiffuseColor.kTexture, interp=interp,
                             diffuse_shading=shading)
-    cams = scenes.multiview_cameras(6)[:3]
+    # small views: the oracle raytracer tests every face at every pixel
+    cams = [PinholeCamera(w, h, look_at_c2w(pos, (0, 0, 0), (0, 1, 0)), fov_y_deg=40.0)
+            for (w, h), pos in (((200, 150), (0.4, 0.3, -2.6)), ((96, 64), (-2.2, 0.5, -1.5)),
+                                ((128, 128), (1.9, -0.8, 1.7)))]
     r = Rasterizer(opt)
     r.set_mesh(mesh)
     assert r.PrepareMesh()
