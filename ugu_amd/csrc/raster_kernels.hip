@@ -1125,27 +1125,56 @@ __global__ void k_zero_i32(int32_t* p, int n) {
 }
 
 // face 0 is never shaded (rasterizer.cc:250), so it has no gradient
+// Runs of equal face ids among a wavefront's consecutive pixels: `head` for
+// the first lane of a run, `first` = that lane, `len` = the run's length (in
+// the head lane). Lanes without a face form runs of their own.
+struct FaceRun {
+  bool head;
+  int first, len;
+};
+
+UGU_DEV FaceRun face_run(int key) {
+  const int lane = __lane_id();
+  const int prev = __shfl_up(key, 1);
+  FaceRun r;
+  r.head = lane == 0 || prev != key;
+  const uint64_t heads = __ballot(r.head);
+  const uint64_t upto = heads & ((2ull << lane) - 1ull);
+  r.first = 63 - __clzll(static_cast<long long>(upto));
+  const uint64_t after = heads & ~((2ull << lane) - 1ull);
+  r.len = (after ? __ffsll(static_cast<unsigned long long>(after)) - 1 : 64) - lane;
+  return r;
+}
+
+// One thread per pixel; one atomic per run of equal face ids.
 __global__ __launch_bounds__(256) void k_bwd_count(const DevView* __restrict__ views,
                                                    int32_t F, int32_t* __restrict__ cnt) {
   const DevView& v = views[blockIdx.y];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= v.W * v.H) return;
-  const int fid = v.face_id[p];
-  if (fid > 0 && fid < F) atomicAdd(&cnt[fid], 1);
+  if (int64_t(blockIdx.x) * blockDim.x >= int64_t(v.W) * v.H) return;  // whole block idle
+  const int fid = p < v.W * v.H ? v.face_id[p] : 0;
+  const bool valid = fid > 0 && fid < F;
+  const FaceRun r = face_run(valid ? fid : -1 - __lane_id());
+  if (valid && r.head) atomicAdd(&cnt[fid], r.len);
 }
 
+// One thread per pixel: the run's head takes len slots from the face's
+// counter (which counts back down to zero); lane j of the run writes slot
+// base - 1 - j of the face's range.
 __global__ __launch_bounds__(256) void k_bwd_scatter(const DevView* __restrict__ views,
                                                      int32_t F, const int32_t* __restrict__ off,
                                                      int32_t* __restrict__ cnt,
                                                      int4* __restrict__ items) {
   const DevView& v = views[blockIdx.y];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= v.W * v.H) return;
-  const int fid = v.face_id[p];
-  if (fid > 0 && fid < F) {
-    const int pos = off[fid] + atomicAdd(&cnt[fid], -1) - 1;
-    items[pos] = make_int4(fid, blockIdx.y, p, 0);
-  }
+  if (int64_t(blockIdx.x) * blockDim.x >= int64_t(v.W) * v.H) return;  // whole block idle
+  const int fid = p < v.W * v.H ? v.face_id[p] : 0;
+  const bool valid = fid > 0 && fid < F;
+  const FaceRun r = face_run(valid ? fid : -1 - __lane_id());
+  int base = 0;
+  if (valid && r.head) base = atomicAdd(&cnt[fid], -r.len);
+  base = __shfl(base, r.first);
+  if (valid) items[off[fid] + base - 1 - (__lane_id() - r.first)] = make_int4(fid, blockIdx.y, p, 0);
 }
 
 // Exclusive scan of n counters (n + 1 outputs, out[n] = total) in three
