@@ -1271,6 +1271,12 @@ UGU_DEV int tex_slot(int key) {
   return static_cast<int>((static_cast<uint32_t>(key) * 2654435761u) >> 21);  // 11 bits
 }
 
+// Items per thread of k_bwd_items: a block handles kBwdRounds x 256
+// consecutive items (round r: items r*256 .. r*256+255 of the block's range,
+// so each wavefront sees consecutive items) and merges the texel gradients
+// of all of them in its LDS table before the global atomics.
+constexpr int kBwdRounds = 4;
+
 __global__ __launch_bounds__(256) void k_bwd_items(
     const DevView* __restrict__ views, DevMesh mesh, const int4* __restrict__ items,
     const int32_t* __restrict__ cursor, DevGrads g) {
@@ -1278,8 +1284,18 @@ __global__ __launch_bounds__(256) void k_bwd_items(
   __shared__ int32_t hkey[kTexSlots];
   __shared__ float hval[3][kTexSlots];
   const int n_items = *cursor;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (int64_t(blockIdx.x) * blockDim.x >= n_items) return;  // whole block idle
+  const int64_t first = int64_t(blockIdx.x) * blockDim.x * kBwdRounds;
+  if (first >= n_items) return;  // whole block idle
+  const bool tex = g.texture && mesh.ftex;  // uniform over the block
+  if (tex) {
+    for (int j = threadIdx.x; j < kTexSlots; j += blockDim.x) {
+      hkey[j] = -1;
+      hval[0][j] = hval[1][j] = hval[2][j] = 0.0f;
+    }
+    __syncthreads();
+  }
+  for (int round = 0; round < kBwdRounds; round++) {
+  const int64_t i = first + int64_t(round) * blockDim.x + threadIdx.x;
   const bool valid = i < n_items;
   int4 item = make_int4(-1, 0, 0, 0);
   if (valid) item = items[i];
@@ -1383,13 +1399,8 @@ __global__ __launch_bounds__(256) void k_bwd_items(
       if (head) unsafeAtomicAdd(&g.verts[int64_t(vid[k / 3]) * 3 + (k % 3)], val);
     }
   }
-  if (g.texture && mesh.ftex) {  // uniform over the block
-    for (int j = threadIdx.x; j < kTexSlots; j += blockDim.x) {
-      hkey[j] = -1;
-      hval[0][j] = hval[1][j] = hval[2][j] = 0.0f;
-    }
-    __syncthreads();
-    if (valid) {
+  if (tex && valid) {
+    {
       const int tw = mesh.ftex_w;
       const int tidx[4] = {ty0 * tw + tx0, ty0 * tw + tx1, ty1 * tw + tx0, ty1 * tw + tx1};
       for (int q = 0; q < 4; q++) {
@@ -1409,6 +1420,9 @@ __global__ __launch_bounds__(256) void k_bwd_items(
             unsafeAtomicAdd(&g.texture[int64_t(key) * 3 + k], gtex[q * 3 + k]);
       }
     }
+  }
+  }  // rounds
+  if (tex) {
     __syncthreads();
     for (int j = threadIdx.x; j < kTexSlots; j += blockDim.x) {
       const int key = hkey[j];
@@ -1540,7 +1554,9 @@ int launch_backward(hipStream_t st, const DevView* views, int n_views,
   hipLaunchKernelGGL(k_scan_add, dim3((F + 255) / 256), dim3(256), 0, st, off, F, sums, nb);
   hipLaunchKernelGGL(k_bwd_scatter, pgrid, dim3(256), 0, st, views, F, off, cnt, items);
   // at most one item per pixel; blocks past the item count (off[F]) exit at once
-  hipLaunchKernelGGL(k_bwd_items, dim3(static_cast<unsigned>((total_pixels + 255) / 256)),
+  hipLaunchKernelGGL(k_bwd_items,
+                     dim3(static_cast<unsigned>((total_pixels + 256 * kBwdRounds - 1) /
+                                                (256 * kBwdRounds))),
                      dim3(256), 0, st, views, mesh, items, off + F, g);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
