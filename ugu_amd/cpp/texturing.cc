@@ -88,13 +88,14 @@ float Edge(const Eigen::Vector2f& a, const Eigen::Vector2f& b, const Eigen::Vect
   return (c[0] - a[0]) * (b[1] - a[1]) - (c[1] - a[1]) * (b[0] - a[0]);
 }
 
-UguCamera keyframe_camera(const Camera& cam) {
-  UguCamera c;
-  to_abi(cam, &c);
+// The keyframe's camera for the C ABI; false for a camera that is not a
+// PinholeCamera (the only model the GPU path implements).
+bool keyframe_camera(const Camera& cam, UguCamera* c) {
+  if (!to_abi(cam, c)) return false;
   // TestVertex multiplies by w2c().cast<float>(): its linear part, not rotation()
   for (int r = 0; r < 3; r++)
-    for (int k = 0; k < 3; k++) c.w2c_R[r * 3 + k] = static_cast<float>(cam.w2c().matrix()(r, k));
-  return c;
+    for (int k = 0; k < 3; k++) c->w2c_R[r * 3 + k] = static_cast<float>(cam.w2c().matrix()(r, k));
+  return true;
 }
 
 }  // namespace
@@ -275,7 +276,10 @@ class VisibilityTester::Impl {
   bool TestItems(const std::vector<Eigen::Vector3f>& pos, const std::vector<Eigen::Vector3f>& vert,
                  const std::vector<Eigen::Vector3f>& nrm, std::vector<UguVertexVisibility>* out) const {
     UguKeyframe k;
-    k.camera = keyframe_camera(*keyframe_->camera);
+    if (keyframe_->camera == nullptr || !keyframe_camera(*keyframe_->camera, &k.camera)) {
+      LOGE("keyframe camera must be a PinholeCamera\n");
+      return false;
+    }
     k.width = keyframe_->color.cols;
     k.height = keyframe_->color.rows;
     k.color = keyframe_->color.data;
