@@ -292,7 +292,7 @@ UGU_DEV AppendTicket append_issue(const DevBins& bins, int seg, uint32_t zmin, b
 // so the wavefront waits once for all R atomics (a store issued before a
 // later wait would be waited for as well).
 template <int R>
-UGU_DEV void append_finish(const DevBins& bins, const AppendTicket* tk, int2 entry) {
+UGU_DEV void append_finish(const DevBins& bins, const AppendTicket* tk, const int2* entry) {
   int slot[R];
 #pragma unroll
   for (int r = 0; r < R; r++) slot[r] = __shfl(tk[r].ret, tk[r].lr & 0xff) + (tk[r].lr >> 8);
@@ -300,7 +300,7 @@ UGU_DEV void append_finish(const DevBins& bins, const AppendTicket* tk, int2 ent
   for (int r = 0; r < R; r++) {
     if (tk[r].seg >= 0 && slot[r] < bins.cap) {
       const int64_t row = int64_t(tk[r].seg >> 2) * bins.cap;
-      bins.bins[row + ((tk[r].seg & 1) ? bins.cap - 1 - slot[r] : slot[r])] = entry;
+      bins.bins[row + ((tk[r].seg & 1) ? bins.cap - 1 - slot[r] : slot[r])] = *entry;
     }
   }
 }
@@ -433,6 +433,8 @@ __global__ __launch_bounds__(256) void k_setup_bin(
     // one bound for all rounds: the minimum over every back face of the
     // wavefront is also below the skip depth of each back face of a round
     const uint32_t zmin = back_zmin(has && back, zskip);
+    const int nx = tx1 - tx0 + 1;
+    const int rest = has ? max(nx * (ty1 - ty0 + 1) - kAppendRounds, 0) : 0;
     int tx = tx0, ty = ty0;
     AppendTicket tk[kAppendRounds];
 #pragma unroll
@@ -444,14 +446,35 @@ __global__ __launch_bounds__(256) void k_setup_bin(
         if (++ty > ty1) has = false;
       }
     }
-    append_finish<kAppendRounds>(bins, tk, entry);
-    while (__any(has)) {
-      const int seg = has ? 4 * (v.tile_base + ty * v.tiles_x + tx) + (back ? 1 : 0) : -1;
-      const AppendTicket t1 = append_issue(bins, seg, zmin, has);
-      append_finish<1>(bins, &t1, entry);
-      if (has && ++tx > tx1) {
-        tx = tx0;
-        if (++ty > ty1) has = false;
+    append_finish<kAppendRounds>(bins, tk, &entry);
+    // Faces over more than kAppendRounds tiles (large triangles): their
+    // remaining tiles, all lanes' together, 64 per round -- lane j of a round
+    // takes pair base + j of the wavefront's list, finds its face's lane (the
+    // last lane whose exclusive prefix is <= the pair) and the tile ordinal.
+    if (__any(rest > 0)) {
+      const int lane = __lane_id();
+      int incl = rest;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+      }
+      const int excl = incl - rest;
+      const int total = __shfl(incl, 63);
+      for (int base = 0; base < total; base += 64) {
+        const int p = base + lane;
+        int src = 0;
+        for (int step = 32; step >= 1; step >>= 1)
+          if (__shfl(excl, src + step) <= p) src += step;
+        const int k = p - __shfl(excl, src) + kAppendRounds;
+        const int snx = __shfl(nx, src);
+        const int sty = __shfl(ty0, src) + k / snx;
+        const int stx = __shfl(tx0, src) + k % snx;
+        const int sback = __shfl(back ? 1 : 0, src);
+        const int2 e = make_int2(__shfl(f, src), __shfl(__float_as_int(zskip), src));
+        const bool act = p < total;
+        const AppendTicket t1 = append_issue(
+            bins, act ? 4 * (v.tile_base + sty * v.tiles_x + stx) + sback : -1, zmin, act);
+        append_finish<1>(bins, &t1, &e);
       }
     }
   }
