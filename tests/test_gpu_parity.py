@@ -298,3 +298,23 @@ def test_multi_material_textures(gpu_available, interp, shading):
     rts = rt.render_batch(cams[:2])
     for i, c in enumerate(cams[:2]):
         RT._compare(rts[i], RT._oracle(mesh, c, opt), f"multi-material raytrace view {i}")
+
+
+@pytest.mark.parametrize("tile", [(8, 8), (32, 8)])
+def test_large_triangles_over_many_tiles(gpu_available, tile):
+    """Faces whose pixel boxes span hundreds of tiles (their tiles past the
+    fourth are appended compacted across the wavefront, 64 per round) mixed
+    with small faces, at several depths and both windings."""
+    w, h = 200, 150
+    rng = np.random.default_rng(4)
+    big = []
+    for k in range(40):
+        c = rng.uniform([-0.2 * w, -0.2 * h], [1.2 * w, 1.2 * h], (3, 2))
+        big.append(np.concatenate([c, np.full((3, 1), [1.0, 2.0, 4.0][k % 3])], 1))
+    tris = np.concatenate([np.array(big), TS.random_triangles(300, w, h, 4)], 0)
+    mesh = TS.mesh_from_pixels(tris, seed=4)
+    cam = TS.exact_camera(w, h)
+    for cull in (False, True):
+        opt = RendererCpuOption(diffuse_color=DiffuseColor.kVertex, backface_culling=cull)
+        out = _render(mesh, cam, opt, tile, cap=16)
+        _compare(out, _oracle(mesh, cam, opt), f"large triangles tile{tile} cull={cull}")
