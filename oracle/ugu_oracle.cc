@@ -1077,10 +1077,11 @@ void eval_pixel(const DiffCtx& d, int fid, int px, int py, PixelEval& e) {
   e.e[0] = E(e.X[1], e.Y[1], e.X[2], e.Y[2], x, y);
   e.e[1] = E(e.X[2], e.Y[2], e.X[0], e.Y[0], x, y);
   e.e[2] = E(e.X[0], e.Y[0], e.X[1], e.Y[1], x, y);
-  for (int i = 0; i < 3; i++) e.q[i] = e.e[i] / e.A / e.Z[i];
+  for (int i = 0; i < 3; i++) e.q[i] = MUT(28) ? e.e[i] / e.A : e.e[i] / e.A / e.Z[i];
   e.s = e.q[0] + e.q[1] + e.q[2];
   e.b1 = e.q[1] / e.s;
   e.b2 = e.q[2] / e.s;
+  if (MUT(29)) std::swap(e.b1, e.b2);
   e.b0 = 1.0 - e.b1 - e.b2;
   for (int k = 0; k < 3; k++) {
     e.c[k] = 1.0; e.tt[k] = 1.0;
@@ -1105,10 +1106,11 @@ void eval_pixel(const DiffCtx& d, int fid, int px, int py, PixelEval& e) {
     }
     const double iu = e.b0 * e.uv0[0] + e.b1 * e.uv1[0] + e.b2 * e.uv2[0];
     const double iv = e.b0 * e.uv0[1] + e.b1 * e.uv1[1] + e.b2 * e.uv2[1];
-    const double fxp = iu * (d.tw - 1), fyp = (1.0 - iv) * (d.th - 1);
+    const double fxp = iu * (d.tw - 1), fyp = (MUT(30) ? iv : 1.0 - iv) * (d.th - 1);
     const double flx = std::floor(fxp), fly = std::floor(fyp);
     e.lu = fxp - flx;
     e.lv = fyp - fly;
+    if (MUT(31)) std::swap(e.lu, e.lv);
     auto clampi = [](double v, int hi) {
       if (!(v >= 0.0)) return 0;
       if (v > hi) return hi;
@@ -1204,7 +1206,7 @@ int oracle_diff_backward_f64(const OracleCamera* cam, const int32_t* faces, cons
       double gu = 0, gv = 0;
       for (int k = 0; k < 3; k++) {
         const double gc = G[k] * e.tt[k];
-        const double gt = G[k] * e.c[k];
+        const double gt = MUT(33) ? G[k] : G[k] * e.c[k];
         if (g_colors && colors)
           for (int i = 0; i < 3; i++) g_colors[int64_t(f[i]) * 3 + k] += gc * beta[i];
         if (g_tex && tex) {
@@ -1225,7 +1227,7 @@ int oracle_diff_backward_f64(const OracleCamera* cam, const int32_t* faces, cons
       double ge[3];
       for (int j = 0; j < 3; j++) {
         ge[j] = gq[j] / (e.A * e.Z[j]);
-        gZ[j] = -gq[j] * e.q[j] / e.Z[j];
+        gZ[j] = MUT(32) ? 0.0 : -gq[j] * e.q[j] / e.Z[j];
         gA -= gq[j] * e.q[j] / e.A;
       }
       const double x = px, y = py;

@@ -784,8 +784,155 @@ def pin_texture_map():
     return True, ""
 
 
+
+# ---------------------------------------------------------------------------
+# differentiable model (oracle_diff_forward_f64 / oracle_diff_backward_f64)
+
+
+def quad_f64():
+    """The tilted quad's corners in float64, exactly on z = 2 + 0.8 x."""
+    P = np.array([[-1.0, -0.8], [1.0, -0.8], [1.0, 0.8], [-1.0, 0.8]])
+    return np.array([(x, y, 2.0 + 0.8 * x) for x, y in P], np.float64)
+
+
+def lin_colors(x, y):
+    return np.stack([30.0 + 50.0 * x + 20.0 * y, 100.0 - 40.0 * x + 60.0 * y,
+                     7.0 + 3.0 * x - 5.0 * y], -1)
+
+
+QUAD_UVS = np.array([[0.05, 0.1], [0.95, 0.1], [0.95, 0.9], [0.05, 0.9]], np.float32)
+
+
+def lin_texture(tw=101, th=71):
+    """tex[r, c] = (2c + 0.5r + 1, 3r - c + 100, 5): bilinear sampling of it is
+    exactly the same affine function of the continuous texel position."""
+    cc, rr = np.meshgrid(np.arange(tw, dtype=np.float64), np.arange(th, dtype=np.float64))
+    return np.stack([2 * cc + 0.5 * rr + 1, 3 * rr - cc + 100, np.full_like(cc, 5.0)], -1)
+
+
+def quad_diff_expected(cam, tw, th):
+    """Closed forms on the tilted quad for colours _lin_colors of the corners
+    and the texture lin_texture(tw, th) with uvs QUAD_UVS: the colour and the
+    texture value at each pixel ray's hit with the plane."""
+    X, Y, _ = _plane_hits(cam)
+    # uv corners as stored (float32): u runs from a to b over x = -1..1, v
+    # from c to d over y = -0.8..0.8
+    a, b = (float(np.float32(t)) for t in (0.05, 0.95))
+    c, d = (float(np.float32(t)) for t in (0.1, 0.9))
+    u = a + (X + 1) / 2 * (b - a)
+    v = c + (Y + 0.8) / 1.6 * (d - c)
+    fxp, fyp = u * (tw - 1), (1 - v) * (th - 1)
+    exp_c = lin_colors(X, Y)
+    exp_t = np.stack([2 * fxp + 0.5 * fyp + 1, 3 * fyp - fxp + 100, np.full_like(fxp, 5.0)], -1)
+    return exp_c, exp_t
+
+
+def _quad_diff_setup():
+    m, cam = _tilted_quad(uvs=QUAD_UVS)
+    oc = O.camera_from_abi(cam.to_abi())
+    fid = render(m, cam, RendererCpuOption())["face_id"]
+    X, Y, _ = _plane_hits(cam)
+    inner = (fid > 0) & (np.abs(X) < 0.97) & (np.abs(Y) < 0.77)
+    return m, cam, oc, fid, X, Y, inner
+
+
+def pin_diff_forward():
+    """The float64 image model against the ray-plane hit of every pixel ray:
+    colours linear in world x, y give that linear function at the hit
+    (perspective-correct weights, u = w1 and v = w2 on the right corners);
+    a texture linear in (column, row) with uv linear in world x, y gives the
+    affine function of the texel position (1 - v) * (rows - 1) (row flip,
+    bilinear weights); with both, their product."""
+    m, cam, oc, fid, X, Y, inner = _quad_diff_setup()
+    if inner.sum() < 500:
+        return False, "quad not covered"
+    V = quad_f64()
+    C = lin_colors(V[:, 0], V[:, 1])
+    tex = lin_texture()
+    th, tw = tex.shape[:2]
+    exp_c, exp_t = quad_diff_expected(cam, tw, th)
+    for cols, tx, exp, name in ((C, None, exp_c, "colours"), (None, tex, exp_t, "texture"),
+                               (C, tex, exp_c * exp_t, "colours x texture")):
+        img = O.diff_forward_f64(oc, m, V, cols, tx, fid)
+        err = np.abs(img[inner] - exp[inner]) / np.maximum(np.abs(exp[inner]), 1.0)
+        if not err.max() <= 1e-9:
+            return False, f"diff forward ({name}) off the ray-plane hit by {err.max():.3g}"
+        if np.any(img[fid <= 0] != 0):
+            return False, f"diff forward ({name}) wrote an uncovered or face-0 pixel"
+    return True, ""
+
+
+def pin_diff_forward_matches_u8_shader():
+    """With the u8 rasterizer's inputs (vertex colours 0..255, a u8 texture as
+    floats) and the rasterizer's face ids, floor(diff image) equals the
+    reference colourizers' truncated u8 output within 1 on every shaded pixel:
+    the float model interpolates the same corners with the same weights."""
+    rng = np.random.default_rng(11)
+    m, cam = _sphere_scene(rotated=True, nseg=24)
+    cols = (rng.random((len(m.vertices), 3)) * 255.0).astype(np.float32)
+    m.set_vertex_colors(cols)
+    out = render(m, cam, RendererCpuOption(diffuse_color=DiffuseColor.kVertex))
+    fid = out["face_id"]
+    sh = fid > 0
+    img = O.diff_forward_f64(O.camera_from_abi(cam.to_abi()), m, m.vertices, cols, None, fid)
+    d = np.abs(np.floor(img[sh]) - out["color"][sh].astype(np.float64))
+    if sh.sum() < 1000 or d.max() > 1:
+        return False, f"vertex-colour model differs from the u8 shader by {d.max()}"
+    texu8 = rng.integers(0, 256, (71, 101, 3), dtype=np.uint8)
+    mq, camq = _tilted_quad(texture=texu8, uvs=QUAD_UVS)
+    outq = render(mq, camq, RendererCpuOption(diffuse_color=DiffuseColor.kTexture,
+                                              interp=ColorInterpolation.kBilinear))
+    fq = outq["face_id"]
+    sq = fq > 0
+    imq = O.diff_forward_f64(O.camera_from_abi(camq.to_abi()), mq, mq.vertices, None,
+                             texu8.astype(np.float64), fq)
+    dq = np.abs(np.floor(imq[sq]) - outq["color"][sq].astype(np.float64))
+    if sq.sum() < 500 or dq.max() > 1:
+        return False, f"texture model differs from the u8 bilinear shader by {dq.max()}"
+    return True, ""
+
+
+def pin_diff_backward():
+    """oracle_diff_backward_f64 against central differences of the
+    pinned float64 forward for L = sum(G * image) on the tilted quad with
+    colours and texture: every colour, a spread of texels and every vertex
+    coordinate, within 1e-5 relative (h = 1e-5: truncation ~h^2, rounding
+    ~1e-16 |L| / h)."""
+    m, cam, oc, fid, X, Y, inner = _quad_diff_setup()
+    V = quad_f64()
+    C = lin_colors(V[:, 0], V[:, 1]) / 100.0
+    tex = lin_texture() / 100.0
+    rng = np.random.default_rng(5)
+    G = rng.normal(size=(cam.height, cam.width, 3)) * inner[..., None]
+    gv, gc, gt = O.diff_backward_f64(oc, m, V, C, tex, fid, G)
+
+    def L(vv, cc, tt):
+        return float((G * O.diff_forward_f64(oc, m, vv, cc, tt, fid)).sum())
+
+    h = 1e-5
+    worst = 0.0
+    params = [("v", V, gv, range(V.size)), ("c", C, gc, range(C.size)),
+              ("t", tex, gt, rng.choice(np.nonzero(gt.reshape(-1))[0], 24, replace=False))]
+    for name, base, g, idx in params:
+        for j in idx:
+            p, q = base.copy().reshape(-1), base.copy().reshape(-1)
+            p[j] += h
+            q[j] -= h
+            args_p, args_q = [V, C, tex], [V, C, tex]
+            k = "vct".index(name)
+            args_p[k], args_q[k] = p.reshape(base.shape), q.reshape(base.shape)
+            fd = (L(*args_p) - L(*args_q)) / (2 * h)
+            an = g.reshape(-1)[j]
+            rel = abs(an - fd) / max(abs(fd), 1e-3)
+            worst = max(worst, rel)
+            if rel > 1e-5:
+                return False, f"diff backward d/d{name}[{j}] = {an:.6g}, central {fd:.6g}"
+    return True, f"max rel {worst:.2g}"
+
+
 ALL_PINS = [pin_brute_force, pin_ties_lowest_id, pin_face_zero_not_shaded,
             pin_perspective_correct, pin_texture_sampling, pin_sphere_closed_forms,
             pin_normals_rotated_camera, pin_lambertian, pin_oren_nayar, pin_depth_u16,
             pin_gbuffer, pin_raytrace_brute_force, pin_raytrace_plane, pin_raytrace_sphere,
-            pin_visibility, pin_texture_map]
+            pin_visibility, pin_texture_map, pin_diff_forward,
+            pin_diff_forward_matches_u8_shader, pin_diff_backward]

@@ -72,6 +72,41 @@ def test_forward_matches_float64(gpu_available, c4):
         assert err <= 1e-5 * max(1.0, np.abs(ref).max()), f"forward err {err}"
 
 
+def test_forward_closed_forms(gpu_available):
+    """The GPU image model against closed forms that do not come from the
+    oracle: on a quad tilted in depth, colours linear in world x, y give that
+    linear function at each pixel ray's hit with the plane; a texture linear
+    in (column, row) with uv linear in world x, y gives the affine function of
+    the texel position (1 - v) * (rows - 1); with both, their product.
+    Tolerance 1e-5 relative (float32 arithmetic)."""
+    import torch
+    import oracle_pins as OP
+    from ugu_amd.diff import DiffRenderer
+    m, cam = OP._tilted_quad(uvs=OP.QUAD_UVS)
+    tex = OP.lin_texture()
+    th, tw = tex.shape[:2]
+    exp_c, exp_t = OP.quad_diff_expected(cam, tw, th)
+    X, Y, _ = OP._plane_hits(cam)
+    dev = torch.device("cuda")
+    dr = DiffRenderer(m)
+    V = torch.tensor(m.vertices, device=dev)
+    C = torch.tensor(OP.lin_colors(m.vertices[:, 0].astype(np.float64),
+                                   m.vertices[:, 1].astype(np.float64)).astype(np.float32),
+                     device=dev)
+    T = torch.tensor(tex.astype(np.float32), device=dev)
+    for c, t, exp, name in ((C, None, exp_c, "colours"), (None, T, exp_t, "texture"),
+                            (C, T, exp_c * exp_t, "colours x texture")):
+        imgs, fids = dr.render([cam], V, c, t, return_face_ids=True)
+        fid = fids[0].cpu().numpy()
+        img = imgs[0].cpu().numpy().astype(np.float64)
+        inner = (fid > 0) & (np.abs(X) < 0.97) & (np.abs(Y) < 0.77)
+        assert inner.sum() > 500
+        err = np.abs(img[inner] - exp[inner]) / np.maximum(np.abs(exp[inner]), 1.0)
+        print(f"[diff] closed form {name}: max rel err {err.max():.3g}", flush=True)
+        assert err.max() <= 1e-5, name
+        assert not img[fid <= 0].any()
+
+
 def _loss_setup(c4, use_tex=True):
     import torch
     mesh, cam, dr, P, colors, tex = c4

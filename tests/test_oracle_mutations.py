@@ -44,6 +44,12 @@ MUTATIONS = {
     25: "texture mapping: the first overlapping face wins",
     26: "texture mapping: padding reads the texture being padded",
     27: "texture mapping: bilinear weights swapped",
+    28: "diff model: affine instead of perspective-correct weights (1/Z dropped)",
+    29: "diff model: u and v swapped",
+    30: "diff model: texel row not flipped (v instead of 1 - v)",
+    31: "diff model: bilinear weights lu/lv swapped",
+    32: "diff backward: the weights' 1/Z dependence dropped from the position gradient",
+    33: "diff backward: texture gradient ignores the colour factor",
 }
 
 

@@ -29,9 +29,12 @@ def test_vertexwise_faces_follow_vertices():
             vis = all(info.vertex_info_list[v].VisibleFrom(kf.id) >= 0 for v in face)
             has = any(r.kf_id == kf.id for r in info.face_info_list[f].visible_keyframes)
             assert vis == has, f"face {f} keyframe {kf.id}"
-    # face records average their corners' angles and distances
+    # the statistics point at the largest projected area (max_area itself
+    # keeps its initial value, as in the reference's FaceInfo::CalcStat)
     fi = next(f for f in info.face_info_list if f.visible_keyframes)
-    assert fi.max_area >= max(r.area for r in fi.visible_keyframes) - 0
+    areas = [r.area for r in fi.visible_keyframes]
+    assert areas[fi.max_area_index] == max(areas)
+    assert fi.max_area == -1.0
 
 
 def test_facewise_first_face_and_short_circuit():
@@ -77,6 +80,8 @@ def test_statistics_helpers():
     w = [1.0, 1.0, 2.0]
     exp = np.sum(np.array(cols, np.float64) * (np.float32(w) / np.float32(4))[:, None], 0)
     assert np.allclose(V.weighted_average(cols, w), exp)
-    # weights 0.25, 0.25, 0.5 sorted ascending: running sums 0.25, 0.5, 1.0 -> third
-    assert np.array_equal(V.weighted_median(cols, w), cols[2])
+    # weights (1, 1, 2) + 1e-6 each over their float sum 4.0000029: 0.25000006
+    # twice, so the running sum passes 0.5 at the second entry (without the
+    # 1e-6 it would reach exactly 0.5 there and pick the third)
+    assert np.array_equal(V.weighted_median(cols, w), cols[1])
     assert np.allclose(V.normalize_weights([0.0, 0.0]), [0.5, 0.5])
