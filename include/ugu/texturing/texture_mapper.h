@@ -1,7 +1,9 @@
-// Subset of ugu/texturing/texture_mapper.h (UGU): TextureMapping with
-// kSimpleProjection and kUseOriginalMeshUv, the faces rasterized into the
-// texture on the GPU (ugu_hip_texture_map). The atlas generators
-// (kGenerateSimpleTile / Triangles / Charts) are not provided.
+// ugu/texturing/texture_mapper.h (UGU): TextureMapping with kSimpleProjection
+// for kGenerateSimpleTile (keyframes tiled into one texture, uv from the
+// projected triangles) and kUseOriginalMeshUv (faces rasterized into the
+// mesh's own uv layout on the GPU, ugu_hip_texture_map). The packing atlas
+// generators kGenerateSimpleTriangles / kGenerateSimpleCharts are not
+// provided: TextureMapping reports them as not implemented.
 #pragma once
 
 #include <memory>
@@ -12,8 +14,10 @@
 
 namespace ugu {
 
-enum class TextureMappingType { kSimpleProjection = 0 };
-enum class TexturingOutputUvType {
+// Plain (unscoped) enums as in texture_mapper.h:12-19, so ugu::kUseOriginalMeshUv
+// names the value directly.
+enum TextureMappingType { kSimpleProjection = 0 };
+enum TexturingOutputUvType {
   kGenerateSimpleTile = 0,
   kUseOriginalMeshUv = 1,
   kGenerateSimpleTriangles = 2,

@@ -1,27 +1,39 @@
-// Subset of ugu/texturing/visibility_tester.h (UGU): the visibility tester
-// and its records, with the per-vertex test on the GPU
-// (ugu_amd/cpp/texturing.cc over ugu_hip_visibility_test). Same names and
-// meaning as UGU; the colour-mode statistics (Mode, occurrence maps) and the
-// custom-statistics callback are not provided.
+// ugu/texturing/visibility_tester.h (UGU): the visibility tester and its
+// records, with the per-vertex test on the GPU (ugu_amd/cpp/texturing.cc over
+// ugu_hip_visibility_test). Same names, members and meaning as UGU.
 #pragma once
 
 #include <array>
+#include <functional>
+#include <iostream>
 #include <limits>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "ugu/camera.h"
 #include "ugu/common.h"
+#include "ugu/eigen_util.h"
 #include "ugu/image.h"
 #include "ugu/mesh.h"
 
 namespace ugu {
 
+// ViewSelectionCriteria (visibility_tester.h:22-35)
 enum class ViewSelectionCriteria {
   kMinViewingAngle = 0,
   kMinDistance = 1,
+  kMeanViewingAngle = 2,
+  kMedianViewingAngle = 3,
+  kMeanDistance = 4,
+  kMedianDistance = 5,
   kMaxArea = 6,
+  kMinIntensity = 7,
+  kMedianIntensity = 8,
+  kMode = 9,  // meaningful only for discrete keyframe colours (class ids)
+  kModeViewingAngle = 10,
+  kCustom = 11
 };
 
 // VisibilityTesterOption (visibility_tester.h:38-57)
@@ -64,11 +76,24 @@ struct VertexInfo {
   Eigen::Vector3f median_intensity_color, min_intensity_color;
   float mean_viewing_angle{-1.0f}, median_viewing_angle{-1.0f};
   float mean_distance{-1.0f}, median_distance{-1.0f};
+  // the most frequent colour (plain and weighted by inverse viewing angle)
+  Eigen::Vector3f mode;
+  int mode_frequency{-1};
+  Eigen::Vector3f mode_viewing_angle_color;
+  float mode_frequency_viewing_angle_color{-1.f};
+  std::unordered_map<Eigen::Vector3f, int> occurrence;
+  std::unordered_map<Eigen::Vector3f, float> occurrence_viewing_angle;
+  // free for the custom statistics callback
+  std::vector<std::vector<Eigen::Vector3f>> customs;
+  Eigen::Vector3f custom_best;
 
   void Update(const VertexInfoPerKeyframe& info) { visible_keyframes.push_back(info); }
-  void CalcStat();
+  void CalcStat(std::function<void(VertexInfo&)> vert_custom_func = nullptr);
   int VisibleFrom(int kf_id) const;
 };
+
+std::ostream& operator<<(std::ostream& os, const VertexInfoPerKeyframe& vi);
+std::ostream& operator<<(std::ostream& os, const VertexInfo& vi);
 
 struct FaceInfoPerKeyframe {
   int kf_id{-1};
@@ -101,8 +126,9 @@ struct VisibilityInfo {
   VisibilityInfo() {}
   explicit VisibilityInfo(const Mesh& mesh)
       : vertex_info_list(mesh.vertices().size()), face_info_list(mesh.vertex_indices().size()) {}
-  void CalcStatVertexInfo();
+  void CalcStatVertexInfo(std::function<void(VertexInfo&)> vert_custom_func = nullptr);
   void CalcStatFaceInfo();
+  std::string SerializeAsJson() const;
   void Update(int vertex_id, const VertexInfoPerKeyframe& info) {
     vertex_info_list[size_t(vertex_id)].Update(info);
   }
@@ -142,9 +168,10 @@ class VisibilityTester {
   void set_keyframe(std::shared_ptr<Keyframe> keyframe);
   // facewise=true: each corner tested 1e-3 towards its face centre; a
   // vertex's record comes from the first face (lowest id) that passes it
-  bool Test(VisibilityInfo* info, bool facewise = true) const;
+  bool Test(VisibilityInfo* info, bool facewise = true,
+            std::function<void(VertexInfo&)> vert_custom_func = nullptr) const;
   bool Test(std::vector<std::shared_ptr<Keyframe>> keyframes, VisibilityInfo* info,
-            bool facewise = true);
+            bool facewise = true, std::function<void(VertexInfo&)> vert_custom_func = nullptr);
 };
 
 }  // namespace ugu

@@ -12,6 +12,8 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <sstream>
+#include <unordered_map>
 
 #include "gpu_common.h"
 #include "ugu/texturing/texture_mapper.h"
@@ -24,64 +26,83 @@ namespace {
 using gpu_detail::device_ordinal;
 using gpu_detail::to_abi;
 
-// NormalizeWeights / WeightedAverage / WeightedMedian / Median / MedianColor
-// (include/ugu/util/math_util.h:29-126, src/util/math_util.cc:44-56)
-void NormalizeWeights(const std::vector<float>& weights, std::vector<float>* out) {
-  *out = weights;
-  const float sum = std::accumulate(out->begin(), out->end(), 0.0f);
-  if (sum > 0.000001) {
-    for (float& n : *out) n /= sum;
-  } else {
-    const float val = 1.0f / static_cast<float>(out->size());
-    std::fill(out->begin(), out->end(), val);
+// The statistics of VertexInfo::CalcStat (visibility_tester.cc:68-153 with
+// math_util.h:29-181 and math_util.cc:44-56), from their definitions:
+//   weights      w_i + 1e-6f, divided by their float sum taken in order; all
+//                1/n when that sum is not above 1e-6
+//   weighted     average: per channel the double sum of float(c_i * w_i),
+//                rounded to float; median: the entries ordered by weight
+//                (std::sort), the first whose running float sum of weights
+//                exceeds 0.5 (the lightest when none does)
+//   median       n = size/2; even sizes (or force_org_val) the n-th smallest;
+//                odd sizes the mean of the (n+1)-th smallest and whatever
+//                std::nth_element leaves at position n (so a libstdc++ build
+//                picks the reference's element); sizes 1 and 2 directly
+//   mode         the first value, in input order, whose running (weighted)
+//                count becomes the largest seen
+std::vector<float> normalized(const std::vector<float>& w) {
+  const float eps = 0.000001f;
+  std::vector<float> out(w.size());
+  float total = 0.0f;
+  for (size_t i = 0; i < w.size(); i++) {
+    out[i] = w[i] + eps;
+    total += out[i];
   }
+  const bool usable = total > 0.000001;
+  for (float& x : out) x = usable ? x / total : 1.0f / static_cast<float>(w.size());
+  return out;
 }
 
-Eigen::Vector3f WeightedAverage(const std::vector<Eigen::Vector3f>& data,
-                                const std::vector<float>& weights) {
-  std::vector<float> nw;
-  NormalizeWeights(weights, &nw);
+Eigen::Vector3f weighted_mean(const std::vector<Eigen::Vector3f>& c, const std::vector<float>& w) {
+  const std::vector<float> nw = normalized(w);
   double acc[3] = {0.0, 0.0, 0.0};
-  for (size_t i = 0; i < data.size(); i++)
-    for (int k = 0; k < 3; k++) acc[k] += (data[i][k] * nw[i]);
+  for (size_t i = 0; i < c.size(); i++)
+    for (int k = 0; k < 3; k++) acc[k] += static_cast<float>(c[i][k] * nw[i]);
   return Eigen::Vector3f(static_cast<float>(acc[0]), static_cast<float>(acc[1]),
                          static_cast<float>(acc[2]));
 }
 
-Eigen::Vector3f WeightedMedian(const std::vector<Eigen::Vector3f>& data,
-                               const std::vector<float>& weights) {
-  std::vector<float> nw;
-  NormalizeWeights(weights, &nw);
-  std::vector<std::pair<float, Eigen::Vector3f>> dw;
-  for (size_t i = 0; i < data.size(); i++) dw.push_back(std::make_pair(nw[i], data[i]));
-  std::sort(dw.begin(), dw.end(),
+Eigen::Vector3f weighted_median(const std::vector<Eigen::Vector3f>& c,
+                                const std::vector<float>& w) {
+  const std::vector<float> nw = normalized(w);
+  std::vector<std::pair<float, Eigen::Vector3f>> by_weight;
+  by_weight.reserve(c.size());
+  for (size_t i = 0; i < c.size(); i++) by_weight.emplace_back(nw[i], c[i]);
+  std::sort(by_weight.begin(), by_weight.end(),
             [](const std::pair<float, Eigen::Vector3f>& a,
                const std::pair<float, Eigen::Vector3f>& b) { return a.first < b.first; });
-  float sum = 0.0f;
-  size_t index = 0;
-  for (size_t i = 0; i < dw.size(); i++) {
-    sum += dw[i].first;
-    if (sum > 0.5f) {
-      index = i;
-      break;
+  float running = 0.0f;
+  for (const auto& e : by_weight) {
+    running += e.first;
+    if (running > 0.5f) return e.second;
+  }
+  return by_weight.front().second;
+}
+
+float median_of(std::vector<float> d, bool force_org_val = false) {
+  if (d.size() == 1) return d[0];
+  if (d.size() == 2) return force_org_val ? d[0] : (d[0] + d[1]) * 0.5f;
+  const size_t n = d.size() / 2;
+  const bool single = force_org_val || d.size() % 2 == 0;
+  std::nth_element(d.begin(), d.begin() + long(single ? n : n + 1), d.end());
+  return single ? d[n] : (d[n] + d[n + 1]) * 0.5f;
+}
+
+template <typename Count>
+void mode_of(const std::vector<Eigen::Vector3f>& c, const std::vector<Count>& weight,
+             Eigen::Vector3f* mode, Count* freq,
+             std::unordered_map<Eigen::Vector3f, Count>* occurrence) {
+  *freq = 0;
+  occurrence->clear();
+  for (size_t i = 0; i < c.size(); i++) {
+    Count& n = (*occurrence)[c[i]];
+    n = n + weight[i];
+    if (*freq < n) {
+      *freq = n;
+      *mode = c[i];
     }
   }
-  return dw[index].second;
 }
-
-float Median(const std::vector<float>& data, bool force_org_val = false) {
-  if (data.size() == 1) return data[0];
-  if (data.size() == 2) return force_org_val ? data[0] : (data[0] + data[1]) * 0.5f;
-  std::vector<float> d(data);
-  const size_t n = d.size() / 2;
-  if (force_org_val || d.size() % 2 == 0) {
-    std::nth_element(d.begin(), d.begin() + long(n), d.end());
-    return d[n];
-  }
-  std::nth_element(d.begin(), d.begin() + long(n) + 1, d.end());
-  return (d[n] + d[n + 1]) * 0.5f;
-}
-
 
 // EdgeFunction of two-vectors (texture_mapper.cc:143-146)
 float Edge(const Eigen::Vector2f& a, const Eigen::Vector2f& b, const Eigen::Vector2f& c) {
@@ -100,54 +121,67 @@ bool keyframe_camera(const Camera& cam, UguCamera* c) {
 
 }  // namespace
 
-void VertexInfo::CalcStat() {
-  if (visible_keyframes.empty()) return;
+void VertexInfo::CalcStat(std::function<void(VertexInfo&)> vert_custom_func) {
+  const size_t n = visible_keyframes.size();
+  if (n == 0) return;
+  std::vector<Eigen::Vector3f> colors(n);
+  std::vector<float> dist(n), angle(n), inten(n), inv_dist(n), inv_angle(n);
   mean_color = Eigen::Vector3f(0, 0, 0);
   mean_distance = 0;
   mean_viewing_angle = 0;
-  for (auto& kf : visible_keyframes) {
+  for (size_t i = 0; i < n; i++) {
+    const VertexInfoPerKeyframe& kf = visible_keyframes[i];
+    colors[i] = kf.color;
+    dist[i] = kf.distance;
+    angle[i] = kf.viewing_angle;
+    inten[i] = kf.intensity;
+    inv_dist[i] = 1.0f / kf.distance;
+    inv_angle[i] = 1.0f / kf.viewing_angle;
     mean_color += kf.color;
     mean_distance += kf.distance;
     mean_viewing_angle += kf.viewing_angle;
   }
-  mean_color /= static_cast<float>(visible_keyframes.size());
-  mean_distance /= visible_keyframes.size();
-  mean_viewing_angle /= visible_keyframes.size();
-  std::vector<Eigen::Vector3f> colors;
-  std::vector<float> distances, viewing_angles, intensities, inv_distances, inv_viewing_angles;
-  for (auto& kf : visible_keyframes) {
-    colors.push_back(kf.color);
-    distances.push_back(kf.distance);
-    viewing_angles.push_back(kf.viewing_angle);
-    intensities.push_back(kf.intensity);
-    inv_distances.push_back(1.0f / kf.distance);
-    inv_viewing_angles.push_back(1.0f / kf.viewing_angle);
-  }
-  min_viewing_angle_index = static_cast<int>(
-      std::min_element(viewing_angles.begin(), viewing_angles.end()) - viewing_angles.begin());
+  mean_color /= static_cast<float>(n);
+  mean_distance /= n;
+  mean_viewing_angle /= n;
+
+  // first minimum, as std::min_element
+  auto argmin = [](const std::vector<float>& v) {
+    return static_cast<int>(std::min_element(v.begin(), v.end()) - v.begin());
+  };
+  min_viewing_angle_index = argmin(angle);
   min_viewing_angle_id = visible_keyframes[size_t(min_viewing_angle_index)].kf_id;
-  min_viewing_angle_color = visible_keyframes[size_t(min_viewing_angle_index)].color;
-  min_distance_index =
-      static_cast<int>(std::min_element(distances.begin(), distances.end()) - distances.begin());
+  min_viewing_angle_color = colors[size_t(min_viewing_angle_index)];
+  min_distance_index = argmin(dist);
   min_distance_id = visible_keyframes[size_t(min_distance_index)].kf_id;
-  min_distance_color = visible_keyframes[size_t(min_distance_index)].color;
-  median_viewing_angle = Median(viewing_angles);
-  median_distance = Median(distances);
+  min_distance_color = colors[size_t(min_distance_index)];
+
+  median_viewing_angle = median_of(angle);
+  median_distance = median_of(dist);
   for (int k = 0; k < 3; k++) {
-    std::vector<float> ch;
-    for (const auto& c : colors) ch.push_back(c[k]);
-    median_color[k] = Median(ch);
+    std::vector<float> channel(n);
+    for (size_t i = 0; i < n; i++) channel[i] = colors[i][k];
+    median_color[k] = median_of(channel);
   }
-  mean_viewing_angle_color = WeightedAverage(colors, inv_viewing_angles);
-  mean_distance_color = WeightedAverage(colors, inv_distances);
-  median_viewing_angle_color = WeightedMedian(colors, inv_viewing_angles);
-  median_distance_color = WeightedMedian(colors, inv_distances);
-  const auto min_it = std::min_element(intensities.begin(), intensities.end());
-  min_intensity = *min_it;
-  min_intensity_color = colors[size_t(min_it - intensities.begin())];
-  median_intensity = Median(intensities, true);
-  const auto med_it = std::find(intensities.begin(), intensities.end(), median_intensity);
-  median_intensity_color = colors[size_t(med_it - intensities.begin())];
+
+  mean_viewing_angle_color = weighted_mean(colors, inv_angle);
+  mean_distance_color = weighted_mean(colors, inv_dist);
+  median_viewing_angle_color = weighted_median(colors, inv_angle);
+  median_distance_color = weighted_median(colors, inv_dist);
+
+  const int imin = argmin(inten);
+  min_intensity = inten[size_t(imin)];
+  min_intensity_color = colors[size_t(imin)];
+  median_intensity = median_of(inten, true);
+  // the colour of the first keyframe with that intensity
+  median_intensity_color =
+      colors[size_t(std::find(inten.begin(), inten.end(), median_intensity) - inten.begin())];
+
+  mode_of(colors, std::vector<int>(n, 1), &mode, &mode_frequency, &occurrence);
+  mode_of(colors, inv_angle, &mode_viewing_angle_color, &mode_frequency_viewing_angle_color,
+          &occurrence_viewing_angle);
+
+  if (vert_custom_func != nullptr) vert_custom_func(*this);
 }
 
 int VertexInfo::VisibleFrom(int kf_id) const {
@@ -156,35 +190,58 @@ int VertexInfo::VisibleFrom(int kf_id) const {
   return -1;
 }
 
+// FaceInfo::CalcStat (visibility_tester.cc:185-224) sets the indices and the
+// keyframe ids only; max_area, min_viewing_angle and min_distance keep their
+// initial values there, and so they do here.
 void FaceInfo::CalcStat() {
-  if (visible_keyframes.empty()) return;
-  std::vector<float> distances, viewing_angles, areas;
-  for (auto& kf : visible_keyframes) {
-    distances.push_back(kf.distance);
-    viewing_angles.push_back(kf.viewing_angle);
-    areas.push_back(kf.area);
+  const size_t n = visible_keyframes.size();
+  if (n == 0) return;
+  size_t imax_area = 0, imin_angle = 0, imin_dist = 0;
+  for (size_t i = 1; i < n; i++) {
+    const FaceInfoPerKeyframe& k = visible_keyframes[i];
+    if (visible_keyframes[imax_area].area < k.area) imax_area = i;
+    if (k.viewing_angle < visible_keyframes[imin_angle].viewing_angle) imin_angle = i;
+    if (k.distance < visible_keyframes[imin_dist].distance) imin_dist = i;
   }
-  max_area_index = static_cast<int>(std::max_element(areas.begin(), areas.end()) - areas.begin());
-  max_area_id = visible_keyframes[size_t(max_area_index)].kf_id;
-  max_area = areas[size_t(max_area_index)];
-  min_viewing_angle_index = static_cast<int>(
-      std::min_element(viewing_angles.begin(), viewing_angles.end()) - viewing_angles.begin());
-  min_viewing_angle_id = visible_keyframes[size_t(min_viewing_angle_index)].kf_id;
-  min_viewing_angle = viewing_angles[size_t(min_viewing_angle_index)];
-  min_distance_index =
-      static_cast<int>(std::min_element(distances.begin(), distances.end()) - distances.begin());
-  min_distance_id = visible_keyframes[size_t(min_distance_index)].kf_id;
-  min_distance = distances[size_t(min_distance_index)];
+  max_area_index = static_cast<int>(imax_area);
+  max_area_id = visible_keyframes[imax_area].kf_id;
+  min_viewing_angle_index = static_cast<int>(imin_angle);
+  min_viewing_angle_id = visible_keyframes[imin_angle].kf_id;
+  min_distance_index = static_cast<int>(imin_dist);
+  min_distance_id = visible_keyframes[imin_dist].kf_id;
 }
 
-void VisibilityInfo::CalcStatVertexInfo() {
-  for (auto& v : vertex_info_list) v.CalcStat();
+void VisibilityInfo::CalcStatVertexInfo(std::function<void(VertexInfo&)> vert_custom_func) {
+  for (auto& v : vertex_info_list) v.CalcStat(vert_custom_func);
   has_vertex_stat = true;
 }
 
 void VisibilityInfo::CalcStatFaceInfo() {
   for (auto& f : face_info_list) f.CalcStat();
   has_face_stat = true;
+}
+
+std::ostream& operator<<(std::ostream& os, const VertexInfoPerKeyframe& vi) {
+  os << "{ \"kf_id\": " << vi.kf_id << ", \"projected_pos\":[" << vi.projected_pos[0] << ","
+     << vi.projected_pos[1] << "]," << "\"viewing_angle\":" << vi.viewing_angle << ","
+     << "\"distance\":" << vi.distance << "}";
+  return os;
+}
+
+std::ostream& operator<<(std::ostream& os, const VertexInfo& vi) {
+  os << "{\"visible_keyframes\":[";
+  for (size_t i = 0; i < vi.visible_keyframes.size(); i++)
+    os << (i ? "," : "") << vi.visible_keyframes[i];
+  os << "]}";
+  return os;
+}
+
+std::string VisibilityInfo::SerializeAsJson() const {
+  std::stringstream ss;
+  ss << "{\"vertex_info_list\":[";
+  for (size_t i = 0; i < vertex_info_list.size(); i++) ss << (i ? "," : "") << vertex_info_list[i];
+  ss << "]}";
+  return ss.str();
 }
 
 class VisibilityTester::Impl {
@@ -332,7 +389,8 @@ class VisibilityTester::Impl {
     return fi;
   }
 
-  bool Test(VisibilityInfo* info, bool facewise) const {
+  bool Test(VisibilityInfo* info, bool facewise,
+            const std::function<void(VertexInfo&)>& custom) const {
     if (!Validate(info)) return false;
     const int kf_id = keyframe_->id;
     if (facewise) {
@@ -369,7 +427,7 @@ class VisibilityTester::Impl {
         }
         if (ok == 3) info->Update(static_cast<int>(f), FaceRecord(kf_id, static_cast<int>(f), cp));
       }
-      if (option_.calc_stat_vertex_info) info->CalcStatVertexInfo();
+      if (option_.calc_stat_vertex_info) info->CalcStatVertexInfo(custom);
       if (option_.calc_stat_face_info) info->CalcStatFaceInfo();
       return true;
     }
@@ -378,7 +436,7 @@ class VisibilityTester::Impl {
     if (!TestItems(vertices_, vertices_, normals_, &rec)) return false;
     for (size_t i = 0; i < rec.size(); i++)
       if (rec[i].visible) info->Update(static_cast<int>(i), Record(kf_id, rec[i]));
-    if (option_.calc_stat_vertex_info) info->CalcStatVertexInfo();
+    if (option_.calc_stat_vertex_info) info->CalcStatVertexInfo(custom);
     if (option_.collect_face_info) {
       for (size_t f = 0; f < indices_.size(); f++) {
         const VertexInfoPerKeyframe* cp[3];
@@ -417,11 +475,12 @@ bool VisibilityTester::PrepareData() { return pimpl_->PrepareData(); }
 void VisibilityTester::set_keyframe(std::shared_ptr<Keyframe> keyframe) {
   pimpl_->keyframe_ = keyframe;
 }
-bool VisibilityTester::Test(VisibilityInfo* info, bool facewise) const {
-  return pimpl_->Test(info, facewise);
+bool VisibilityTester::Test(VisibilityInfo* info, bool facewise,
+                            std::function<void(VertexInfo&)> vert_custom_func) const {
+  return pimpl_->Test(info, facewise, vert_custom_func);
 }
 bool VisibilityTester::Test(std::vector<std::shared_ptr<Keyframe>> keyframes, VisibilityInfo* info,
-                            bool facewise) {
+                            bool facewise, std::function<void(VertexInfo&)> vert_custom_func) {
   // visibility_tester.cc:693-727: statistics only after the last keyframe
   if (keyframes.empty()) return false;
   bool ret = true;
@@ -431,60 +490,106 @@ bool VisibilityTester::Test(std::vector<std::shared_ptr<Keyframe>> keyframes, Vi
   pimpl_->option_.calc_stat_vertex_info = false;
   for (size_t i = 0; i + 1 < keyframes.size(); i++) {
     set_keyframe(keyframes[i]);
-    if (!Test(info, facewise)) {
+    if (!Test(info, facewise, vert_custom_func)) {
       ret = false;
       break;
     }
   }
   set_option(org);
   set_keyframe(keyframes.back());
-  if (!Test(info, facewise)) ret = false;
+  if (!Test(info, facewise, vert_custom_func)) ret = false;
   return ret;
 }
 
-bool TextureMapping(const std::vector<std::shared_ptr<Keyframe>>& keyframes,
-                    const VisibilityInfo& info, Mesh* mesh, const TextureMappingOption& option) {
-  if (option.type != TextureMappingType::kSimpleProjection) {
-    LOGE("TextureMappingType %d is not implemented", static_cast<int>(option.type));
-    return false;
+namespace {
+
+// kGenerateSimpleTile (texture_mapper.cc:30-136): the keyframe images laid out
+// row-major on a grid of floor(sqrt(n)) columns, and every face's uv the
+// chosen keyframe's projected corners moved into its tile, at texel centres
+// ((x + 0.5) / W, 1 - (y + 0.5) / H); faces seen by no keyframe get uv 0.
+// Departures, both where the reference's memory accesses are undefined: the
+// grid has ceil(n / columns) rows (the reference's n / columns rows leave the
+// last keyframes outside the image when n is not a multiple of the column
+// count), and keyframes must all have the first one's size.
+bool SimpleTile(const std::vector<std::shared_ptr<Keyframe>>& keyframes,
+                const std::vector<const FaceInfoPerKeyframe*>& best, Mesh* mesh,
+                const TextureMappingOption& option) {
+  const int n = static_cast<int>(keyframes.size());
+  const int w0 = keyframes[0]->color.cols, h0 = keyframes[0]->color.rows;
+  for (const auto& kf : keyframes)
+    if (kf->color.cols != w0 || kf->color.rows != h0) {
+      LOGE("kGenerateSimpleTile needs keyframes of one size\n");
+      return false;
+    }
+  const int cols = std::max(1, static_cast<int>(std::sqrt(keyframes.size())));
+  const int rows = (n + cols - 1) / cols;
+  const int tex_w = cols * w0, tex_h = rows * h0;
+  Image3b texture = Image3b::zeros(tex_h, tex_w);
+  const size_t row_bytes = size_t(w0) * 3;
+  for (int i = 0; i < n; i++) {
+    const size_t x0 = size_t(i % cols) * size_t(w0), y0 = size_t(i / cols) * size_t(h0);
+    for (int y = 0; y < h0; y++)
+      std::memcpy(texture.data + ((y0 + size_t(y)) * size_t(tex_w) + x0) * 3,
+                  keyframes[size_t(i)]->color.data + size_t(y) * row_bytes, row_bytes);
   }
-  if (option.uv_type != TexturingOutputUvType::kUseOriginalMeshUv) {
-    LOGE("OutputUvType %d is not implemented\n", static_cast<int>(option.uv_type));
-    return false;
+  // keyframe id -> list position (the first keyframe with that id); an id
+  // missing from the list lands on tile 0, as the reference's map lookup does
+  std::unordered_map<int, int> position;
+  for (int i = 0; i < n; i++) position.emplace(keyframes[size_t(i)]->id, i);
+
+  const size_t F = best.size();
+  std::vector<Eigen::Vector2f> uv(F * 3, Eigen::Vector2f(0.0f, 0.0f));
+  std::vector<Eigen::Vector3i> uv_indices(F);
+  for (size_t f = 0; f < F; f++) {
+    const int base = static_cast<int>(f * 3);
+    uv_indices[f] = Eigen::Vector3i(base, base + 1, base + 2);
+    const FaceInfoPerKeyframe* b = best[f];
+    if (b == nullptr || b->kf_id < 0) continue;
+    const auto it = position.find(b->kf_id);
+    const int slot = it == position.end() ? 0 : it->second;
+    const int ox = (slot % cols) * w0, oy = (slot / cols) * h0;
+    for (int j = 0; j < 3; j++) {
+      const float tx = b->projected_tri[size_t(j)].x() + ox;
+      const float ty = b->projected_tri[size_t(j)].y() + oy;
+      uv[f * 3 + size_t(j)] = Eigen::Vector2f((tx + 0.5f) / tex_w, 1.0f - ((ty + 0.5f) / tex_h));
+    }
   }
+  mesh->set_uv(uv);
+  mesh->set_uv_indices(uv_indices);
+  std::vector<ObjMaterial> materials(1);
+  materials[0].name = option.texture_base_name;
+  materials[0].diffuse_tex = texture;
+  mesh->set_materials(materials);
+  mesh->set_material_ids(std::vector<int>(mesh->vertex_indices().size(), 0));
+  return true;
+}
+
+// kUseOriginalMeshUv (texture_mapper.cc:295-345): every face's uv triangle
+// rasterized into the texture on the GPU, then PaddingSimple.
+bool OriginalUv(const std::vector<std::shared_ptr<Keyframe>>& keyframes,
+                const std::vector<const FaceInfoPerKeyframe*>& best, Mesh* mesh,
+                const TextureMappingOption& option) {
   if (mesh->uv().empty() || mesh->uv_indices().size() != mesh->vertex_indices().size()) {
     LOGE("OutputUvType kUseOriginalMeshUv is specified but UV on mesh is invalid\n");
     return false;
   }
-  const size_t F = info.face_info_list.size();
+  const size_t F = best.size();
   std::vector<UguTexFace> faces(F);
   for (size_t i = 0; i < F; i++) {
-    const FaceInfo& fi = info.face_info_list[i];
     UguTexFace& r = faces[i];
-    if (fi.visible_keyframes.empty()) {
+    const FaceInfoPerKeyframe* b = best[i];
+    if (b == nullptr || b->kf_id < 0) {
       r.keyframe = -1;
       continue;
     }
-    int idx;
-    if (option.criteria == ViewSelectionCriteria::kMinViewingAngle) {
-      idx = fi.min_viewing_angle_index;
-    } else if (option.criteria == ViewSelectionCriteria::kMinDistance) {
-      idx = fi.min_distance_index;
-    } else if (option.criteria == ViewSelectionCriteria::kMaxArea) {
-      idx = fi.max_area_index;
-    } else {
-      LOGE("ViewSelectionCriteria %d is not implemented\n", static_cast<int>(option.criteria));
-      return false;
-    }
-    const FaceInfoPerKeyframe& best = fi.visible_keyframes[size_t(idx)];
-    if (best.kf_id < 0 || best.kf_id >= static_cast<int>(keyframes.size())) {
+    if (b->kf_id >= static_cast<int>(keyframes.size())) {
       LOGE("keyframe ids must index the keyframe list\n");
       return false;
     }
-    r.keyframe = best.kf_id;
+    r.keyframe = b->kf_id;
     for (int j = 0; j < 3; j++) {
-      r.src[2 * j] = best.projected_tri[size_t(j)].x();
-      r.src[2 * j + 1] = best.projected_tri[size_t(j)].y();
+      r.src[2 * j] = b->projected_tri[size_t(j)].x();
+      r.src[2 * j + 1] = b->projected_tri[size_t(j)].y();
       const Eigen::Vector2f& uv = mesh->uv()[size_t(mesh->uv_indices()[i][j])];
       r.dst[2 * j] = uv.x() * option.tex_w - 0.5f;
       r.dst[2 * j + 1] = (1.0f - uv.y()) * option.tex_h - 0.5f;
@@ -513,6 +618,56 @@ bool TextureMapping(const std::vector<std::shared_ptr<Keyframe>>& keyframes,
   mesh->set_materials(materials);
   mesh->set_material_ids(std::vector<int>(mesh->vertex_indices().size(), 0));
   return true;
+}
+
+}  // namespace
+
+// TextureMapping (texture_mapper.cc:894-1017): choose each face's keyframe by
+// the criterion, then build the texture for the requested uv type.
+bool TextureMapping(const std::vector<std::shared_ptr<Keyframe>>& keyframes,
+                    const VisibilityInfo& info, Mesh* mesh, const TextureMappingOption& option) {
+  if (option.type != TextureMappingType::kSimpleProjection) {
+    LOGE("TextureMappingType %d is not implemented", static_cast<int>(option.type));
+    return false;
+  }
+  if (option.criteria != ViewSelectionCriteria::kMinViewingAngle &&
+      option.criteria != ViewSelectionCriteria::kMinDistance &&
+      option.criteria != ViewSelectionCriteria::kMaxArea) {
+    LOGE("ViewSelectionCriteria %d is not implemented\n", static_cast<int>(option.criteria));
+    return false;
+  }
+  if (keyframes.empty()) {
+    LOGE("no keyframe\n");
+    return false;
+  }
+  const size_t F = info.face_info_list.size();
+  std::vector<FaceInfo> restat;  // faces whose statistics were never computed
+  restat.reserve(F);
+  std::vector<const FaceInfoPerKeyframe*> best(F, nullptr);
+  for (size_t i = 0; i < F; i++) {
+    const FaceInfo* fi = &info.face_info_list[i];
+    if (fi->visible_keyframes.empty()) continue;
+    if (fi->max_area_index < 0) {
+      restat.push_back(*fi);
+      restat.back().CalcStat();
+      fi = &restat.back();
+    }
+    int idx = fi->max_area_index;
+    if (option.criteria == ViewSelectionCriteria::kMinViewingAngle)
+      idx = fi->min_viewing_angle_index;
+    else if (option.criteria == ViewSelectionCriteria::kMinDistance)
+      idx = fi->min_distance_index;
+    best[i] = &info.face_info_list[i].visible_keyframes[size_t(idx)];
+  }
+  switch (option.uv_type) {
+    case kGenerateSimpleTile:
+      return SimpleTile(keyframes, best, mesh, option);
+    case kUseOriginalMeshUv:
+      return OriginalUv(keyframes, best, mesh, option);
+    default:
+      LOGE("OutputUvType %d is not implemented\n", static_cast<int>(option.uv_type));
+      return false;
+  }
 }
 
 }  // namespace ugu

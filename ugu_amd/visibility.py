@@ -110,14 +110,33 @@ class FaceInfoPerKeyframe:
 
 
 def normalize_weights(weights):
-    """NormalizeWeights (math_util.h:29-48) in float."""
-    w = np.asarray(weights, np.float32)
+    """NormalizeWeights (math_util.h:29-48) in float: every weight plus 1e-6,
+    divided by the float sum of those taken in order; all 1/n when that sum
+    is not above 1e-6."""
+    w = np.asarray(weights, np.float32) + np.float32(0.000001)
     s = np.float32(0.0)
     for x in w:
         s = np.float32(s + x)
     if s > 0.000001:
         return (w / s).astype(np.float32)
     return np.full(len(w), np.float32(1.0) / np.float32(len(w)), np.float32)
+
+
+def mode(colors, weights=None):
+    """Mode (math_util.h:130-181): the first colour, in input order, whose
+    running (weighted) count becomes the largest so far. Returns (mode,
+    frequency, occurrence dict keyed by the colour's float tuple)."""
+    occ = {}
+    best, freq = None, (0 if weights is None else np.float32(0.0))
+    for i, c in enumerate(colors):
+        key = tuple(float(x) for x in np.asarray(c, np.float32))
+        w = 1 if weights is None else np.float32(weights[i])
+        occ[key] = (occ[key] + w) if key in occ else w
+        if weights is not None:
+            occ[key] = np.float32(occ[key])
+        if freq < occ[key]:
+            freq, best = occ[key], np.asarray(c, np.float32)
+    return best, freq, occ
 
 
 def weighted_average(colors, weights):
@@ -172,12 +191,14 @@ class VertexInfo:
                 return i
         return -1
 
-    def CalcStat(self):
-        """VertexInfo::CalcStat (visibility_tester.cc:68-158) without Mode.
-        Distances of FLT_MAX (no hit) overflow float sums to inf, as in the
-        reference."""
+    def CalcStat(self, vert_custom_func=None):
+        """VertexInfo::CalcStat (visibility_tester.cc:68-153). Distances of
+        FLT_MAX (no hit) overflow float sums to inf, as in the reference.
+        ``vert_custom_func(self)`` runs last when given."""
         with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
             self._calc_stat()
+        if vert_custom_func is not None and self.visible_keyframes:
+            vert_custom_func(self)
 
     def _calc_stat(self):
         kfs = self.visible_keyframes
@@ -218,26 +239,35 @@ class VertexInfo:
         self.min_intensity, self.min_intensity_color = inten[i], colors[i]
         self.median_intensity = median(inten, True)
         self.median_intensity_color = colors[inten.index(self.median_intensity)]
+        self.mode, self.mode_frequency, self.occurrence = mode(colors)
+        (self.mode_viewing_angle_color, self.mode_frequency_viewing_angle_color,
+         self.occurrence_viewing_angle) = mode(colors, inv_a)
 
 
 class FaceInfo:
     def __init__(self):
         self.visible_keyframes = []
+        self.max_area_index = self.max_area_id = -1
+        self.max_area = np.float32(-1.0)
+        self.min_viewing_angle_index = self.min_viewing_angle_id = -1
+        self.min_viewing_angle = np.float32(np.finfo(np.float32).max)
+        self.min_distance_index = self.min_distance_id = -1
+        self.min_distance = np.float32(np.finfo(np.float32).max)
 
     def CalcStat(self):
-        """FaceInfo::CalcStat (visibility_tester.cc:185-224)."""
+        """FaceInfo::CalcStat (visibility_tester.cc:185-224): the indices and
+        keyframe ids of the largest area and the smallest angle and distance
+        (first one on ties); max_area, min_viewing_angle and min_distance keep
+        their initial values, as in the reference."""
         kfs = self.visible_keyframes
         if not kfs:
             return
         i = int(np.argmax([k.area for k in kfs]))
         self.max_area_index, self.max_area_id = i, kfs[i].kf_id
-        self.max_area = kfs[i].area
         i = int(np.argmin([k.viewing_angle for k in kfs]))
         self.min_viewing_angle_index, self.min_viewing_angle_id = i, kfs[i].kf_id
-        self.min_viewing_angle = kfs[i].viewing_angle
         i = int(np.argmin([k.distance for k in kfs]))
         self.min_distance_index, self.min_distance_id = i, kfs[i].kf_id
-        self.min_distance = kfs[i].distance
 
 
 class VisibilityInfo:
@@ -251,9 +281,9 @@ class VisibilityInfo:
         self.has_vertex_stat = False
         self.has_face_stat = False
 
-    def CalcStatVertexInfo(self):
+    def CalcStatVertexInfo(self, vert_custom_func=None):
         for v in self.vertex_info_list:
-            v.CalcStat()
+            v.CalcStat(vert_custom_func)
         self.has_vertex_stat = True
 
     def CalcStatFaceInfo(self):
@@ -411,10 +441,13 @@ class VisibilityTester:
         nrm = self.normals_[self.indices_].reshape(-1, 3)
         return pos, v.reshape(-1, 3), nrm
 
-    def Test(self, info, facewise=True, keyframes=None):
+    def Test(self, info, facewise=True, keyframes=None, vert_custom_func=None):
         """Test(info, facewise) for the current keyframe, or, with
         `keyframes`, Test(keyframes, info, facewise) (statistics only after
-        the last keyframe, visibility_tester.cc:693-727)."""
+        the last keyframe, visibility_tester.cc:693-727).
+        ``vert_custom_func(vertex_info)`` runs at the end of every vertex's
+        CalcStat."""
+        cf = vert_custom_func
         if keyframes is not None:
             org = VisibilityTesterOption()
             self.option_.CopyTo(org)
@@ -423,12 +456,12 @@ class VisibilityTester:
             ok = True
             for kf in keyframes[:-1]:
                 self.set_keyframe(kf)
-                if not self.Test(info, facewise):
+                if not self.Test(info, facewise, vert_custom_func=cf):
                     ok = False
                     break
             self.set_option(org)
             self.set_keyframe(keyframes[-1])
-            return self.Test(info, facewise) and ok
+            return self.Test(info, facewise, vert_custom_func=cf) and ok
         if not self._validate(info):
             return False
         kf_id = self.keyframe_.id
@@ -452,7 +485,7 @@ class VisibilityTester:
                     info.face_info_list[f].visible_keyframes.append(
                         _face_record(kf_id, f, corners))
             if o.calc_stat_vertex_info:
-                info.CalcStatVertexInfo()
+                info.CalcStatVertexInfo(cf)
             if o.calc_stat_face_info:
                 info.CalcStatFaceInfo()
             return True
@@ -460,7 +493,7 @@ class VisibilityTester:
         for i in np.nonzero(rec["visible"])[0]:
             info.vertex_info_list[i].visible_keyframes.append(VertexInfoPerKeyframe(kf_id, rec[i]))
         if o.calc_stat_vertex_info:
-            info.CalcStatVertexInfo()
+            info.CalcStatVertexInfo(cf)
         if o.collect_face_info:
             for f, face in enumerate(self.indices_):
                 corners = []
