@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# UGU_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of tests/sanitize); the default is the one `make -C oracle` writes.
+LIB_PATH = os.environ.get("UGU_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 _P = ctypes.c_void_p
 _i = ctypes.c_int32

@@ -18,7 +18,9 @@ from PIL import Image
 from ugu_amd import io as uio
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-TOOL = os.path.join(ROOT, "ugu_amd", "cpp", "ugu_io_tool")
+# UGU_IO_TOOL: another build of the same driver (tests/sanitize builds one
+# under AddressSanitizer and UBSan)
+TOOL = os.environ.get("UGU_IO_TOOL") or os.path.join(ROOT, "ugu_amd", "cpp", "ugu_io_tool")
 DATA = os.path.join(ROOT, "tests", "golden", "data")
 
 KINDS = {"1b": (np.uint8, 1), "3b": (np.uint8, 3), "4b": (np.uint8, 4), "1w": (np.uint16, 1)}

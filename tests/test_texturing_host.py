@@ -14,7 +14,8 @@ from ugu_amd import texturing as T
 from ugu_amd import visibility as V
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-TOOL = os.path.join(ROOT, "ugu_amd", "cpp", "ugu_stats_tool")
+# UGU_STATS_TOOL: another build of the same driver (tests/sanitize)
+TOOL = os.environ.get("UGU_STATS_TOOL") or os.path.join(ROOT, "ugu_amd", "cpp", "ugu_stats_tool")
 FLT_MAX = float(np.finfo(np.float32).max)
 
 C0, C1, C2 = (200.0, 100.0, 50.0), (10.0, 20.0, 30.0), (200.0, 100.0, 50.0)

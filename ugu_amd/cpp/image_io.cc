@@ -127,6 +127,13 @@ bool decode_png(const std::vector<uint8_t>& f, bool out8, Decoded* d, std::strin
   const size_t W = size_t(d->w), H = size_t(d->h);
   const size_t row = (W * size_t(n) * size_t(bd) + 7) / 8;
   const size_t bpp = std::max<size_t>(1, size_t(n) * size_t(bd) / 8);
+  // Deflate expands at most 1032:1 (258-byte matches coded in two bits), so
+  // a header whose rows cannot come out of the compressed data is rejected
+  // before anything that large is allocated.
+  if ((row + 1) * H > idat.size() * 1032 + 1024) {
+    *err = "image data too short for the header's size";
+    return false;
+  }
   std::vector<uint8_t> raw((row + 1) * H);
   uLongf raw_len = uLongf(raw.size());
   if (uncompress(raw.data(), &raw_len, idat.data(), uLong(idat.size())) != Z_OK ||
