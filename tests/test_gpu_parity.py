@@ -235,6 +235,20 @@ def test_subnormal_depth_sentinel(gpu_available, flip, cull):
         _compare(out, ref, f"subnormal flip={flip} cull={cull} tile{tile}")
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_subnormal_depths_in_dense_tiles(gpu_available, seed):
+    """Hundreds of faces per tile, many of them with a corner at a subnormal
+    depth (pixel depth 0 < FLT_MIN): the slow path's two passes over a large
+    bin against the oracle's sequential loop, for every tile shape."""
+    mesh, cam = TS.random_subnormal_layers(600, 40, 24, seed)
+    for cull in (False, True):
+        opt = RendererCpuOption(diffuse_color=DiffuseColor.kVertex, backface_culling=cull)
+        ref = _oracle(mesh, cam, opt)
+        for tile in ((8, 8), (16, 16), (32, 8)):
+            out = _render(mesh, cam, opt, tile, cap=64)
+            _compare(out, ref, f"dense subnormal seed{seed} cull={cull} tile{tile}")
+
+
 def test_offscreen_and_empty(gpu_available, uv_scene):
     mesh, cam = uv_scene
     c2 = scenes.front_camera(64, 48, 500.0, 3.0)

@@ -94,6 +94,34 @@ def subnormal_depth_scene(w=16, h=16, flip=False):
     return m, cam
 
 
+def random_subnormal_layers(n, w, h, seed, p_sub=0.15):
+    """n random triangles with pixel-exact projected corners over a w x h
+    image (camera f = 2^-10 as subnormal_depth_scene), each corner at a
+    random depth in [0.5, 4] or, with probability p_sub, at the subnormal
+    depth 2^-135: many faces per tile and many pixels whose depth is 0 for
+    some covering face, in random face order (both windings)."""
+    fx = 2.0 ** -10
+    zsub = 2.0 ** -135
+    rng = np.random.default_rng(seed)
+    tris = []
+    for _ in range(n):
+        c = []
+        for _k in range(3):
+            u, v = rng.integers(-2, w + 2), rng.integers(-2, h + 2)
+            z = zsub if rng.random() < p_sub else float(rng.choice([0.5, 1.0, 1.5, 2.0, 4.0]))
+            c.append((u * z / fx, v * z / fx, z))
+        tris.append(c)
+    verts = np.asarray(tris, np.float64).reshape(-1, 3).astype(np.float32)
+    faces = np.arange(len(verts), dtype=np.int32).reshape(-1, 3)
+    m = Mesh(verts, faces)
+    m.CalcFaceNormal()
+    m.set_normals(np.repeat(m.face_normals, 3, axis=0))
+    m.set_normal_indices(faces)
+    m.set_vertex_colors(rng.random((len(verts), 3), dtype=np.float32) * 255.0)
+    cam = PinholeCamera(w, h, np.eye(4), (0.0, 0.0), (fx, fx))
+    return m, cam
+
+
 def layered_grid_scene(w=96, h=64, cell=1.5, seed=0):
     """Stacked triangle grids over the whole image, many faces per tile: the
     multi-chunk path of the rasterizer with depth-based skipping of hidden

@@ -466,7 +466,9 @@ __global__ __launch_bounds__(256) void k_setup_bin(
         for (int step = 32; step >= 1; step >>= 1)
           if (__shfl(excl, src + step) <= p) src += step;
         const int k = p - __shfl(excl, src) + kAppendRounds;
-        const int snx = __shfl(nx, src);
+        // lanes past the last pair (act false) may land on a face without
+        // tiles (nx 0): clamp so their unused division stays defined
+        const int snx = max(__shfl(nx, src), 1);
         const int sty = __shfl(ty0, src) + k / snx;
         const int stx = __shfl(tx0, src) + k % snx;
         const int sback = __shfl(back ? 1 : 0, src);
@@ -502,38 +504,80 @@ struct RasterLds {
   int4 runseg[4];              // counters of the persistent block's current run
 };
 
-// Exact sequential z-buffer for one pixel (slow path): visits the covering
-// faces of the tile's bin in increasing face id and applies the reference's
-// update rule `if (d < FLT_MIN || z < d)` (rasterizer.cc:228-237). Used only
-// for pixels where some covering face produced a depth below FLT_MIN or NaN,
-// where the packed atomicMin order is not the sequential order.
+// Exact sequential z-buffer for one pixel (slow path): the result of the
+// reference's update rule `if (d < FLT_MIN || z < d)` (rasterizer.cc:228-237),
+// applied in increasing face id over the faces of the tile's bin that cover the
+// pixel, starting from d = 0. Used only for pixels where some covering face
+// produced a depth below FLT_MIN or NaN, where the packed atomicMin order is
+// not the sequential order.
+//
+// Without NaN depths the rule has a closed form, computed in two passes over
+// the bin: a covering face with depth below FLT_MIN is always taken, so the
+// last such face E (largest id) resets the pixel, the next face is taken
+// unconditionally, and from there the strict-less rule keeps the first
+// smallest depth. The winner is therefore the smallest (depth, id) among the
+// covering faces after E with depth >= FLT_MIN, or E itself when there is none.
+// A NaN depth is taken only right after a reset and then blocks every later
+// face; when a covering face has one, the rule is simulated in id order
+// instead (one pass over the bin per covering face).
 UGU_DEV void sequential_pixel(const DevView& v, const DevMesh& mesh,
                               const int2* bin, int nA, int cap, int n,
                               int gx, int gy, float r0, float r1, float r2,
                               int& out_fid) {
+  const float x = static_cast<float>(gx), y = static_cast<float>(gy);
+  // face of bin entry i and its depth at the pixel; false when it does not
+  // cover the pixel
+  auto covers = [&](int i, int& f, float& z) {
+    f = bin_entry(bin, nA, cap, i).x;
+    float P[9], N[3];
+    load_face_rec(mesh.soup, f, P, N);
+    const float4 a = project_point(v, P[0], P[1], P[2]);
+    const float4 b = project_point(v, P[3], P[4], P[5]);
+    const float4 c = project_point(v, P[6], P[7], P[8]);
+    int x0, x1, y0, y1;
+    float area;
+    if (!face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) return false;
+    if (gx < x0 || gx > x1 || gy < y0 || gy > y1) return false;
+    const bool bf = mesh.fnormals != nullptr && sum3(N[0] * r0, N[1] * r1, N[2] * r2) > 0.0f;
+    float w0, w1, w2;
+    return cover_test(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, area, bf, x, y, z, w0, w1,
+                      w2);
+  };
+  // pass 1: the last reset face and whether any depth is NaN
+  int emax = -1;
+  bool any_nan = false;
+  for (int i = 0; i < n; i++) {
+    int f;
+    float z;
+    if (!covers(i, f, z)) continue;
+    if (z != z) any_nan = true;
+    else if (z < FLT_MIN) emax = max(emax, f);
+  }
+  if (!any_nan) {
+    // pass 2: first smallest depth after the reset
+    unsigned long long best = ~0ull;
+    for (int i = 0; i < n; i++) {
+      int f;
+      float z;
+      if (!covers(i, f, z) || f <= emax || !(z >= FLT_MIN)) continue;
+      const unsigned long long key =
+          (static_cast<unsigned long long>(__float_as_uint(z)) << 32) | static_cast<unsigned int>(f);
+      best = key < best ? key : best;
+    }
+    out_fid = best != ~0ull ? static_cast<int>(best & 0xffffffffull) : emax;
+    return;
+  }
+  // some depth is NaN: the rule in id order
   int cur = -1;
   float d = 0.0f;
   int dfid = -1;
-  const float x = static_cast<float>(gx), y = static_cast<float>(gy);
   for (;;) {
     int best = INT_MAX;
     float bestz = 0.0f;
     for (int i = 0; i < n; i++) {
-      const int f = bin_entry(bin, nA, cap, i).x;
-      if (f <= cur || f >= best) continue;
-      float P[9], N[3];
-      load_face_rec(mesh.soup, f, P, N);
-      const float4 a = project_point(v, P[0], P[1], P[2]);
-      const float4 b = project_point(v, P[3], P[4], P[5]);
-      const float4 c = project_point(v, P[6], P[7], P[8]);
-      int x0, x1, y0, y1;
-      float area;
-      if (!face_setup(a, b, c, v.W, v.H, x0, x1, y0, y1, area)) continue;
-      if (gx < x0 || gx > x1 || gy < y0 || gy > y1) continue;
-      const bool bf = mesh.fnormals != nullptr && sum3(N[0] * r0, N[1] * r1, N[2] * r2) > 0.0f;
-      float z, w0, w1, w2;
-      if (cover_test(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, area, bf, x,
-                     y, z, w0, w1, w2)) {
+      int f;
+      float z;
+      if (covers(i, f, z) && f > cur && f < best) {
         best = f;
         bestz = z;
       }
