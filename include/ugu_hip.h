@@ -272,6 +272,18 @@ int ugu_hip_autotune(ugu_hip_ctx* ctx, const UguCamera* cameras, int n,
                      const UguOptions* options, int repeats,
                      UguTiling* chosen);
 
+/* ugu_hip_autotune with a persistent choice: the tiling is looked up in the
+ * text file cache_path under a key of the device (architecture, CU count),
+ * the prepared mesh's face count, the options that change the work and the
+ * views' sizes; on a hit it is applied without rendering (*from_cache = 1),
+ * otherwise the views are tuned and one line is appended to the file (an
+ * unwritable file only loses the persistence). Tiling changes speed only:
+ * results are the same for every tiling, and a bin capacity that a later
+ * render outgrows is grown as usual. No reference counterpart. */
+int ugu_hip_autotune_cached(ugu_hip_ctx* ctx, const UguCamera* cameras, int n,
+                            const UguOptions* options, int repeats, const char* cache_path,
+                            UguTiling* chosen, int* from_cache);
+
 /* Profiling (hipEvents around each kernel on the context's stream). */
 int ugu_hip_enable_profiling(ugu_hip_ctx* ctx, int enable);
 int ugu_hip_get_timings(const ugu_hip_ctx* ctx, UguTimings* timings);

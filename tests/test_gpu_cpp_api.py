@@ -55,8 +55,9 @@ def test_cpp_rasterizer_matches_oracle(gpu_available, tmp_path, variant):
     scene.mkdir()
     out.mkdir()
     _dump_scene(str(scene), mesh, cams, opt)
+    env = dict(os.environ, UGU_HIP_TUNE_CACHE=str(tmp_path / "tiling.txt"))
     res = subprocess.run([DRIVER, str(scene), str(out)], capture_output=True, text=True,
-                         timeout=120)
+                         timeout=120, env=env)
     print(res.stdout[-2000:], res.stderr[-2000:])
     assert res.returncode == 0, res.stdout + res.stderr
     for name in ("prepare_without_mesh_fails", "render_before_prepare_fails", "all_null_fails",

@@ -13,6 +13,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libugu_hip.so")
 
+# status codes of include/ugu_hip.h (enum UguHipStatus) beyond the validation codes
+UGU_HIP_ERR_INVALID_ARGUMENT = -20
+UGU_HIP_ERR_HIP = -21
+UGU_HIP_ERR_OUT_OF_MEMORY = -22
 UGU_HIP_MEMORY_HOST = 0
 UGU_HIP_MEMORY_DEVICE = 1
 
@@ -164,6 +168,10 @@ SIGNATURES = {
     "ugu_hip_autotune": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,
                                         ctypes.POINTER(UguOptions), ctypes.c_int,
                                         ctypes.POINTER(UguTiling)]),
+    "ugu_hip_autotune_cached": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,
+                                               ctypes.POINTER(UguOptions), ctypes.c_int,
+                                               ctypes.c_char_p, ctypes.POINTER(UguTiling),
+                                               ctypes.POINTER(ctypes.c_int)]),
     "ugu_hip_enable_profiling": (ctypes.c_int, [_P, ctypes.c_int]),
     "ugu_hip_get_timings": (ctypes.c_int, [_P, ctypes.POINTER(UguTimings)]),
     "ugu_hip_depth_to_u16": (ctypes.c_int, [_P, _P, ctypes.c_int64, _P, ctypes.c_int]),

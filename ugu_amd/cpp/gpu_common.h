@@ -4,7 +4,10 @@
 #pragma once
 
 #include <cstdlib>
+#include <filesystem>
 #include <memory>
+#include <string>
+#include <system_error>
 #include <vector>
 
 #include "ugu/log.h"
@@ -17,6 +20,27 @@ namespace gpu_detail {
 inline int device_ordinal() {
   const char* s = std::getenv("UGU_HIP_DEVICE");
   return s ? std::atoi(s) : 0;
+}
+
+// File that keeps the autotuner's choices across processes:
+// $UGU_HIP_TUNE_CACHE if set (empty: no file), else
+// $XDG_CACHE_HOME/ugu_amd/tiling.txt or ~/.cache/ugu_amd/tiling.txt (the
+// directory is created when missing). "" when there is nowhere to keep it.
+inline std::string tune_cache_path() {
+  if (const char* e = std::getenv("UGU_HIP_TUNE_CACHE")) return e;
+  std::string dir;
+  if (const char* x = std::getenv("XDG_CACHE_HOME"); x && *x) {
+    dir = x;
+  } else if (const char* h = std::getenv("HOME"); h && *h) {
+    dir = std::string(h) + "/.cache";
+  } else {
+    return "";
+  }
+  dir += "/ugu_amd";
+  std::error_code ec;
+  std::filesystem::create_directories(dir, ec);
+  if (ec) return "";
+  return dir + "/tiling.txt";
 }
 
 inline UguOptions to_abi(const RendererCpuOption& o) {

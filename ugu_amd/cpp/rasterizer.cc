@@ -78,11 +78,15 @@ class Rasterizer::Impl {
                              mask[i], face_id[i]);
     const UguOptions opt = to_abi(option_);
     // tile size and bin capacity are tuned once per view set (results do
-    // not depend on them)
+    // not depend on them); the choice is kept in a file across processes, so
+    // only the first process on a device tunes a given workload
     const auto key = std::make_pair(abi[0].width * 65536 + abi[0].height, static_cast<int>(n));
     if (!tuned_.count(key)) {
       UguTiling t;
-      if (ugu_hip_autotune(ctx_, abi.data(), static_cast<int>(n), &opt, 2, &t) == UGU_HIP_OK)
+      int cached = 0;
+      const std::string path = tune_cache_path();
+      if (ugu_hip_autotune_cached(ctx_, abi.data(), static_cast<int>(n), &opt, 2,
+                                  path.c_str(), &t, &cached) == UGU_HIP_OK)
         tuned_[key] = true;
     }
     const int rc = n == 1 ? ugu_hip_render(ctx_, abi.data(), &opt, outs.data(), UGU_HIP_MEMORY_HOST)

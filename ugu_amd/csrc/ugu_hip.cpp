@@ -135,6 +135,7 @@ struct ugu_hip_ctx {
 
   UguTiling tiling{16, 16, 1024, 1};
   int cu_count = 0;
+  std::string arch;  // gcnArchName of the device (tuning-cache key)
   std::vector<GraphEntry> graphs;
 
   bool profile = false;
@@ -408,6 +409,24 @@ void set_view_planes(DevView& v, void* const p[kPlanes]) {
   v.geo_id = static_cast<int32_t*>(p[10]);
 }
 
+// Camera floats and tile grid of one view descriptor; output planes null.
+void fill_view_camera(DevView& v, const UguCamera& cam, int tw, int th, int32_t tile_base) {
+  std::memset(&v, 0, sizeof(v));
+  v.W = cam.width;
+  v.H = cam.height;
+  v.fx = cam.fx;
+  v.fy = cam.fy;
+  v.cx = cam.cx;
+  v.cy = cam.cy;
+  std::memcpy(v.w2c_R, cam.w2c_R, sizeof(v.w2c_R));
+  std::memcpy(v.w2c_t, cam.w2c_t, sizeof(v.w2c_t));
+  std::memcpy(v.c2w_R, cam.c2w_R, sizeof(v.c2w_R));
+  std::memcpy(v.c2w_t, cam.c2w_t, sizeof(v.c2w_t));
+  v.tiles_x = (v.W + tw - 1) / tw;
+  v.tiles_y = (v.H + th - 1) / th;
+  v.tile_base = tile_base;
+}
+
 int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm,
                  const DevOptions& dopt, int mode, const UguOutputs* outs,
                  int memory, float* const* images, int32_t* const* fids,
@@ -459,20 +478,7 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
   DevView* hv = static_cast<DevView*>(c->h_views);
   for (int i = 0; i < n; i++) {
     DevView& v = hv[i];
-    std::memset(&v, 0, sizeof(v));
-    v.W = cams[i].width;
-    v.H = cams[i].height;
-    v.fx = cams[i].fx;
-    v.fy = cams[i].fy;
-    v.cx = cams[i].cx;
-    v.cy = cams[i].cy;
-    std::memcpy(v.w2c_R, cams[i].w2c_R, sizeof(v.w2c_R));
-    std::memcpy(v.w2c_t, cams[i].w2c_t, sizeof(v.w2c_t));
-    std::memcpy(v.c2w_R, cams[i].c2w_R, sizeof(v.c2w_R));
-    std::memcpy(v.c2w_t, cams[i].c2w_t, sizeof(v.c2w_t));
-    v.tiles_x = (v.W + tw - 1) / tw;
-    v.tiles_y = (v.H + th - 1) / th;
-    v.tile_base = c->h_tile_base[i];
+    fill_view_camera(v, cams[i], tw, th, c->h_tile_base[i]);
     if (mode == 0) {
       void* p[kPlanes];
       plane_ptrs(outs[i], gbufs ? &gbufs[i] : nullptr, p);
@@ -778,7 +784,12 @@ ugu_hip_ctx* ugu_hip_create(int device) {
     return nullptr;
   }
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    c->cu_count = prop.multiProcessorCount;
+    c->arch = prop.gcnArchName;
+    for (char& ch : c->arch)
+      if (ch == ' ' || ch == '\n' || ch == '\t') ch = '_';
+  }
   if (hipHostMalloc(reinterpret_cast<void**>(&c->h_status), 64, hipHostMallocDefault) != hipSuccess) {
     (void)hipStreamDestroy(c->own);
     delete c;
@@ -1254,6 +1265,69 @@ int ugu_hip_autotune(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOpti
   c->tiling = best;
   drop_graphs(c);
   if (chosen) *chosen = best;
+  return UGU_HIP_OK;
+}
+
+int ugu_hip_autotune_cached(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions* o,
+                            int repeats, const char* cache_path, UguTiling* chosen,
+                            int* from_cache) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  if (from_cache) *from_cache = 0;
+  if (n < 1 || !cams || !o) return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT, "no views to tune on");
+  // key: device, mesh size, options that change the work, the views' sizes
+  uint64_t sizes = 1469598103934665603ull;  // FNV-1a over (W, H) of every view
+  int64_t px = 0;
+  for (int i = 0; i < n; i++) {
+    for (int32_t v : {cams[i].width, cams[i].height}) {
+      sizes = (sizes ^ uint64_t(uint32_t(v))) * 1099511628211ull;
+    }
+    px += int64_t(cams[i].width) * cams[i].height;
+  }
+  char key[256];
+  std::snprintf(key, sizeof(key), "%s|cu%d|F%d|n%d|px%lld|s%016llx|o%d%d%d%d",
+                c->arch.empty() ? "unknown" : c->arch.c_str(), c->cu_count, c->F, n,
+                static_cast<long long>(px), static_cast<unsigned long long>(sizes),
+                o->diffuse_color, o->interp, o->shading_normal, o->diffuse_shading);
+  if (cache_path && *cache_path) {
+    // the last line with this key wins
+    FILE* f = std::fopen(cache_path, "r");
+    if (f) {
+      char line[512];
+      UguTiling found{};
+      bool hit = false;
+      while (std::fgets(line, sizeof(line), f)) {
+        char k[256];
+        UguTiling t{};
+        if (std::sscanf(line, "%255s %d %d %d %d", k, &t.tile_w, &t.tile_h, &t.persistent,
+                        &t.bin_capacity) == 5 &&
+            std::strcmp(k, key) == 0 && valid_tile(t.tile_w, t.tile_h) && t.bin_capacity > 0 &&
+            (t.persistent == 0 || t.persistent == 1)) {
+          found = t;
+          hit = true;
+        }
+      }
+      std::fclose(f);
+      if (hit) {
+        c->tiling = found;
+        drop_graphs(c);
+        if (chosen) *chosen = found;
+        if (from_cache) *from_cache = 1;
+        return UGU_HIP_OK;
+      }
+    }
+  }
+  UguTiling t{};
+  int err = ugu_hip_autotune(c, cams, n, o, repeats, &t);
+  if (err) return err;
+  if (chosen) *chosen = t;
+  if (cache_path && *cache_path) {
+    // one short line per append: concurrent writers do not interleave inside it
+    FILE* f = std::fopen(cache_path, "a");
+    if (f) {
+      std::fprintf(f, "%s %d %d %d %d\n", key, t.tile_w, t.tile_h, t.persistent, t.bin_capacity);
+      std::fclose(f);
+    }
+  }
   return UGU_HIP_OK;
 }
 

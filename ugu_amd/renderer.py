@@ -249,14 +249,26 @@ class Rasterizer:
         self._check(self._lib.ugu_hip_get_timings(self._ctx, ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in _abi.UguTimings._fields_}
 
-    def autotune(self, cameras, repeats=3):
+    def autotune(self, cameras, repeats=3, cache_path=None):
+        """Tile shape and bin capacity for these views (ugu_hip_autotune); with
+        cache_path, a choice kept in that file for the same device, mesh size,
+        options and view sizes is applied without tuning
+        (ugu_hip_autotune_cached). Returns the tiling and whether it came from
+        the file."""
         cams = (_abi.UguCamera * len(cameras))(*[c.to_abi() for c in cameras])
         opt = options_to_abi(self.option_)
         t = _abi.UguTiling()
-        self._check(self._lib.ugu_hip_autotune(self._ctx, cams, len(cameras), ctypes.byref(opt),
-                                               int(repeats), ctypes.byref(t)))
+        hit = ctypes.c_int(0)
+        if cache_path is None:
+            self._check(self._lib.ugu_hip_autotune(self._ctx, cams, len(cameras),
+                                                   ctypes.byref(opt), int(repeats),
+                                                   ctypes.byref(t)))
+        else:
+            self._check(self._lib.ugu_hip_autotune_cached(
+                self._ctx, cams, len(cameras), ctypes.byref(opt), int(repeats),
+                str(cache_path).encode(), ctypes.byref(t), ctypes.byref(hit)))
         return dict(tile_w=t.tile_w, tile_h=t.tile_h, bin_capacity=t.bin_capacity,
-                    persistent=bool(t.persistent))
+                    persistent=bool(t.persistent), from_cache=bool(hit.value))
 
     def render_batch(self, cameras, outputs=("color", "depth", "normal", "mask", "face_id"),
                      graph=True):
