@@ -179,11 +179,7 @@ def diff_benchmarks(torch, np, scenes, mesh, cams, args):
     # -- inverse texturing: 32 views (16 at 512x512, 16 at 1024x768), GT
     #    2048x2048 float texture, learned texture from 0.5 grey, Adam lr 1e-2
     log("diff: inverse texturing, 32 views, 200 Adam iterations")
-    vc = scenes.ring_cameras(32, 3.0, 41, 42, res_choices=[(512, 512)])
-    for k in range(16, 32):
-        vc[k].set_size(1024, 768)
-        vc[k].principal_point[:] = (511.5, 383.5)
-        vc[k].set_fov_y(45.0)
+    vc = scenes.inverse_texturing_cameras()
     a5 = [c.to_abi() for c in vc]
     gt_tex = torch.tensor(scenes.checker_noise_texture(2048, 8, dtype=np.float32), device=dev)
     with torch.no_grad():

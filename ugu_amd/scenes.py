@@ -242,3 +242,15 @@ def multiview_cameras(n=64):
     """n cameras on a ring of radius 3, elevations U[-30,30] deg (seed 1),
     resolutions from {320x240, 640x480, 1280x720, 1920x1080} (seed 2), fov_y 45."""
     return ring_cameras(n, 3.0, 1, 2)
+
+
+def inverse_texturing_cameras():
+    """The 32 views of the inverse-texturing workload: a ring of radius 3
+    (elevations seed 41), views 0-15 at 512x512 and 16-31 at 1024x768, fov_y
+    45 deg, principal points at the image centres."""
+    cams = ring_cameras(32, 3.0, 41, 42, res_choices=[(512, 512)])
+    for k in range(16, 32):
+        cams[k].set_size(1024, 768)
+        cams[k].principal_point[:] = (511.5, 383.5)
+        cams[k].set_fov_y(45.0)
+    return cams
