@@ -83,7 +83,8 @@ def cpp_run(tmp_path_factory):
     os.makedirs(out)
     print(f"[cpp-texturing] driver on {obj}", flush=True)
     res = subprocess.run([DRIVER, obj, os.path.join(d, "cams.f64"), str(NCAM), out],
-                         capture_output=True, text=True, timeout=120)
+                         capture_output=True, text=True, timeout=120,
+                         env=dict(os.environ, UGU_HIP_TUNE_CACHE=os.path.join(d, "tiling.txt")))
     print(res.stdout[-2000:], res.stderr[-2000:], flush=True)
     mesh, ok = uio.load_obj(obj)
     assert ok
@@ -129,7 +130,16 @@ def test_cpp_matches_python_pipeline(gpu_available, cpp_run, facewise):
     for c, crit in enumerate([T.ViewSelectionCriteria.kMaxArea,
                               T.ViewSelectionCriteria.kMinViewingAngle]):
         mm = copy.copy(mesh)  # TextureMapping replaces the materials
-        opt = T.TextureMappingOption(criteria=crit, tex_w=512, tex_h=512)
+        opt = T.TextureMappingOption(criteria=crit, tex_w=512,
+                                     uv_type=T.TexturingOutputUvType.kUseOriginalMeshUv, tex_h=512)
         assert T.TextureMapping(kfs, info, mm, opt)
         tex = np.fromfile(p + f"tex{c}.u8", np.uint8).reshape(512, 512, 3)
         assert np.array_equal(tex, mm.materials[0]), crit.name
+    # default option (kGenerateSimpleTile): the C++ tiled texture and uvs
+    # equal the Python host's
+    mm = copy.copy(mesh)
+    assert T.TextureMapping(kfs, info, mm, T.TextureMappingOption())
+    tex = np.fromfile(p + "tile_tex.u8", np.uint8).reshape(mm.materials[0].shape)
+    assert np.array_equal(tex, mm.materials[0])
+    uv = np.fromfile(p + "tile_uv.f32", np.float32).reshape(-1, 2)
+    assert np.array_equal(uv, np.asarray(mm.uv, np.float32).reshape(-1, 2))

@@ -84,7 +84,8 @@ def test_spot_pipeline_matches_oracle(gpu_available, spot, criteria):
     assert vt.PrepareData()
     info = V.VisibilityInfo(mesh)
     assert vt.Test(info, facewise=True, keyframes=kfs)
-    opt = T.TextureMappingOption(criteria=criteria, tex_w=512, tex_h=512)
+    opt = T.TextureMappingOption(criteria=criteria, tex_w=512,
+                                 uv_type=T.TexturingOutputUvType.kUseOriginalMeshUv, tex_h=512)
     m_gpu = copy.copy(mesh)  # TextureMapping replaces the materials
     assert T.TextureMapping(kfs, info, m_gpu, opt)
     tex = m_gpu.materials[0]

@@ -7,7 +7,8 @@ Python entry points:
                       set_camera/Render...), plus batched and device-memory renders
   Raytracer           mirror of ugu::Raytracer (same interface, BVH ray casting)
   Mesh, PinholeCamera, RendererCpuOption, Image*   reference types
-  diff.render_diff    differentiable rendering (torch.autograd.Function)
+  diff                differentiable rendering (DiffRenderer, RasterizeFunction:
+                      a torch.autograd.Function)
   scenes              seeded benchmark scenes
 """
 from .types import (ColorInterpolation, DiffuseColor, DiffuseShading, Image, Image1b,

@@ -8,6 +8,7 @@
 // usage: test_texturing_api <obj_path> <cams.f64> <ncam> <out_dir>
 // cams.f64: ncam x (W H fx fy cx cy c2w[16 row-major])
 
+#include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <memory>
@@ -139,9 +140,14 @@ int main(int argc, char** argv) {
     nomask->mask = ugu::Image1b();
     vt.set_keyframe(nomask);
     check(!vt.Test(&info), "test_use_mask_without_mask_fails");
-    ugu::TextureMappingOption topt;  // default uv type generates an atlas
-    ugu::Mesh m2 = *mesh;
-    check(!ugu::TextureMapping(keyframes, info, &m2, topt), "atlas_uv_type_not_provided");
+    // the chart-atlas uv generators are not provided (LOGE, false)
+    for (ugu::TexturingOutputUvType t :
+         {ugu::kGenerateSimpleTriangles, ugu::kGenerateSimpleCharts}) {
+      ugu::TextureMappingOption topt;
+      topt.uv_type = t;
+      ugu::Mesh m2 = *mesh;
+      check(!ugu::TextureMapping(keyframes, info, &m2, topt), "atlas_uv_type_not_provided");
+    }
   }
   ugu::set_log_level(ugu::LogLevel::kError);
 
@@ -168,6 +174,32 @@ int main(int argc, char** argv) {
                 m2.material_ids().size() == m2.vertex_indices().size(),
             "texture_material_set");
       write_img(p + "tex" + std::to_string(c) + ".u8", m2.materials()[0].diffuse_tex);
+    }
+    {
+      // default option: kGenerateSimpleTile (texture_mapper.cc:62-136): the
+      // keyframes tiled floor(sqrt(n)) across and ceil(n / that) down, three
+      // uvs per face
+      ugu::TextureMappingOption topt;
+      ugu::Mesh m2 = *mesh;
+      check(ugu::TextureMapping(keyframes, info, &m2, topt), "simple_tile_mapping");
+      const int n = int(keyframes.size());
+      const int xt = std::max(1, int(std::sqrt(double(n))));
+      const int yt = (n + xt - 1) / xt;
+      const auto& tex = m2.materials()[0].diffuse_tex;
+      const size_t F = m2.vertex_indices().size();
+      bool uv_ok = m2.uv().size() == 3 * F && m2.uv_indices().size() == F;
+      for (size_t i = 0; uv_ok && i < F; i++)
+        for (int k = 0; k < 3; k++) uv_ok = uv_ok && m2.uv_indices()[i][k] == int(3 * i) + k;
+      check(m2.materials().size() == 1 && tex.cols == xt * keyframes[0]->color.cols &&
+                tex.rows == yt * keyframes[0]->color.rows && uv_ok,
+            "simple_tile_layout");
+      write_img(p + "tile_tex.u8", tex);
+      std::vector<float> uv;
+      for (const auto& t : m2.uv()) {
+        uv.push_back(t[0]);
+        uv.push_back(t[1]);
+      }
+      write_vec(p + "tile_uv.f32", uv);
     }
   }
   std::printf("FAILURES %d\n", g_fail);
