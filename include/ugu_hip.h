@@ -218,6 +218,26 @@ int ugu_hip_render_batch_eager(ugu_hip_ctx* ctx, const UguCamera* cameras,
                                int n, const UguOptions* options,
                                const UguOutputs* outputs, int memory);
 
+/* Allocation-free ugu_hip_render_batch. No reference counterpart: for
+ * callers that cannot allocate device memory during the call (code between
+ * graph-capture regions, tight serving loops, harnesses that forbid
+ * hipMalloc). Outputs are device pointers (UGU_HIP_MEMORY_DEVICE); results
+ * are bitwise those of ugu_hip_render_batch. The call allocates nothing:
+ * the work memory (bin counters, bins, view descriptors) is one 256 MiB
+ * device array per GPU, reserved when the library's code is loaded on that
+ * GPU and held until the process ends; the mesh stays the context's prepared
+ * copy. Views are rendered in groups whose bins fit that array; a call is
+ * rejected (UGU_HIP_ERR_INVALID_ARGUMENT) above 1024 views, and
+ * (UGU_HIP_ERR_OUT_OF_MEMORY, nothing launched) when one view's screen tiles
+ * need more than the array holds, or when one screen tile of a view holds
+ * more faces than the array can give each tile of that view. Calls on one
+ * GPU share the array: the library serialises them within a process; any
+ * other overlapping use on the GPU would race. The context's tiling is used
+ * as by ugu_hip_render_batch; kernels are launched directly (no graph). */
+int ugu_hip_render_batch_noalloc(ugu_hip_ctx* ctx, const UguCamera* cameras,
+                                 int n, const UguOptions* options,
+                                 const UguOutputs* outputs);
+
 /* G-buffer planes of a view (any may be null; memory kind as the outputs).
  * A pixel is covered where the mask output is 255 (face id > 0 and not
  * culled); with u = w1, v = w2 the perspective-correct weights of the face's

@@ -149,6 +149,9 @@ SIGNATURES = {
     "ugu_hip_render_batch_eager": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,
                                                   ctypes.POINTER(UguOptions),
                                                   ctypes.POINTER(UguOutputs), ctypes.c_int]),
+    "ugu_hip_render_batch_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,
+                                                    ctypes.POINTER(UguOptions),
+                                                    ctypes.POINTER(UguOutputs)]),
     "ugu_hip_render_gbuffer": (ctypes.c_int, [_P, ctypes.POINTER(UguCamera), ctypes.c_int,
                                               ctypes.POINTER(UguOptions),
                                               ctypes.POINTER(UguOutputs),

@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -48,7 +49,55 @@ int launch_visibility(hipStream_t st, const DevView& v, const uint8_t* color, co
 
 using namespace ugu_hip;
 
+// Work memory of ugu_hip_render_batch_noalloc: one copy per GPU, allocated
+// with the code object and held until the process ends. Host code never
+// passes this name to a kernel or a copy: its host-side value is not a device
+// address. It uses the address hipGetSymbolAddress returns for the current
+// GPU (looked up once per GPU in ugu_hip_create, kept in g_scratch).
+constexpr size_t kScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
 namespace {
+
+constexpr int kNoAllocMaxViews = 1024;
+constexpr int kMaxDevices = 64;
+// Layout of the scratch array (offsets 256-B aligned): status words (max bin
+// count, work-queue head), the view descriptors, the views' first global
+// tiles, then per render group the tiles' counters and the bins.
+constexpr size_t kNaViewsOff = 256;
+constexpr size_t kNaTileBaseOff =
+    (kNaViewsOff + size_t(kNoAllocMaxViews) * sizeof(DevView) + 255) / 256 * 256;
+constexpr size_t kNaCountsOff =
+    (kNaTileBaseOff + size_t(kNoAllocMaxViews + 1) * 4 + 255) / 256 * 256;
+static_assert(kNaCountsOff < kScratchBytes / 2, "descriptor area fits the scratch");
+
+std::mutex g_scratch_mu;                          // guards the two tables below
+uint8_t* g_scratch[kMaxDevices] = {};             // device address of `scratch` per GPU
+hipError_t g_scratch_err[kMaxDevices] = {};       // lookup result per GPU
+std::mutex g_scratch_use[kMaxDevices];            // one allocation-free render per GPU at a time
+
+// With `device` current: looks up this GPU's copy of `scratch` (each GPU
+// holds its own copy at its own address). Repeated calls keep the first
+// successful result.
+void init_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  if (e == hipSuccess && p != nullptr) g_scratch[device] = static_cast<uint8_t*>(p);
+}
+
+uint8_t* scratch_of(int device, hipError_t* err) {
+  if (device < 0 || device >= kMaxDevices) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  *err = g_scratch_err[device];
+  return g_scratch[device];
+}
 
 struct DevBuf {
   void* p = nullptr;
@@ -141,6 +190,13 @@ struct ugu_hip_ctx {
   bool profile = false;
   UguTimings tm{};
   hipEvent_t ev[6] = {};
+
+  // allocation-free render: host copies of one group's view descriptors and
+  // first tiles (sized at create), and the per-tile bin capacity the last
+  // calls needed (groups are formed so every tile gets at least this much)
+  std::vector<DevView> na_views;
+  std::vector<int32_t> na_tile_base;
+  int64_t na_cap_hint = 0;
 };
 
 namespace {
@@ -644,6 +700,110 @@ int render_common(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions
                       gbufs);
 }
 
+// The pipeline of run_pipeline (clear, setup+bin, raster; eager launches, no
+// graph) over the scratch array at `base` instead of the context's buffers.
+// Views are rendered in consecutive groups whose tiles all get at least
+// `want` bin entries; a group in which a tile overflows is rendered again as
+// smaller groups with the measured occupancy as the new `want` (the tiles of
+// one view do not depend on the other views of a batch, so results equal a
+// single batched render). A single view whose largest tile does not fit is
+// an error. Nothing is allocated: the descriptors are staged in the
+// context's host arrays (sized at create) and the status word is read back
+// into its pinned word.
+int run_noalloc(ugu_hip_ctx* c, uint8_t* base, const UguCamera* cams, int n, const DevMesh& dm,
+                const DevOptions& dopt, const UguOutputs* outs) {
+  const int tw = c->tiling.tile_w, th = c->tiling.tile_h;
+  int32_t* status = reinterpret_cast<int32_t*>(base);
+  DevView* dviews = reinterpret_cast<DevView*>(base + kNaViewsOff);
+  int32_t* dtile_base = reinterpret_cast<int32_t*>(base + kNaTileBaseOff);
+  const size_t avail = kScratchBytes - kNaCountsOff;
+  auto tiles_of = [&](const UguCamera& cam) {
+    return int64_t((cam.width + tw - 1) / tw) * int64_t((cam.height + th - 1) / th);
+  };
+  auto counter_bytes = [](int64_t tiles) { return (size_t(tiles) * 16 + 255) / 256 * 256; };
+  int persistent_blocks_per_cu = 0;
+  if (c->tiling.persistent) persistent_blocks_per_cu = std::max(1, raster_blocks_per_cu(tw, th, 0));
+
+  int64_t want = std::max<int64_t>({c->na_cap_hint, int64_t(c->tiling.bin_capacity), 64});
+  int s = 0;
+  while (s < n) {
+    // the longest run of views from s whose tiles all get `want` entries
+    int e = s;
+    int64_t tiles = 0;
+    while (e < n) {
+      const int64_t t = tiles + tiles_of(cams[e]);
+      if (e > s && counter_bytes(t) + size_t(t) * 8 * size_t(want) > avail) break;
+      tiles = t;
+      e++;
+    }
+    const size_t cbytes = counter_bytes(tiles);
+    const int64_t cap_fit = cbytes < avail ? int64_t((avail - cbytes) / (size_t(tiles) * 8)) : 0;
+    const int64_t cap = std::min<int64_t>(cap_fit, INT32_MAX / 2) / 8 * 8;
+    if (cap < 8) {
+      char msg[200];
+      std::snprintf(msg, sizeof(msg),
+                    "view %d (%dx%d, %lld tiles of %dx%d) is too large for the %zu MiB "
+                    "allocation-free scratch",
+                    s, cams[s].width, cams[s].height, static_cast<long long>(tiles), tw, th,
+                    kScratchBytes >> 20);
+      return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, msg);
+    }
+    // descriptors of the group, tiles numbered from 0
+    const int g = e - s;
+    int32_t tb = 0;
+    for (int i = 0; i < g; i++) {
+      DevView& v = c->na_views[size_t(i)];
+      fill_view_camera(v, cams[s + i], tw, th, tb);
+      void* p[kPlanes];
+      plane_ptrs(outs[s + i], nullptr, p);
+      set_view_planes(v, p);
+      c->na_tile_base[size_t(i)] = tb;
+      tb += static_cast<int32_t>(tiles_of(cams[s + i]));
+    }
+    c->na_tile_base[size_t(g)] = tb;
+    const int total_tiles = static_cast<int>(tiles);
+    DevBins b;
+    b.counts = reinterpret_cast<int32_t*>(base + kNaCountsOff);
+    b.bins = reinterpret_cast<int2*>(base + kNaCountsOff + cbytes);
+    b.cap = static_cast<int32_t>(cap);
+    b.total_tiles = total_tiles;
+    b.status = status;
+    b.tile_base = dtile_base;
+    b.n_views = g;
+    int persistent_blocks = persistent_blocks_per_cu * std::max(1, c->cu_count);
+    if (!c->tiling.persistent || persistent_blocks > total_tiles) persistent_blocks = 0;
+    UGU_HIP_CHECK(c, hipMemcpyAsync(dviews, c->na_views.data(), size_t(g) * sizeof(DevView),
+                                    hipMemcpyHostToDevice, c->stream));
+    UGU_HIP_CHECK(c, hipMemcpyAsync(dtile_base, c->na_tile_base.data(), size_t(g + 1) * 4,
+                                    hipMemcpyHostToDevice, c->stream));
+    if (launch_clear(c->stream, b.counts, total_tiles, b.status, b.tile_base, g) ||
+        launch_setup_bin(c->stream, tw, th, dviews, g, dm, b) ||
+        launch_raster(c->stream, tw, th, 0, dviews, dm, b, dopt, persistent_blocks))
+      return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
+    UGU_HIP_CHECK(c, hipMemcpyAsync(c->h_status, status, 16, hipMemcpyDeviceToHost, c->stream));
+    UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    const int64_t max_count = c->h_status[0];
+    if (max_count <= cap) {
+      s = e;
+      continue;
+    }
+    if (g == 1) {
+      char msg[200];
+      std::snprintf(msg, sizeof(msg),
+                    "view %d: a screen tile holds %lld faces; the %zu MiB allocation-free "
+                    "scratch holds %lld per tile of this view",
+                    s, static_cast<long long>(max_count), kScratchBytes >> 20,
+                    static_cast<long long>(cap));
+      return fail(c, UGU_HIP_ERR_OUT_OF_MEMORY, msg);
+    }
+    // `want` grows past the measured occupancy, so the next groups are
+    // smaller and this run of views fits
+    want = std::max(want, max_count + max_count / 8 + 64);
+  }
+  c->na_cap_hint = want;
+  return UGU_HIP_OK;
+}
+
 // Raytracer::Render for n views (outputs[i] belongs to cams[i]).
 int raytrace_common(ugu_hip_ctx* c, const UguCamera* cams, int n, const UguOptions* o,
                     const UguOutputs* outs, int memory) {
@@ -796,6 +956,9 @@ ugu_hip_ctx* ugu_hip_create(int device) {
     return nullptr;
   }
   for (auto& e : c->ev) (void)hipEventCreate(&e);
+  init_scratch(device);
+  c->na_views.resize(kNoAllocMaxViews);
+  c->na_tile_base.resize(kNoAllocMaxViews + 1);
   return c;
 }
 
@@ -988,6 +1151,35 @@ int ugu_hip_render_gbuffer(ugu_hip_ctx* c, const UguCamera* cams, int n, const U
 int ugu_hip_render_batch_eager(ugu_hip_ctx* c, const UguCamera* cams, int n,
                                const UguOptions* o, const UguOutputs* outs, int memory) {
   return render_common(c, cams, n, o, outs, memory, false);
+}
+
+int ugu_hip_render_batch_noalloc(ugu_hip_ctx* c, const UguCamera* cams, int n,
+                                 const UguOptions* o, const UguOutputs* outs) {
+  if (!c) return UGU_HIP_ERR_INVALID_ARGUMENT;
+  if (n < 1 || cams == nullptr || outs == nullptr)
+    return fail(c, n < 1 ? UGU_HIP_ERR_INVALID_ARGUMENT : UGU_HIP_ERR_NO_CAMERA,
+                cams == nullptr ? "camera has not been set\n" : "invalid view count or outputs");
+  if (n > kNoAllocMaxViews)
+    return fail(c, UGU_HIP_ERR_INVALID_ARGUMENT,
+                "the allocation-free render takes at most 1024 views per call");
+  for (int i = 0; i < n; i++) {
+    void* p[kPlanes];
+    plane_ptrs(outs[i], nullptr, p);
+    bool any = false;
+    for (int k = 0; k < kPlanes; k++) any = any || p[k] != nullptr;
+    int err = validate(c, &cams[i], o, any);
+    if (err) return err;
+    if ((err = check_camera(c, cams[i]))) return err;
+  }
+  hipError_t se = hipSuccess;
+  uint8_t* base = scratch_of(c->device, &se);
+  if (base == nullptr)
+    return hip_fail(c, se == hipSuccess ? hipErrorInvalidSymbol : se,
+                    "scratch of the allocation-free render unavailable on this device");
+  UGU_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int oe = order_after_user(c)) return oe;
+  std::lock_guard<std::mutex> lk(g_scratch_use[c->device]);
+  return run_noalloc(c, base, cams, n, dev_mesh(c), dev_options(*o), outs);
 }
 
 int ugu_hip_prepare_raytracer(ugu_hip_ctx* c) {
