@@ -112,6 +112,37 @@ def cpu_baseline(mesh, cams, opt, budget_s):
     return done, t_total, O.num_threads()
 
 
+def backward_roofline(dr, step, mesh, F, tex_texels, colors, positions):
+    """k_bwd_items of one profiled fit step: algorithmic bytes = 28 B per item
+    (the 16-B (face, view, pixel) item and the pixel's 12-B dL/dimage) + the
+    inputs once (positions 12 V, faces 12 F; colours 12 V; uv 8 U, uv indices
+    12 F and the float texture 12 T) + the gradients once (12 V per vertex
+    quantity, 12 T); updates = per-item gradient contributions before the
+    wavefront and LDS pre-reduction (9 colour, 9 position, 12 texel values)."""
+    dr.enable_profiling(True)
+    step()
+    tm = dr.timings()
+    dr.enable_profiling(False)
+    n, ms = tm["bwd_items"], tm["bwd_items_ms"]
+    V, U = len(mesh.vertices), len(mesh.uv)
+    b = 28 * n + 12 * V + 12 * F
+    per_item = 0
+    if colors:
+        b += 24 * V
+        per_item += 9
+    if positions:
+        b += 12 * V
+        per_item += 9
+    if tex_texels:
+        b += 8 * U + 12 * F + 24 * tex_texels
+        per_item += 12
+    gbs = b / (ms * 1e-3) / 1e9
+    return {"kernel": "k_bwd_items", "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": b, "items": n,
+            "kernel_ms": ms, "sort_ms": tm["bwd_sort_ms"],
+            "updates_per_s": n * per_item / (ms * 1e-3)}
+
+
 def diff_benchmarks(torch, np, scenes, mesh, cams, args):
     """Forward+backward legs of the differentiable renderer (float colours and
     float texture as leaf tensors, L2 image loss)."""
@@ -174,7 +205,9 @@ def diff_benchmarks(torch, np, scenes, mesh, cams, args):
             fit_step(dr, [a], pv, pc, pt, [g])
     tl = time_calls(torch, per_view, 3, 1)
     out["diff_64view"] = {"batched_iters_per_s": 1.0 / tb, "per_view_iters_per_s": 1.0 / tl,
-                          "batched_vs_loop": tl / tb, "batched_ms": tb * 1e3, "loop_ms": tl * 1e3}
+                          "batched_vs_loop": tl / tb, "batched_ms": tb * 1e3, "loop_ms": tl * 1e3,
+                          "roofline": backward_roofline(dr, batched_step, mesh, len(mesh.vertex_indices),
+                                                        tt.shape[0] * tt.shape[1], True, True)}
 
     # -- inverse texturing: 32 views (16 at 512x512, 16 at 1024x768), GT
     #    2048x2048 float texture, learned texture from 0.5 grey, Adam lr 1e-2
@@ -204,10 +237,17 @@ def diff_benchmarks(torch, np, scenes, mesh, cams, args):
     with torch.no_grad():
         m = seen if seen is not None else torch.ones(gt_tex.shape[:2], dtype=torch.bool, device=dev)
         mse = ((learn.clamp(0, 1) - gt_tex) ** 2).mean(-1)[m].mean().item()
+
+    def tex_step():
+        flat, _ = dr.render_flat(a5, vv, None, learn)
+        ((flat - tg5) ** 2).sum().backward()
+    roof5 = backward_roofline(dr, tex_step, mesh, len(mesh.vertex_indices),
+                              gt_tex.shape[0] * gt_tex.shape[1], False, False)
     out["inverse_texturing_32view"] = {"iters_per_s": 1.0 / t5, "ms_per_iter": t5 * 1e3,
                                        "iterations": n_iter,
                                        "psnr_db": 10.0 * math.log10(1.0 / max(mse, 1e-12)),
-                                       "texels_seen": int(m.sum().item())}
+                                       "texels_seen": int(m.sum().item()),
+                                       "roofline": roof5}
     del dr
     return out
 

@@ -61,7 +61,7 @@
 extern "C" {
 #endif
 
-#define UGU_HIP_ABI_VERSION 3
+#define UGU_HIP_ABI_VERSION 4
 
 typedef struct ugu_hip_ctx ugu_hip_ctx;
 
@@ -179,6 +179,12 @@ typedef struct {
   float total_ms;
   int32_t max_bin_count;
   int32_t retries;
+  /* last ugu_hip_diff_backward: the per-face item sort (count, scan,
+   * scatter) and the gradient kernel, and the number of (face, view, pixel)
+   * items it processed */
+  float bwd_sort_ms;
+  float bwd_items_ms;
+  int32_t bwd_items;
 } UguTimings;
 
 /* Context lifetime. device = HIP device ordinal. Returns null on failure. */

@@ -27,7 +27,7 @@ def test_python_binding_covers_header():
 
 
 def test_abi_version():
-    assert _abi.load_library().ugu_hip_abi_version() == 3
+    assert _abi.load_library().ugu_hip_abi_version() == 4
 
 
 def test_create_without_device_returns_null():

@@ -121,6 +121,7 @@ class UguTimings(ctypes.Structure):
     _fields_ = [
         ("clear_ms", _f), ("setup_bin_ms", _f),
         ("raster_ms", _f), ("total_ms", _f), ("max_bin_count", _i), ("retries", _i),
+        ("bwd_sort_ms", _f), ("bwd_items_ms", _f), ("bwd_items", _i),
     ]
 
 

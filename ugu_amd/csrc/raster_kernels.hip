@@ -1344,6 +1344,48 @@ UGU_DEV int tex_slot(int key) {
 // of all of them in its LDS table before the global atomics.
 constexpr int kBwdRounds = 4;
 
+// Geometry of the image model at one pixel of a face, in float64 (the values
+// diff_eval computes): screen positions, inverse camera depths and camera x, y
+// of the corners, the edge functions' quotients q_i = e_i / (A Z_i), 1 / A,
+// 1 / (q0 + q1 + q2) and the weights b1, b2 of the second and third corner.
+struct DiffGeo {
+  double X[3], Y[3], iZ[3], Qx[3], Qy[3], q[3], iA, is, b1, b2;
+};
+
+UGU_DEV void diff_geo(const DevView& v, const float* __restrict__ verts, const int32_t* vid,
+                      int px, int py, DiffGeo& e) {
+  for (int i = 0; i < 3; i++) {
+    const float* P = verts + int64_t(vid[i]) * 3;
+    double Q[3];
+    for (int r = 0; r < 3; r++)
+      Q[r] = double(v.w2c_R[r * 3 + 0]) * P[0] + double(v.w2c_R[r * 3 + 1]) * P[1] +
+             double(v.w2c_R[r * 3 + 2]) * P[2] + double(v.w2c_t[r]);
+    e.Qx[i] = Q[0];
+    e.Qy[i] = Q[1];
+    e.iZ[i] = 1.0 / Q[2];
+    e.X[i] = double(v.fx) * Q[0] * e.iZ[i] + double(v.cx);
+    e.Y[i] = double(v.fy) * Q[1] * e.iZ[i] + double(v.cy);
+  }
+  auto E = [](double ax, double ay, double bx, double by, double cx, double cy) {
+    return (cx - ax) * (by - ay) - (cy - ay) * (bx - ax);
+  };
+  const double x = px, y = py;
+  const double A = E(e.X[0], e.Y[0], e.X[1], e.Y[1], e.X[2], e.Y[2]);
+  e.iA = 1.0 / A;
+  e.q[0] = E(e.X[1], e.Y[1], e.X[2], e.Y[2], x, y) * e.iA * e.iZ[0];
+  e.q[1] = E(e.X[2], e.Y[2], e.X[0], e.Y[0], x, y) * e.iA * e.iZ[1];
+  e.q[2] = E(e.X[0], e.Y[0], e.X[1], e.Y[1], x, y) * e.iA * e.iZ[2];
+  e.is = 1.0 / (e.q[0] + e.q[1] + e.q[2]);
+  e.b1 = e.q[1] * e.is;
+  e.b2 = e.q[2] * e.is;
+}
+
+// One thread per item. The work is ordered so that few float64 values are
+// live at a time (the kernel is bound by the latency of its gathers, so
+// registers buy occupancy): the weights, then the colour and texture terms
+// channel by channel into the colour and texel gradients and dL/du, dL/dv,
+// whose atomics are issued next; the geometry is then evaluated again for the
+// position gradients.
 __global__ __launch_bounds__(256) void k_bwd_items(
     const DevView* __restrict__ views, DevMesh mesh, const int4* __restrict__ items,
     const int32_t* __restrict__ cursor, DevGrads g) {
@@ -1367,43 +1409,141 @@ __global__ __launch_bounds__(256) void k_bwd_items(
   int4 item = make_int4(-1, 0, 0, 0);
   if (valid) item = items[i];
   const int fid = item.x;
-
-  float gcol[9], gpos[9], gtex[12];
-  for (int k = 0; k < 9; k++) { gcol[k] = 0.0f; gpos[k] = 0.0f; }
-  for (int k = 0; k < 12; k++) gtex[k] = 0.0f;
-  int tx0 = 0, ty0 = 0, tx1 = 0, ty1 = 0;
   int vid[3] = {0, 0, 0};
+  double gu = 0.0, gv = 0.0, b1 = 0.0, b2 = 0.0;
+  int px = 0, py = 0;
+  const DevView* vp = views;
 
-  if (valid) {
-    const DevView& v = views[item.y];
-    const int p = item.z;
-    const int px = p % v.W, py = p / v.W;
-    const double G[3] = {v.grad[int64_t(p) * 3 + 0], v.grad[int64_t(p) * 3 + 1],
-                         v.grad[int64_t(p) * 3 + 2]};
-    const int32_t* fi = mesh.faces + int64_t(fid) * 3;
-    vid[0] = fi[0]; vid[1] = fi[1]; vid[2] = fi[2];
-    DiffEval e;
-    diff_eval(v, mesh, fid, px, py, vid, e);
-    tx0 = e.tx0; ty0 = e.ty0; tx1 = e.tx1; ty1 = e.ty1;
-    const double beta[3] = {e.b0, e.b1, e.b2};
-    // dL/dc = G t, dL/dt = G c; u = b1, v = b2 (b0 = 1 - u - v)
-    double gu = 0.0, gv = 0.0, gt[3];
-    for (int k = 0; k < 3; k++) {
-      const double gc = G[k] * e.t[k];
-      gt[k] = G[k] * e.c[k];
-      for (int j = 0; j < 3; j++) gcol[j * 3 + k] = static_cast<float>(gc * beta[j]);
-      gu += gc * e.dcdu[k] + gt[k] * e.dtdu[k];
-      gv += gc * e.dcdv[k] + gt[k] * e.dtdv[k];
-    }
-    if (mesh.ftex) {
-      const double lu = e.lu, lv = e.lv;
+  // ---- colour and texture terms: dL/dC, dL/dT and dL/du, dL/dv
+  // (dL/dc = G t, dL/dt = G c; u = b1, v = b2, b0 = 1 - u - v)
+  {
+    float gcol[9], gtex[12];
+    for (int k = 0; k < 9; k++) gcol[k] = 0.0f;
+    for (int k = 0; k < 12; k++) gtex[k] = 0.0f;
+    int tx0 = 0, ty0 = 0, tx1 = 0, ty1 = 0;
+    if (valid) {
+      vp = views + item.y;
+      const DevView& v = *vp;
+      const int p = item.z;
+      px = p % v.W;
+      py = p / v.W;
+      const int32_t* fi = mesh.faces + int64_t(fid) * 3;
+      vid[0] = fi[0]; vid[1] = fi[1]; vid[2] = fi[2];
+      const float* gp = v.grad + int64_t(p) * 3;
+      const double G[3] = {gp[0], gp[1], gp[2]};
+      {
+        DiffGeo e;
+        diff_geo(v, mesh.verts, vid, px, py, e);
+        b1 = e.b1;
+        b2 = e.b2;
+      }
+      const double b0 = 1.0 - b1 - b2;
+      const float *C0 = nullptr, *C1 = nullptr, *C2 = nullptr;
+      if (mesh.vcolors) {
+        C0 = mesh.vcolors + int64_t(vid[0]) * 3;
+        C1 = mesh.vcolors + int64_t(vid[1]) * 3;
+        C2 = mesh.vcolors + int64_t(vid[2]) * 3;
+      }
+      double lu = 0.0, lv = 0.0, dlu_du = 0.0, dlu_dv = 0.0, dlv_du = 0.0, dlv_dv = 0.0;
+      const float *p00 = nullptr, *p01 = nullptr, *p10 = nullptr, *p11 = nullptr;
+      if (mesh.ftex) {
+        const int32_t* ui = mesh.uvidx + int64_t(fid) * 3;
+        const float* a = mesh.uv + int64_t(ui[0]) * 2;
+        const float* b = mesh.uv + int64_t(ui[1]) * 2;
+        const float* c = mesh.uv + int64_t(ui[2]) * 2;
+        const double iu = b0 * a[0] + b1 * b[0] + b2 * c[0];
+        const double iv = b0 * a[1] + b1 * b[1] + b2 * c[1];
+        const int tw = mesh.ftex_w, th = mesh.ftex_h;
+        const double fxp = iu * (tw - 1), fyp = (1.0 - iv) * (th - 1);
+        const double flx = floor(fxp), fly = floor(fyp);
+        lu = fxp - flx;
+        lv = fyp - fly;
+        tx0 = clamp_texel(flx, tw - 1);
+        ty0 = clamp_texel(fly, th - 1);
+        tx1 = clamp_texel(flx + 1.0, tw - 1);
+        ty1 = clamp_texel(fly + 1.0, th - 1);
+        p00 = mesh.ftex + (int64_t(ty0) * tw + tx0) * 3;
+        p01 = mesh.ftex + (int64_t(ty0) * tw + tx1) * 3;
+        p10 = mesh.ftex + (int64_t(ty1) * tw + tx0) * 3;
+        p11 = mesh.ftex + (int64_t(ty1) * tw + tx1) * 3;
+        const double sx = tw - 1, sy = -(th - 1.0);
+        dlu_du = sx * (double(b[0]) - a[0]);
+        dlu_dv = sx * (double(c[0]) - a[0]);
+        dlv_du = sy * (double(b[1]) - a[1]);
+        dlv_dv = sy * (double(c[1]) - a[1]);
+      }
       const double wq[4] = {(1 - lu) * (1 - lv), lu * (1 - lv), (1 - lu) * lv, lu * lv};
-      for (int j = 0; j < 4; j++)
-        for (int k = 0; k < 3; k++) gtex[j * 3 + k] = static_cast<float>(gt[k] * wq[j]);
+      for (int k = 0; k < 3; k++) {
+        double cc = 1.0, dcdu = 0.0, dcdv = 0.0;
+        if (C0) {
+          cc = b0 * C0[k] + b1 * C1[k] + b2 * C2[k];
+          dcdu = double(C1[k]) - C0[k];
+          dcdv = double(C2[k]) - C0[k];
+        }
+        double t = 1.0, dtdu = 0.0, dtdv = 0.0;
+        if (p00) {
+          t = wq[0] * p00[k] + wq[1] * p01[k] + wq[2] * p10[k] + wq[3] * p11[k];
+          const double da = (1 - lv) * (double(p01[k]) - p00[k]) + lv * (double(p11[k]) - p10[k]);
+          const double db = (1 - lu) * (double(p10[k]) - p00[k]) + lu * (double(p11[k]) - p01[k]);
+          dtdu = da * dlu_du + db * dlv_du;
+          dtdv = da * dlu_dv + db * dlv_dv;
+        }
+        const double gc = G[k] * t, gt = G[k] * cc;
+        gcol[0 * 3 + k] = static_cast<float>(gc * b0);
+        gcol[1 * 3 + k] = static_cast<float>(gc * b1);
+        gcol[2 * 3 + k] = static_cast<float>(gc * b2);
+        for (int q = 0; q < 4; q++) gtex[q * 3 + k] = static_cast<float>(gt * wq[q]);
+        gu += gc * dcdu + gt * dtdu;
+        gv += gc * dcdv + gt * dtdv;
+      }
     }
-    if (g.verts) {
-      // b_i = q_i / s  =>  dL/dq_j = (dL/db_j - (gu b1 + gv b2)) / s
-      const double S = gu * e.b1 + gv * e.b2;
+
+    // wavefront pre-reduction over runs of equal face id, then atomics
+    const int lane = __lane_id();
+    const int prev_fid = __shfl_up(fid, 1);
+    const bool head = valid && (lane == 0 || prev_fid != fid);
+    const uint64_t heads = __ballot(head);
+    const uint64_t vmask = __ballot(valid);
+    const bool reduce = heads != vmask;  // some run longer than one lane
+    const int run = valid ? __popcll(heads & ((2ull << lane) - 1ull)) : -1 - lane;
+    if (g.colors && mesh.vcolors) {
+      for (int k = 0; k < 9; k++) {
+        const float val = reduce ? run_sum(gcol[k], run) : gcol[k];
+        if (head) unsafeAtomicAdd(&g.colors[int64_t(vid[k / 3]) * 3 + (k % 3)], val);
+      }
+    }
+    if (tex && valid) {
+      const int tw = mesh.ftex_w;
+      const int tidx[4] = {ty0 * tw + tx0, ty0 * tw + tx1, ty1 * tw + tx0, ty1 * tw + tx1};
+      for (int q = 0; q < 4; q++) {
+        const int key = tidx[q];
+        int h = tex_slot(key);
+        bool done = false;
+        for (int probe = 0; probe < kTexProbes && !done; probe++) {
+          const int old = atomicCAS(&hkey[h], -1, key);
+          if (old == -1 || old == key) {
+            for (int k = 0; k < 3; k++) atomicAdd(&hval[k][h], gtex[q * 3 + k]);
+            done = true;
+          }
+          h = (h + 1) & (kTexSlots - 1);
+        }
+        if (!done)
+          for (int k = 0; k < 3; k++)
+            unsafeAtomicAdd(&g.texture[int64_t(key) * 3 + k], gtex[q * 3 + k]);
+      }
+    }
+  }
+
+  // ---- positions (the geometry again): b_i = q_i / s
+  if (g.verts) {
+    float gpos[9];
+    for (int k = 0; k < 9; k++) gpos[k] = 0.0f;
+    if (valid) {
+      const DevView& v = *vp;
+      DiffGeo e;
+      diff_geo(v, mesh.verts, vid, px, py, e);
+      // dL/dq_j = (dL/db_j - (gu b1 + gv b2)) / s
+      const double S = gu * b1 + gv * b2;
       const double gq[3] = {-S * e.is, (gu - S) * e.is, (gv - S) * e.is};
       // q_j = e_j / (A Z_j)
       double ge[3], gZ[3], gA = 0.0;
@@ -1444,48 +1584,16 @@ __global__ __launch_bounds__(256) void k_bwd_items(
                                                   double(v.w2c_R[2 * 3 + cdim]) * gQz);
       }
     }
-  }
-
-  // ---- wavefront pre-reduction over runs of equal face id, then atomics
-  const int lane = __lane_id();
-  const int prev_fid = __shfl_up(fid, 1);
-  const bool head = valid && (lane == 0 || prev_fid != fid);
-  const uint64_t heads = __ballot(head);
-  const uint64_t vmask = __ballot(valid);
-  const bool reduce = heads != vmask;  // some run longer than one lane
-  const int run = valid ? __popcll(heads & ((2ull << lane) - 1ull)) : -1 - lane;
-  if (g.colors && mesh.vcolors) {
-    for (int k = 0; k < 9; k++) {
-      const float val = reduce ? run_sum(gcol[k], run) : gcol[k];
-      if (head) unsafeAtomicAdd(&g.colors[int64_t(vid[k / 3]) * 3 + (k % 3)], val);
-    }
-  }
-  if (g.verts) {
+    const int lane = __lane_id();
+    const int prev_fid = __shfl_up(fid, 1);
+    const bool head = valid && (lane == 0 || prev_fid != fid);
+    const uint64_t heads = __ballot(head);
+    const uint64_t vmask = __ballot(valid);
+    const bool reduce = heads != vmask;
+    const int run = valid ? __popcll(heads & ((2ull << lane) - 1ull)) : -1 - lane;
     for (int k = 0; k < 9; k++) {
       const float val = reduce ? run_sum(gpos[k], run) : gpos[k];
       if (head) unsafeAtomicAdd(&g.verts[int64_t(vid[k / 3]) * 3 + (k % 3)], val);
-    }
-  }
-  if (tex && valid) {
-    {
-      const int tw = mesh.ftex_w;
-      const int tidx[4] = {ty0 * tw + tx0, ty0 * tw + tx1, ty1 * tw + tx0, ty1 * tw + tx1};
-      for (int q = 0; q < 4; q++) {
-        const int key = tidx[q];
-        int h = tex_slot(key);
-        bool done = false;
-        for (int probe = 0; probe < kTexProbes && !done; probe++) {
-          const int old = atomicCAS(&hkey[h], -1, key);
-          if (old == -1 || old == key) {
-            for (int k = 0; k < 3; k++) atomicAdd(&hval[k][h], gtex[q * 3 + k]);
-            done = true;
-          }
-          h = (h + 1) & (kTexSlots - 1);
-        }
-        if (!done)
-          for (int k = 0; k < 3; k++)
-            unsafeAtomicAdd(&g.texture[int64_t(key) * 3 + k], gtex[q * 3 + k]);
-      }
     }
   }
   }  // rounds
@@ -1606,7 +1714,7 @@ int64_t backward_scratch_ints(int32_t F) {
 
 int launch_backward(hipStream_t st, const DevView* views, int n_views,
                     int max_pixels, int64_t total_pixels, const DevMesh& mesh,
-                    const DevGrads& g, int4* items, int32_t* scratch) {
+                    const DevGrads& g, int4* items, int32_t* scratch, hipEvent_t before_items) {
   if (max_pixels <= 0 || total_pixels <= 0 || mesh.F <= 0) return 0;
   const int F = mesh.F;
   int32_t* cnt = scratch;
@@ -1620,6 +1728,7 @@ int launch_backward(hipStream_t st, const DevView* views, int n_views,
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, sums, nb);
   hipLaunchKernelGGL(k_scan_add, dim3((F + 255) / 256), dim3(256), 0, st, off, F, sums, nb);
   hipLaunchKernelGGL(k_bwd_scatter, pgrid, dim3(256), 0, st, views, F, off, cnt, items);
+  if (before_items) (void)hipEventRecord(before_items, st);
   // at most one item per pixel; blocks past the item count (off[F]) exit at once
   hipLaunchKernelGGL(k_bwd_items,
                      dim3(static_cast<unsigned>((total_pixels + 256 * kBwdRounds - 1) /

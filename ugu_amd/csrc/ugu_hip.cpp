@@ -35,7 +35,7 @@ int raster_blocks_per_cu(int tw, int th, int mode);
 int64_t backward_scratch_ints(int32_t F);
 int launch_backward(hipStream_t st, const DevView* views, int n_views,
                     int max_pixels, int64_t total_pixels, const DevMesh& mesh,
-                    const DevGrads& g, int4* items, int32_t* scratch);
+                    const DevGrads& g, int4* items, int32_t* scratch, hipEvent_t before_items);
 int launch_depth_to_u16(hipStream_t st, const float* d, int64_t n, uint16_t* out);
 int launch_raytrace(hipStream_t st, const DevView* views, int n_views, int total_tiles,
                     const DevBvh& bvh, const DevMesh& m, const DevOptions& o);
@@ -1644,10 +1644,26 @@ int ugu_hip_diff_backward(ugu_hip_ctx* c, const UguCamera* cams, int n,
   g.verts = grad_vertices;
   g.colors = p->vertex_colors ? grad_vertex_colors : nullptr;
   g.texture = p->texture ? grad_texture : nullptr;
+  if (c->profile) (void)hipEventRecord(c->ev[4], c->stream);
   if (launch_backward(c->stream, dv, n, max_px, total_px, dm, g, c->bwd_items.as<int4>(),
-                      c->bwd_scratch.as<int32_t>()))
+                      c->bwd_scratch.as<int32_t>(), c->profile ? c->ev[5] : nullptr))
     return fail(c, UGU_HIP_ERR_HIP, "kernel launch failed");
+  if (c->profile) {
+    // item total (the scan's last offset) and the two phases' times
+    (void)hipEventRecord(c->ev[3], c->stream);
+    UGU_HIP_CHECK(c, hipMemcpyAsync(c->h_status + 4,
+                                    c->bwd_scratch.as<int32_t>() + (int64_t(dm.F) + 1) + dm.F, 4,
+                                    hipMemcpyDeviceToHost, c->stream));
+  }
   UGU_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  if (c->profile) {
+    float a = 0.0f, b = 0.0f;
+    (void)hipEventElapsedTime(&a, c->ev[4], c->ev[5]);
+    (void)hipEventElapsedTime(&b, c->ev[5], c->ev[3]);
+    c->tm.bwd_sort_ms = a;
+    c->tm.bwd_items_ms = b;
+    c->tm.bwd_items = c->h_status[4];
+  }
   return UGU_HIP_OK;
 }
 

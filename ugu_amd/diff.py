@@ -57,6 +57,17 @@ class DiffRenderer:
         if rc != 0:
             raise _abi.HipError(rc, self._lib.ugu_hip_last_error(self._ctx).decode())
 
+    def enable_profiling(self, on=True):
+        """HIP-event timings of the following calls on the context's stream."""
+        self._check(self._lib.ugu_hip_enable_profiling(self._ctx, 1 if on else 0))
+
+    def timings(self):
+        """Timings of the last forward (rasterizer kernels) and backward (item
+        sort, gradient kernel, item count) while profiling is on."""
+        t = _abi.UguTimings()
+        self._check(self._lib.ugu_hip_get_timings(self._ctx, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in _abi.UguTimings._fields_}
+
     def set_tiling(self, tile_w, tile_h, bin_capacity=1024, persistent=True):
         t = _abi.UguTiling(tile_w, tile_h, bin_capacity, 1 if persistent else 0)
         self._check(self._lib.ugu_hip_set_tiling(self._ctx, ctypes.byref(t)))
