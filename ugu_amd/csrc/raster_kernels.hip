@@ -6,7 +6,8 @@
 //   k_setup_bin   per-face setup of the face loop (rasterizer.cc:162-193):
 //                 z<0 reject, bbox, off-screen cull, pixel bbox, area; then
 //                 appends (face id, skip depth) to the front or back segment
-//                 of every screen tile its pixel bbox touches
+//                 of every screen tile its pixel bbox touches (a face of at
+//                 most 4 bbox pixels only where it may cover a pixel centre)
 //   k_raster      one workgroup per screen tile: stages the tile's faces in
 //                 LDS (skipping faces proven hidden), expands (face, pixel)
 //                 pairs so every lane tests one pair, resolves depth with a
@@ -263,27 +264,27 @@ UGU_DEV AppendTicket append_issue(const DevBins& bins, int seg, uint32_t zmin, b
   tk.ret = 0;
   tk.lr = 0;
   tk.seg = has ? seg : -1;
-  uint64_t todo = __ballot(has);
-  if (!todo) return tk;
-  const int lane = __lane_id();
-  const bool back = has && (seg & 1);
-  int leader = 0, rank = 0, size = 0;
+  const uint64_t hasm = __builtin_amdgcn_ballot_w64(has);
+  if (!hasm) return tk;
+  // Each pass takes the lowest pending lane; the lanes with its segment form
+  // one group, whose lowest lane (that one) leads it. A lane keeps its
+  // group's mask; rank, leader and size follow from it after the loop.
+  uint64_t todo = hasm, grp_mine = 0;
   while (todo) {
     const int l = __ffsll(static_cast<unsigned long long>(todo)) - 1;
     const int ls = __builtin_amdgcn_readlane(seg, l);
-    const bool mine = has && seg == ls;
-    const uint64_t grp = __ballot(mine);
-    if (mine) {
-      leader = l;
-      rank = __popcll(grp & ((1ull << lane) - 1ull));
-    }
-    if (lane == l) size = __popcll(grp);
+    const uint64_t grp = __builtin_amdgcn_ballot_w64(seg == ls) & hasm;
+    grp_mine = (has && seg == ls) ? grp : grp_mine;
     todo &= ~grp;
   }
-  if (has && lane == leader) {
-    tk.ret = atomicAdd(&bins.counts[seg], size);
-    if (back) atomicMin(reinterpret_cast<unsigned int*>(bins.counts) + (seg + 1), zmin);
+  const int rank = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+      static_cast<uint32_t>(grp_mine >> 32),
+      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(grp_mine), 0u)));
+  if (has && rank == 0) {
+    tk.ret = atomicAdd(&bins.counts[seg], __popcll(grp_mine));
+    if (seg & 1) atomicMin(reinterpret_cast<unsigned int*>(bins.counts) + (seg + 1), zmin);
   }
+  const int leader = has ? __ffsll(static_cast<unsigned long long>(grp_mine)) - 1 : 0;
   tk.lr = leader | (rank << 8);
   return tk;
 }
@@ -351,6 +352,44 @@ constexpr int kAppendRounds = 4;
 // them through an LDS pair list; otherwise by binary search over the prefix.
 constexpr int kScatterMaxPairs = 32;
 
+// Faces whose pixel bbox holds at most this many pixel centres are tested at
+// those centres before they are binned.
+constexpr int kCoverProbe = 4;
+
+// Tiles of a small face's bbox (at most kCoverProbe pixel centres, so at most
+// 2 x 2 tiles) holding a centre the face may cover: bit (tx - tx0) + 2 (ty -
+// ty0). The edge functions are the coverage test's (cover_test, same operands
+// and order, so the same bits); a centre where they are neither all >= 0 nor
+// all <= 0 is covered for neither back-face flag, so the rasterizer's pair
+// test would reject the face there whatever the ray. A face with no such
+// centre (most sub-pixel faces) gets no bin entry at all.
+template <int TW, int TH>
+UGU_DEV uint32_t probe_cover(const float4& a, const float4& b, const float4& c, int x0, int x1,
+                             int y0, int y1, int tx0, int ty0) {
+  static_assert(TW >= kCoverProbe && TH >= kCoverProbe, "a probed bbox spans at most 2 x 2 tiles");
+  uint32_t m = 0u;
+  int px = x0, py = y0;
+#pragma unroll
+  for (int k = 0; k < kCoverProbe; k++) {
+    if (py <= y1) {
+      const float x = static_cast<float>(px), y = static_cast<float>(py);
+      const float w0 = edge_fn(b.x, b.y, c.x, c.y, x, y);
+      const float w1 = edge_fn(c.x, c.y, a.x, a.y, x, y);
+      const float w2 = edge_fn(a.x, a.y, b.x, b.y, x, y);
+      // all >= 0 or all <= 0 (a NaN fails both in cover_test; here fminf /
+      // fmaxf drop it, which only keeps more centres)
+      if (fminf(fminf(w0, w1), w2) >= 0.0f || fmaxf(fmaxf(w0, w1), w2) <= 0.0f)
+        m |= 1u << ((static_cast<int>(static_cast<unsigned>(px) / TW) - tx0) +
+                    2 * (static_cast<int>(static_cast<unsigned>(py) / TH) - ty0));
+      if (++px > x1) {
+        px = x0;
+        py++;
+      }
+    }
+  }
+  return m;
+}
+
 template <int TW, int TH>
 __global__ __launch_bounds__(256) void k_setup_bin(
     const DevView* __restrict__ views, int n_views, DevMesh mesh,
@@ -395,6 +434,9 @@ __global__ __launch_bounds__(256) void k_setup_bin(
     bool has = false, back = false;
     float zskip = -1.0f;
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+    // tiles of the bbox that receive the face, bit (tx - tx0) + 2 (ty - ty0)
+    // (all set unless the probe below cleared some)
+    uint32_t tmask = ~0u;
     float4 a, b, c;
     if (shared) {
       for (int j = 0; j < 3; j++) {
@@ -424,9 +466,14 @@ __global__ __launch_bounds__(256) void k_setup_bin(
         tx1 = x1 / TW;
         ty0 = y0 / TH;
         ty1 = y1 / TH;
+        if ((x1 - x0 + 1) * (y1 - y0 + 1) <= kCoverProbe) {
+          tmask = probe_cover<TW, TH>(a, b, c, x0, x1, y0, y1, tx0, ty0);
+          has = tmask != 0u;
+        }
       }
     }
-    // one append per tile of the face's pixel bbox: up to kAppendRounds
+    // one append per tile of the face's pixel bbox (of a probed small face,
+    // per tile where it may cover a centre): up to kAppendRounds
     // rounds are issued before any is finished, further rounds (faces over
     // more tiles) one at a time
     const int2 entry = make_int2(f, __float_as_int(zskip));
@@ -439,8 +486,10 @@ __global__ __launch_bounds__(256) void k_setup_bin(
     AppendTicket tk[kAppendRounds];
 #pragma unroll
     for (int r = 0; r < kAppendRounds; r++) {
-      const int seg = has ? 4 * (v.tile_base + ty * v.tiles_x + tx) + (back ? 1 : 0) : -1;
-      tk[r] = append_issue(bins, seg, zmin, has);
+      const bool put =
+          has && ((tmask >> (min(tx - tx0, 1) + 2 * min(ty - ty0, 1))) & 1u) != 0u;
+      const int seg = put ? 4 * (v.tile_base + ty * v.tiles_x + tx) + (back ? 1 : 0) : -1;
+      tk[r] = append_issue(bins, seg, zmin, put);
       if (has && ++tx > tx1) {
         tx = tx0;
         if (++ty > ty1) has = false;
