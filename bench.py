@@ -330,6 +330,12 @@ def main():
                     help="run only the raytracer legs (prints their JSON)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (counter calibration)")
+    ap.add_argument("--shading", default="texture", choices=("texture", "vertex-normals",
+                                                             "face-normals"),
+                    help="options of the headline leg, for attributing k_raster's reads under "
+                         "a counter profile: texture (the workload), vertex-normals "
+                         "(DiffuseColor kNone: no texels), face-normals (kNone and "
+                         "ShadingNormal kFace: no per-face shading records either)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the views "
@@ -340,7 +346,7 @@ def main():
     import numpy as np
     import torch
     from ugu_amd import (ColorInterpolation, DiffuseColor, Rasterizer, RendererCpuOption,
-                         scenes)
+                         ShadingNormal, scenes)
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.init()
@@ -365,7 +371,13 @@ def main():
         res.update(visibility_benchmarks(torch, np, mesh, cam2, opt))
         print(json.dumps(res))
         return
-    r = Rasterizer(opt)
+    head_opt = opt
+    if args.shading != "texture":
+        head_opt = RendererCpuOption(
+            diffuse_color=DiffuseColor.kNone, interp=ColorInterpolation.kBilinear,
+            shading_normal=(ShadingNormal.kFace if args.shading == "face-normals"
+                            else ShadingNormal.kVertex))
+    r = Rasterizer(head_opt)
     r.set_mesh(mesh)
     assert r.PrepareMesh()
     abi_cams = [c.to_abi() for c in cams]
@@ -406,7 +418,7 @@ def main():
             kt[k] += t[k] / n_prof
     r.enable_profiling(False)
     b_in, b_out = algorithmic_bytes(mesh, cams)
-    traffic = measured_traffic(tiling)
+    traffic = measured_traffic(tiling) if args.shading == "texture" else None
     dom = max(("setup_bin_ms", "raster_ms"), key=lambda k: kt[k])
     raster_gbs = (b_in + b_out) / (kt["raster_ms"] * 1e-3) / 1e9
     pipeline_ms = kt["clear_ms"] + kt["setup_bin_ms"] + kt["raster_ms"]
@@ -483,7 +495,8 @@ def main():
         "config": {"workload": "icosphere-L8 1,310,720 faces, 64 ring views 320x240..1920x1080 "
                                "(56.5 Mpx), texture 2048^2 RGB8 bilinear, vertex normals, "
                                "culling, 5 outputs",
-                   "faces": F, "views": len(cams), "pixels": px_total, "tiling": tiling},
+                   "faces": F, "views": len(cams), "pixels": px_total, "tiling": tiling,
+                   "shading": args.shading},
         "mpix_per_s": mpix,
         "roofline": {"bound": "hbm", "kernel": "k_raster",
                      "achieved": raster_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

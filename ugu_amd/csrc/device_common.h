@@ -105,10 +105,12 @@ UGU_DEV uchar4 tex_at(const uint8_t* base, int w, int h, int row, int col) {
   return reinterpret_cast<const uchar4*>(base)[lin];
 }
 
-// Shading inputs of one face: its corners' vertex normals and uvs and its
-// material. With the mesh's shading records (one 64-byte record per face,
-// built at PrepareMesh) they are one load; otherwise gathered through the
-// index arrays (normal_indices -> normals, uv_indices -> uv, material_ids).
+// Shading inputs of one face: its corners' vertex normals (need_n) and its
+// corners' uvs and material (need_uv); fields not asked for are left unset.
+// With the mesh's shading records (one 64-byte record per face, built at
+// PrepareMesh) they are the record's first 32 bytes (uvs, material) and last
+// 48 bytes (normals); otherwise gathered through the index arrays
+// (normal_indices -> normals, uv_indices -> uv, material_ids).
 struct FaceShade {
   float na[3], nb[3], nc[3];
   float ua[2], ub[2], uc[2];
@@ -118,13 +120,27 @@ struct FaceShade {
 UGU_DEV void face_shade(const DevMesh& m, int fid, bool need_n, bool need_uv, FaceShade& s) {
   if (m.shade) {
     const float4* r = m.shade + int64_t(fid) * 4;
-    const float4 a = r[0], b = r[1], c = r[2], d = r[3];
-    s.ua[0] = a.x; s.ua[1] = a.y; s.ub[0] = a.z; s.ub[1] = a.w;
-    s.uc[0] = b.x; s.uc[1] = b.y;
-    s.na[0] = b.z; s.na[1] = b.w; s.na[2] = c.x;
-    s.nb[0] = c.y; s.nb[1] = c.z; s.nb[2] = c.w;
-    s.nc[0] = d.x; s.nc[1] = d.y; s.nc[2] = d.z;
-    s.mat = __float_as_int(d.w);
+    auto uvs = [&](const float4& a, const float4& b) {
+      s.ua[0] = a.x; s.ua[1] = a.y; s.ub[0] = a.z; s.ub[1] = a.w;
+      s.uc[0] = b.x; s.uc[1] = b.y;
+      s.mat = __float_as_int(b.z);
+    };
+    auto nrms = [&](const float4& b, const float4& c, const float4& d) {
+      s.na[0] = b.w; s.na[1] = c.x; s.na[2] = c.y;
+      s.nb[0] = c.z; s.nb[1] = c.w; s.nb[2] = d.x;
+      s.nc[0] = d.y; s.nc[1] = d.z; s.nc[2] = d.w;
+    };
+    if (need_n && need_uv) {
+      const float4 a = r[0], b = r[1], c = r[2], d = r[3];
+      uvs(a, b);
+      nrms(b, c, d);
+    } else if (need_uv) {
+      const float4 a = r[0], b = r[1];
+      uvs(a, b);
+    } else if (need_n) {
+      const float4 b = r[1], c = r[2], d = r[3];
+      nrms(b, c, d);
+    }
     return;
   }
   if (need_n) {

@@ -150,9 +150,11 @@ UGU_DEV void normalize3(float& x, float& y, float& z) {
 
 // ---------------------------------------------------------------------------
 // k_shade_rec: one 64-byte shading record per face (DevMesh::shade): the
-// corners' uvs (uv[uv_indices]) and vertex normals (normals[normal_indices])
-// and the material id, copied bit for bit (zero where the mesh has none), so
-// the resolve fetches a winner's shading inputs with one load.
+// corners' uvs (uv[uv_indices]), the material id and the corners' vertex
+// normals (normals[normal_indices]), copied bit for bit (zero where the mesh
+// has none), so the resolve fetches a winner's shading inputs with one load.
+// Layout (u0 v0 u1 v1)(u2 v2 mat n0x)(n0y n0z n1x n1y)(n1z n2x n2y n2z): the
+// texture inputs are the first 32 bytes, the normals the last 36.
 __global__ void k_shade_rec(DevMesh m, float4* __restrict__ out) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
@@ -173,9 +175,9 @@ __global__ void k_shade_rec(DevMesh m, float4* __restrict__ out) {
   const int mat = m.matid ? m.matid[f] : 0;
   float4* r = out + int64_t(f) * 4;
   r[0] = make_float4(u[0], u[1], u[2], u[3]);
-  r[1] = make_float4(u[4], u[5], n[0], n[1]);
-  r[2] = make_float4(n[2], n[3], n[4], n[5]);
-  r[3] = make_float4(n[6], n[7], n[8], __int_as_float(mat));
+  r[1] = make_float4(u[4], u[5], __int_as_float(mat), n[0]);
+  r[2] = make_float4(n[1], n[2], n[3], n[4]);
+  r[3] = make_float4(n[5], n[6], n[7], n[8]);
 }
 
 // ---------------------------------------------------------------------------
