@@ -291,17 +291,20 @@ UGU_DEV AppendTicket append_issue(const DevBins& bins, int seg, uint32_t zmin, b
   return tk;
 }
 
-// Finishes R tickets: every group base is broadcast before the first store,
-// so the wavefront waits once for all R atomics (a store issued before a
-// later wait would be waited for as well).
+// Finishes the first `used` (wavefront-uniform) of R tickets: every group
+// base is broadcast before the first store, so the wavefront waits once for
+// all its atomics (a store issued before a later wait would be waited for as
+// well).
 template <int R>
-UGU_DEV void append_finish(const DevBins& bins, const AppendTicket* tk, const int2* entry) {
+UGU_DEV void append_finish(const DevBins& bins, const AppendTicket* tk, const int2* entry,
+                           int used = R) {
   int slot[R];
 #pragma unroll
-  for (int r = 0; r < R; r++) slot[r] = __shfl(tk[r].ret, tk[r].lr & 0xff) + (tk[r].lr >> 8);
+  for (int r = 0; r < R; r++)
+    slot[r] = r < used ? __shfl(tk[r].ret, tk[r].lr & 0xff) + (tk[r].lr >> 8) : 0;
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    if (tk[r].seg >= 0 && slot[r] < bins.cap) {
+    if (r < used && tk[r].seg >= 0 && slot[r] < bins.cap) {
       const int64_t row = int64_t(tk[r].seg >> 2) * bins.cap;
       bins.bins[row + ((tk[r].seg & 1) ? bins.cap - 1 - slot[r] : slot[r])] = *entry;
     }
@@ -436,8 +439,8 @@ __global__ __launch_bounds__(256) void k_setup_bin(
     bool has = false, back = false;
     float zskip = -1.0f;
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
-    // tiles of the bbox that receive the face, bit (tx - tx0) + 2 (ty - ty0)
-    // (all set unless the probe below cleared some)
+    // tiles of a probed face's bbox that receive it, bit (tx - tx0) + 2 (ty -
+    // ty0); all ones for a face that is not probed (every bbox tile)
     uint32_t tmask = ~0u;
     float4 a, b, c;
     if (shared) {
@@ -485,19 +488,32 @@ __global__ __launch_bounds__(256) void k_setup_bin(
     const int nx = tx1 - tx0 + 1;
     const int rest = has ? max(nx * (ty1 - ty0 + 1) - kAppendRounds, 0) : 0;
     int tx = tx0, ty = ty0;
+    // a probed face visits only its marked tiles, one per round
+    const bool probed = tmask != ~0u;
+    uint32_t pm = tmask;
     AppendTicket tk[kAppendRounds];
+    int used = 0;  // rounds issued: until no lane has a tile left
 #pragma unroll
     for (int r = 0; r < kAppendRounds; r++) {
-      const bool put =
-          has && ((tmask >> (min(tx - tx0, 1) + 2 * min(ty - ty0, 1))) & 1u) != 0u;
-      const int seg = put ? 4 * (v.tile_base + ty * v.tiles_x + tx) + (back ? 1 : 0) : -1;
-      tk[r] = append_issue(bins, seg, zmin, put);
-      if (has && ++tx > tx1) {
+      if (!__builtin_amdgcn_ballot_w64(has)) break;
+      used = r + 1;
+      int ax = tx, ay = ty;
+      if (probed) {
+        const int bit = __ffs(pm) - 1;
+        ax = tx0 + (bit & 1);
+        ay = ty0 + (bit >> 1);
+      }
+      const int seg = has ? 4 * (v.tile_base + ay * v.tiles_x + ax) + (back ? 1 : 0) : -1;
+      tk[r] = append_issue(bins, seg, zmin, has);
+      if (probed) {
+        pm &= pm - 1u;
+        has = has && pm != 0u;
+      } else if (has && ++tx > tx1) {
         tx = tx0;
         if (++ty > ty1) has = false;
       }
     }
-    append_finish<kAppendRounds>(bins, tk, &entry);
+    append_finish<kAppendRounds>(bins, tk, &entry, used);
     // Faces over more than kAppendRounds tiles (large triangles): their
     // remaining tiles, all lanes' together, 64 per round -- lane j of a round
     // takes pair base + j of the wavefront's list, finds its face's lane (the
