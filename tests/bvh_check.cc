@@ -43,7 +43,7 @@ float traverse(const ugu_hip::BvhBuild& b, float ox, float oy, float oz, float r
       const F4 alo{nd[0], nd[1], nd[2], 0}, ahi{nd[4], nd[5], nd[6], 0};
       const F4 blo{nd[8], nd[9], nd[10], 0}, bhi{nd[12], nd[13], nd[14], 0};
       const int ra = bits(nd[3]), rb = bits(nd[11]);
-      const float tmax = bt + bt * 9.5367431640625e-07f;
+      const float tmax = ugu_hip::box_tmax(bt);
       const float ta = ugu_hip::box_entry(alo, ahi, ox, oy, oz, ix, iy, iz, tmax);
       const float tb = rb == INT32_MIN ? __builtin_huge_valf()
                                        : ugu_hip::box_entry(blo, bhi, ox, oy, oz, ix, iy, iz, tmax);

@@ -73,6 +73,7 @@ __global__ __launch_bounds__(64 * kRtWaves) void k_raytrace(const DevView* __res
   const float kInf = __builtin_huge_valf();
 
   float bt = kInf, bu = 0.0f, bv = 0.0f;
+  float tmax = kInf;  // box_tmax(bt), updated with bt
   int bf = -1;
   int sp = 0;
   int ref = 0;  // the root is inner node 0
@@ -83,7 +84,6 @@ __global__ __launch_bounds__(64 * kRtWaves) void k_raytrace(const DevView* __res
     if (ref >= 0) {
       const BvhNode nd = nodes[ref];
       const int ra = __float_as_int(nd.a_lo.w), rb = __float_as_int(nd.b_lo.w);
-      const float tmax = bt + bt * 9.5367431640625e-07f;
       // both tests on every lane (no branch between the node's loads), then
       // inactive lanes and the empty child are masked out
       const float ea = box_entry(nd.a_lo, nd.a_hi, ox, oy, oz, ix, iy, iz, tmax);
@@ -116,6 +116,7 @@ __global__ __launch_bounds__(64 * kRtWaves) void k_raytrace(const DevView* __res
             bu = u;
             bv = w;
             bf = f;
+            tmax = box_tmax(bt);
           }
         }
       }
@@ -204,12 +205,12 @@ UGU_DEV float closest_t(const BvhNode* __restrict__ nodes, const int2* __restric
   const float ix = 1.0f / r0, iy = 1.0f / r1, iz = 1.0f / r2;
   const double dox = ox, doy = oy, doz = oz;
   float bt = __builtin_huge_valf();
+  float tmax = bt;  // box_tmax(bt), updated with bt
   int sp = 0, ref = 0;
   for (;;) {
     if (ref >= 0) {
       const BvhNode nd = nodes[ref];
       const int ra = __float_as_int(nd.a_lo.w), rb = __float_as_int(nd.b_lo.w);
-      const float tmax = bt + bt * 9.5367431640625e-07f;
       const float ta = box_entry(nd.a_lo, nd.a_hi, ox, oy, oz, ix, iy, iz, tmax);
       const float tb = rb == kBvhEmpty ? __builtin_huge_valf()
                                        : box_entry(nd.b_lo, nd.b_hi, ox, oy, oz, ix, iy, iz, tmax);
@@ -230,7 +231,10 @@ UGU_DEV float closest_t(const BvhNode* __restrict__ nodes, const int2* __restric
       for (int k = 0; k < lf.y; k++) {
         const float4* tr = tris + int64_t(lf.x + k) * 3;
         float t, u, w;
-        if (mt_hit(dox, doy, doz, r0, r1, r2, tr[0], tr[1], tr[2], t, u, w) && t < bt) bt = t;
+        if (mt_hit(dox, doy, doz, r0, r1, r2, tr[0], tr[1], tr[2], t, u, w) && t < bt) {
+          bt = t;
+          tmax = box_tmax(bt);
+        }
       }
     }
     if (sp == 0) break;

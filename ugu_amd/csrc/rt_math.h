@@ -57,9 +57,26 @@ RT_HD bool mt_hit(double ox, double oy, double oz, double dx, double dy, double 
   return true;
 }
 
+// Relative slack of the box test (2^-20): covers the float rounding of the
+// slab arithmetic (a few ulp).
+constexpr float kBoxSlack = 9.5367431640625e-07f;
+
+// End of the interval a ray still searches once it holds a hit at t: t
+// widened by the slack (t + t 2^-20, one rounding).
+RT_HD float box_tmax(float t) { return t + t * kBoxSlack; }
+
+// Slack applied to a slab interval [tn, tf] (tn >= 0): the box is entered at
+// tn (1 - 2^-20) if that is <= tf (1 + 2^-20), else missed (+inf). Both
+// products are single roundings of tn - tn 2^-20 and tf + tf 2^-20 (the
+// scaled terms are exact); when tf < 0 both forms miss.
+RT_HD float slab_entry(float tn, float tf) {
+  const float tn_lo = tn * (1.0f - kBoxSlack);
+  const float tf_hi = tf * (1.0f + kBoxSlack);
+  return tn_lo <= tf_hi ? tn_lo : __builtin_huge_valf();
+}
+
 // Float slab test of a widened box. Returns the interval entry (>= 0) or
-// +inf when the box is missed or lies beyond tmax. The relative slack of
-// 2^-20 covers the float rounding of the slab arithmetic (a few ulp).
+// +inf when the box is missed or lies beyond tmax.
 template <typename V4>
 RT_HD float box_entry(const V4& lo, const V4& hi, float ox, float oy, float oz, float ix,
                       float iy, float iz, float tmax) {
@@ -71,10 +88,7 @@ RT_HD float box_entry(const V4& lo, const V4& hi, float ox, float oy, float oz, 
   // geometry.
   const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
   const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-  const float k = 9.5367431640625e-07f;  // 2^-20
-  const float tn_lo = tn - tn * k;
-  const float tf_hi = tf + fabsf(tf) * k;
-  return tn_lo <= tf_hi ? tn_lo : __builtin_huge_valf();
+  return slab_entry(tn, tf);
 }
 
 }  // namespace ugu_hip
