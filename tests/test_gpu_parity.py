@@ -249,6 +249,23 @@ def test_subnormal_depths_in_dense_tiles(gpu_available, seed):
             _compare(out, ref, f"dense subnormal seed{seed} cull={cull} tile{tile}")
 
 
+@pytest.mark.parametrize("seed", [5, 6])
+def test_small_faces_at_tile_edges(gpu_available, seed):
+    """Thousands of faces whose bboxes hold at most a few pixel centres, around
+    tile boundaries, edges through pixel centres, both windings: setup tests
+    such faces at their pixel centres and bins them only into the tiles where
+    they may cover one, so every tile shape must still match the oracle."""
+    w, h = 72, 40
+    mesh = TS.mesh_from_pixels(TS.small_faces_at_tile_edges(4000, w, h, seed), seed=seed)
+    cam = TS.exact_camera(w, h)
+    for cull in (False, True):
+        opt = RendererCpuOption(diffuse_color=DiffuseColor.kVertex, backface_culling=cull)
+        ref = _oracle(mesh, cam, opt)
+        for tile in ((8, 8), (16, 16), (32, 8)):
+            out = _render(mesh, cam, opt, tile, cap=64)
+            _compare(out, ref, f"small faces seed{seed} cull={cull} tile{tile}")
+
+
 def test_offscreen_and_empty(gpu_available, uv_scene):
     mesh, cam = uv_scene
     c2 = scenes.front_camera(64, 48, 500.0, 3.0)

@@ -53,6 +53,24 @@ def random_triangles(n, w, h, seed, grid=0.5, zs=(1.0, 2.0, 4.0)):
     return tris
 
 
+def small_faces_at_tile_edges(n, w, h, seed, grid=0.25, zs=(1.0, 2.0, 4.0)):
+    """n faces whose pixel bboxes hold at most a few pixel centres (0.3 to 2.5
+    pixels across), placed around multiples of 8 pixels (tile boundaries of
+    every tile shape) with corners on a `grid`-pixel lattice, so edges pass
+    exactly through pixel centres; either winding; power-of-two depths."""
+    rng = np.random.default_rng(seed)
+    tris = np.empty((n, 3, 3))
+    for i in range(n):
+        cx = 8.0 * rng.integers(0, w // 8 + 1) + rng.uniform(-1.5, 1.5)
+        cy = 8.0 * rng.integers(0, h // 8 + 1) + rng.uniform(-1.5, 1.5)
+        size = rng.uniform(0.3, 2.5)
+        for k in range(3):
+            tris[i, k, 0] = np.round((cx + rng.uniform(-size, size) / 2) / grid) * grid
+            tris[i, k, 1] = np.round((cy + rng.uniform(-size, size) / 2) / grid) * grid
+        tris[i, :, 2] = rng.choice(zs)
+    return tris
+
+
 def add_texture(mesh, size, seed):
     rng = np.random.default_rng(seed)
     tex = (rng.random((size, size, 3)) * 255).astype(np.uint8)
