@@ -64,10 +64,35 @@ UGU_DEV void load_face_rec(const float4* soup, int f, float P[9], float N[3]) {
 
 // Per-face setup of rasterizer.cc:164-193. Returns false when the
 // reference skips the face or its pixel bbox is empty.
+// True when the screen x, y of the three corners are all below 2^30 in
+// magnitude (no NaN or infinity): the sum of their magnitudes propagates NaN
+// and exceeds 2^30 whenever one of them does.
+UGU_DEV bool small_screen(const float4& a, const float4& b, const float4& c) {
+  return ((fabsf(a.x) + fabsf(b.x)) + (fabsf(c.x) + fabsf(a.y))) + (fabsf(b.y) + fabsf(c.y)) <
+         1073741824.0f;
+}
+
 UGU_DEV bool face_setup(const float4& a, const float4& b, const float4& c,
                         int W, int H, int& x0, int& x1, int& y0, int& y1,
                         float& area) {
   if (a.z < 0.0f || b.z < 0.0f || c.z < 0.0f) return false;
+  if (small_screen(a, b, c)) {
+    // no NaN: std::min / std::max are the hardware min / max (their choice
+    // between +0 and -0 aside, which ceil, floor and the comparisons below
+    // do not see), and below 2^30 the x86 conversions are plain truncations
+    const float xmin = fminf(fminf(a.x, b.x), c.x), xmax = fmaxf(fmaxf(a.x, b.x), c.x);
+    const float ymin = fminf(fminf(a.y, b.y), c.y), ymax = fmaxf(fmaxf(a.y, b.y), c.y);
+    if (xmin > static_cast<float>(W - 1) || xmax < 0.0f ||
+        ymin > static_cast<float>(H - 1) || ymax < 0.0f)
+      return false;
+    x0 = max(0, static_cast<int>(ceilf(xmin)));
+    x1 = min(W - 1, static_cast<int>(floorf(xmax)));
+    y0 = max(0, static_cast<int>(ceilf(ymin)));
+    y1 = min(H - 1, static_cast<int>(floorf(ymax)));
+    area = edge_fn(a.x, a.y, b.x, b.y, c.x, c.y);
+    if (fabsf(area) < FLT_MIN) return false;
+    return x0 <= x1 && y0 <= y1;
+  }
   float xmin = min3_std(a.x, b.x, c.x);
   float ymin = min3_std(a.y, b.y, c.y);
   float xmax = max3_std(a.x, b.x, c.x);
@@ -330,13 +355,21 @@ UGU_DEV int2 bin_entry(const int2* bin, int nA, int cap, int i) {
 // z >= min(za, zb, zc) (1 - 2^-10 - 2^-20). NaN or huge values fail every
 // comparison and give -1.
 UGU_DEV float skip_depth(const float4& a, const float4& b, const float4& c, float area) {
-  const float ex = max3_std(a.x, b.x, c.x) - min3_std(a.x, b.x, c.x);
-  const float ey = max3_std(a.y, b.y, c.y) - min3_std(a.y, b.y, c.y);
+  float ex, ey;
+  if (small_screen(a, b, c)) {
+    // no NaN: the hardware min / max (see face_setup)
+    ex = fmaxf(fmaxf(a.x, b.x), c.x) - fminf(fminf(a.x, b.x), c.x);
+    ey = fmaxf(fmaxf(a.y, b.y), c.y) - fminf(fminf(a.y, b.y), c.y);
+  } else {
+    ex = max3_std(a.x, b.x, c.x) - min3_std(a.x, b.x, c.x);
+    ey = max3_std(a.y, b.y, c.y) - min3_std(a.y, b.y, c.y);
+  }
   if (!(ex <= 1048576.0f && ey <= 1048576.0f)) return -1.0f;
   const float L = ex > ey ? ex : ey;
   if (!(L * L * 0.001953125f <= fabsf(area))) return -1.0f;
   if (!(a.z > 0.0f && b.z > 0.0f && c.z > 0.0f)) return -1.0f;
-  return min3_std(a.z, b.z, c.z);
+  // three positive depths: std::min is the hardware min
+  return fminf(fminf(a.z, b.z), c.z);
 }
 
 // True when the tile's current depth bound proves the face hidden.
