@@ -223,6 +223,32 @@ def test_degenerate_vertices(gpu_available):
         np.testing.assert_allclose(out["depth"][ok], ref["depth"][ok], rtol=RTOL)
 
 
+def test_huge_screen_coordinates(gpu_available):
+    """Corners projected far outside the image, around and beyond 2^30 and
+    2^31 pixels (finite): the exact bbox path (std::min, x86 conversions
+    that saturate to INT32_MIN) next to the faces on the hardware min / max
+    path, for every tile shape."""
+    w, h = 40, 24
+    big = [2.0 ** 30 - 64.0, 2.0 ** 30, 2.0 ** 30 + 64.0, 2.0 ** 31, 2.0 ** 31 + 2.0 ** 9, 1e12]
+    tris = list(TS.random_triangles(30, w, h, 11))
+    rng = np.random.default_rng(3)
+    for i, b in enumerate(big):
+        for sx, sy in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+            cx, cy = rng.uniform(2, w - 2), rng.uniform(2, h - 2)
+            z = float(rng.choice([1.0, 2.0]))
+            far = (cx + sx * b, cy + sy * b)
+            side = (cx + sy * 3.0, cy + sx * 3.0)
+            tris.append([[cx, cy, z], [far[0], far[1], z], [side[0], side[1], z]])
+    mesh = TS.mesh_from_pixels(np.array(tris), seed=4)
+    cam = TS.exact_camera(w, h)
+    for cull in (False, True):
+        opt = RendererCpuOption(diffuse_color=DiffuseColor.kVertex, backface_culling=cull)
+        ref = _oracle(mesh, cam, opt)
+        for tile in ((8, 8), (16, 16), (32, 8)):
+            out = _render(mesh, cam, opt, tile, cap=64)
+            _compare(out, ref, f"huge coordinates cull={cull} tile{tile}")
+
+
 @pytest.mark.parametrize("flip,cull", [(False, False), (True, True), (True, False)])
 def test_subnormal_depth_sentinel(gpu_available, flip, cull):
     """Depth 0 (< FLT_MIN) written by a face lets every later face overwrite
