@@ -62,8 +62,6 @@ UGU_DEV void load_face_rec(const float4* soup, int f, float P[9], float N[3]) {
   N[0] = c.y; N[1] = c.z; N[2] = c.w;
 }
 
-// Per-face setup of rasterizer.cc:164-193. Returns false when the
-// reference skips the face or its pixel bbox is empty.
 // True when the screen x, y of the three corners are all below 2^30 in
 // magnitude (no NaN or infinity): the sum of their magnitudes propagates NaN
 // and exceeds 2^30 whenever one of them does.
@@ -72,6 +70,8 @@ UGU_DEV bool small_screen(const float4& a, const float4& b, const float4& c) {
          1073741824.0f;
 }
 
+// Per-face setup of rasterizer.cc:164-193. Returns false when the
+// reference skips the face or its pixel bbox is empty.
 UGU_DEV bool face_setup(const float4& a, const float4& b, const float4& c,
                         int W, int H, int& x0, int& x1, int& y0, int& y1,
                         float& area) {
