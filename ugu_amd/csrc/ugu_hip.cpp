@@ -1475,8 +1475,11 @@ int ugu_hip_autotune_cached(ugu_hip_ctx* c, const UguCamera* cams, int n, const 
     }
     px += int64_t(cams[i].width) * cams[i].height;
   }
+  // kTuneRevision changes whenever the kernels' binning or cost changes, so
+  // choices tuned against older kernels are not reused
+  constexpr int kTuneRevision = 2;
   char key[256];
-  std::snprintf(key, sizeof(key), "%s|cu%d|F%d|n%d|px%lld|s%016llx|o%d%d%d%d",
+  std::snprintf(key, sizeof(key), "r%d|%s|cu%d|F%d|n%d|px%lld|s%016llx|o%d%d%d%d", kTuneRevision,
                 c->arch.empty() ? "unknown" : c->arch.c_str(), c->cu_count, c->F, n,
                 static_cast<long long>(px), static_cast<unsigned long long>(sizes),
                 o->diffuse_color, o->interp, o->shading_normal, o->diffuse_shading);
