@@ -1,9 +1,11 @@
 // ugu/texturing/texture_mapper.h (UGU): TextureMapping with kSimpleProjection
-// for kGenerateSimpleTile (keyframes tiled into one texture, uv from the
-// projected triangles) and kUseOriginalMeshUv (faces rasterized into the
-// mesh's own uv layout on the GPU, ugu_hip_texture_map). The packing atlas
-// generators kGenerateSimpleTriangles / kGenerateSimpleCharts are not
-// provided: TextureMapping reports them as not implemented.
+// for every uv type: kGenerateSimpleTile (keyframes tiled into one texture, uv
+// from the projected triangles), kUseOriginalMeshUv (faces rasterized into the
+// mesh's own uv layout on the GPU, ugu_hip_texture_map),
+// kGenerateSimpleTriangles (each face rasterized into its half of a fixed
+// rectangle grid on the GPU) and kGenerateSimpleCharts (charts of adjacent
+// faces that chose one keyframe, their keyframe patches packed into one
+// texture on the host).
 #pragma once
 
 #include <memory>

@@ -15,6 +15,8 @@ import subprocess
 import numpy as np
 import pytest
 
+import oracle as O
+
 from ugu_amd import PinholeCamera, io as uio, look_at_c2w, texturing as T, visibility as V
 
 pytestmark = pytest.mark.gpu
@@ -143,3 +145,99 @@ def test_cpp_matches_python_pipeline(gpu_available, cpp_run, facewise):
     assert np.array_equal(tex, mm.materials[0])
     uv = np.fromfile(p + "tile_uv.f32", np.float32).reshape(-1, 2)
     assert np.array_equal(uv, np.asarray(mm.uv, np.float32).reshape(-1, 2))
+
+
+def _atlas(out, p, name):
+    h, w = np.fromfile(os.path.join(out, p + name + "size.i32"), np.int32)
+    tex = np.fromfile(os.path.join(out, p + name + "tex.u8"), np.uint8).reshape(h, w, 3)
+    uv = np.fromfile(os.path.join(out, p + name + "uv.f32"), np.float32).reshape(-1, 2)
+    uvi = np.fromfile(os.path.join(out, p + name + "uvi.i32"), np.int32).reshape(-1, 3)
+    return tex, uv, uvi
+
+
+def _corner_texels(tex, uv, uvi):
+    """Texel under every face corner's uv (nearest texel centre)."""
+    h, w = tex.shape[:2]
+    q = uv[uvi].astype(np.float64)
+    x = np.rint(q[..., 0] * w - 0.5).astype(np.int64).clip(0, w - 1)
+    y = np.rint((1.0 - q[..., 1]) * h - 0.5).astype(np.int64).clip(0, h - 1)
+    return tex[y, x]
+
+
+@pytest.mark.parametrize("facewise", [True, False])
+def test_cpp_atlas_generators(gpu_available, cpp_run, facewise):
+    """kGenerateSimpleTriangles and kGenerateSimpleCharts through the C++
+    drop-in on Spot. Triangles: texture and uvs equal the Python host's and
+    oracle_texture_map of the faces into their grid halves. Charts: each
+    seen corner's uv lands on its projected position in the chosen keyframe
+    moved by a whole number of texels (within 0.1 texel: the face-wise test
+    projects a vertex nudged by 1e-3 of its distance towards each face's
+    centre, up to 0.022 pixel apart on Spot, and a chart keeps the position
+    of its first face naming the vertex), the texel there is
+    the keyframe pixel moved by the same amount, and the texel colours at
+    every corner equal the Python host's (the two may place equal-area
+    charts differently: the C++ drop-in orders them with std::sort as the
+    reference does, the Python host with a stable sort)."""
+    res, out, mesh, kfs = cpp_run
+    assert res.returncode == 0, res.stdout + res.stderr
+    vt = V.VisibilityTester()
+    vt.set_data(mesh.vertices, mesh.normals, mesh.vertex_indices)
+    assert vt.PrepareData()
+    info = V.VisibilityInfo(mesh)
+    assert vt.Test(info, facewise=facewise, keyframes=kfs)
+    p = "fw_" if facewise else "vw_"
+    best = T.best_keyframes(info, T.ViewSelectionCriteria.kMaxArea)
+    F = len(mesh.vertex_indices)
+
+    tex, uv, uvi = _atlas(out, p, "tri_")
+    mm = copy.copy(mesh)
+    opt = T.TextureMappingOption(uv_type=T.TexturingOutputUvType.kGenerateSimpleTriangles,
+                                 tex_w=512, tex_h=512)
+    assert T.TextureMapping(kfs, info, mm, opt)
+    assert np.array_equal(tex, mm.materials[0])
+    assert np.array_equal(uv, np.asarray(mm.uv, np.float32).reshape(-1, 2))
+    assert np.array_equal(uvi, np.arange(3 * F).reshape(F, 3))
+    grid = T.triangle_grid(F, 512, 512)
+    orecs = (O.OracleTexFace * F)()
+    for f, b in enumerate(best):
+        r = orecs[f]
+        r.keyframe = -1 if b is None else int(b.kf_id)
+        if b is None:
+            continue
+        for j, (x, y) in enumerate(T.triangle_corners(grid, f)):
+            r.src[2 * j], r.src[2 * j + 1] = (float(b.projected_tri[j][0]),
+                                              float(b.projected_tri[j][1]))
+            r.dst[2 * j], r.dst[2 * j + 1] = float(x), float(y)
+    rt, _ = O.texture_map([k.color for k in kfs], orecs, 512, 512, 3)
+    assert np.array_equal(tex, rt)
+
+    tex, uv, uvi = _atlas(out, p, "chart_")
+    h, w = tex.shape[:2]
+    checked = 0
+    clear = np.zeros((F, 3), bool)  # corners whose nearest texel is unambiguous
+    for f, b in enumerate(best):
+        if b is None:
+            continue
+        for j in range(3):
+            x, y = (float(c) for c in b.projected_tri[j])
+            tx = float(uv[uvi[f, j], 0]) * w - 0.5
+            ty = (1.0 - float(uv[uvi[f, j], 1])) * h - 0.5
+            ox, oy = int(np.rint(tx - x)), int(np.rint(ty - y))
+            assert abs(tx - ox - x) < 0.1 and abs(ty - oy - y) < 0.1, (f, j)
+            if abs(tx - np.floor(tx) - 0.5) < 0.01 or abs(ty - np.floor(ty) - 0.5) < 0.01:
+                continue
+            clear[f, j] = True
+            px, py = int(np.rint(tx)), int(np.rint(ty))
+            assert np.array_equal(tex[py, px], kfs[b.kf_id].color[py - oy, px - ox]), (f, j)
+            checked += 1
+    mm = copy.copy(mesh)
+    opt = T.TextureMappingOption(uv_type=T.TexturingOutputUvType.kGenerateSimpleCharts,
+                                 tex_w=512, tex_h=512)
+    assert T.TextureMapping(kfs, info, mm, opt)
+    py = _corner_texels(mm.materials[0], np.asarray(mm.uv, np.float32).reshape(-1, 2),
+                        np.asarray(mm.uv_indices))
+    print(f"[cpp-texturing] facewise={facewise}: charts atlas {w}x{h} (python "
+          f"{mm.materials[0].shape[1]}x{mm.materials[0].shape[0]}), corners checked {checked}",
+          flush=True)
+    assert checked > F
+    assert np.array_equal(_corner_texels(tex, uv, uvi)[clear], py[clear])

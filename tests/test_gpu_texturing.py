@@ -108,3 +108,59 @@ def test_spot_pipeline_matches_oracle(gpu_available, spot, criteria):
     ref = ref.reshape(512, 2, 512, 2, 3).mean((1, 3))
     err = np.abs(tex.astype(np.float64) - ref)[seen].mean()
     assert seen.sum() > 0.3 * seen.size and err < 8.0, (seen.sum(), err)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_simple_triangles_match_oracle(gpu_available, seed):
+    """kGenerateSimpleTriangles: the texture equals oracle_texture_map of the
+    seen faces' projected triangles into their grid halves (the layout
+    pinned by hand in tests/test_texturing_host.py) with a 3 x 3 padding
+    kernel whatever the option says; uvs at the corners' texel centres, 0
+    for unseen faces."""
+    from ugu_amd import Mesh
+    rng = np.random.default_rng(seed)
+    imgs = [rng.integers(0, 256, (40 + 9 * k, 56 + 4 * k, 3), dtype=np.uint8) for k in range(3)]
+    kfs = []
+    for i, im in enumerate(imgs):
+        k = V.Keyframe()
+        k.id, k.color = i, im
+        kfs.append(k)
+    F, tw, th = 23, 96, 72
+    mesh = Mesh(np.zeros((30, 3), np.float32), rng.integers(0, 30, (F, 3)).astype(np.int32))
+    info = V.VisibilityInfo(num_vertices=30, num_faces=F)
+    chosen = []
+    for f in range(F):
+        k = int(rng.integers(-1, 3))
+        chosen.append(k)
+        if k < 0:
+            continue
+        r = V.FaceInfoPerKeyframe(k, f)
+        r.area = np.float32(1)
+        h, w = imgs[k].shape[:2]
+        r.projected_tri = rng.uniform([0, 0], [w - 1, h - 1], (3, 2)).astype(np.float32)
+        info.face_info_list[f].visible_keyframes.append(r)
+    info.CalcStatFaceInfo()
+    opt = T.TextureMappingOption(uv_type=T.TexturingOutputUvType.kGenerateSimpleTriangles,
+                                 tex_w=tw, tex_h=th, padding_kernel=5)
+    assert T.TextureMapping(kfs, info, mesh, opt)
+    grid = T.triangle_grid(F, tw, th)
+    orecs = (O.OracleTexFace * F)()
+    exp_uv = np.zeros((F, 3, 2), np.float32)
+    for f in range(F):
+        r = orecs[f]
+        r.keyframe = chosen[f]
+        if chosen[f] < 0:
+            continue
+        src = info.face_info_list[f].visible_keyframes[0].projected_tri
+        for j, (x, y) in enumerate(T.triangle_corners(grid, f)):
+            r.src[2 * j], r.src[2 * j + 1] = float(src[j][0]), float(src[j][1])
+            r.dst[2 * j], r.dst[2 * j + 1] = float(x), float(y)
+            exp_uv[f, j] = ((np.float32(x) + np.float32(0.5)) / np.float32(tw),
+                            np.float32(1) - (np.float32(y) + np.float32(0.5)) / np.float32(th))
+    rt, rm = O.texture_map(imgs, orecs, tw, th, 3)
+    print(f"[texturing] triangles seed={seed}: grid {grid}, set texels {int((rm == 255).sum())}",
+          flush=True)
+    assert np.array_equal(mesh.materials[0], rt)
+    assert np.array_equal(np.asarray(mesh.uv, np.float32).reshape(F, 3, 2)[np.asarray(
+        mesh.uv_indices)[:, 0] // 3], exp_uv)
+    assert np.array_equal(np.asarray(mesh.uv_indices), np.arange(3 * F).reshape(F, 3))

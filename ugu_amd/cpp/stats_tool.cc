@@ -1,5 +1,6 @@
 // Host-only driver of the visibility statistics (VertexInfo::CalcStat,
-// FaceInfo::CalcStat) and of TextureMapping with kGenerateSimpleTile, run by
+// FaceInfo::CalcStat) and of TextureMapping with kGenerateSimpleTile and
+// kGenerateSimpleCharts (the uv types that need no GPU), run by
 // tests/test_texturing_host.py against hand-computed values. Nothing here
 // touches a GPU.
 //
@@ -14,7 +15,11 @@
 //       dir/kf<i>.raw: h*w*3 u8; dir/faces.txt: per face "m" then m lines
 //       "kf_id area angle distance x0 y0 x1 y1 x2 y2";
 //       writes dir/uv.f32, dir/uvi.i32, dir/tex.u8 and prints "rc w h"
+//   ugu_stats_tool map <dir>
+//       as tile, with dir/meta.txt "n_kf w h n_faces criteria uv_type tex_w
+//       tex_h" and the mesh's faces in dir/vi.txt (three vertex ids per face)
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -125,10 +130,17 @@ void dump(const std::string& path, const T* p, size_t n) {
   f.write(reinterpret_cast<const char*>(p), std::streamsize(n * sizeof(T)));
 }
 
-int tile(const std::string& dir) {
+int tile(const std::string& dir, bool with_mesh) {
   std::ifstream meta(dir + "/meta.txt");
   int n = 0, w = 0, h = 0, F = 0, criteria = 0;
   meta >> n >> w >> h >> F >> criteria;
+  ugu::TextureMappingOption opt;  // default uv type: kGenerateSimpleTile
+  opt.criteria = static_cast<ugu::ViewSelectionCriteria>(criteria);
+  if (with_mesh) {
+    int uv_type = 0;
+    meta >> uv_type >> opt.tex_w >> opt.tex_h;
+    opt.uv_type = static_cast<ugu::TexturingOutputUvType>(uv_type);
+  }
   std::ifstream ids(dir + "/ids.txt");
   std::vector<std::shared_ptr<ugu::Keyframe>> kfs;
   for (int i = 0; i < n; i++) {
@@ -141,7 +153,15 @@ int tile(const std::string& dir) {
   }
   ugu::Mesh mesh;
   std::vector<Eigen::Vector3i> faces(size_t(F), Eigen::Vector3i(0, 1, 2));
-  mesh.set_vertices(std::vector<Eigen::Vector3f>(3, Eigen::Vector3f(0, 0, 0)));
+  int n_vertices = 3;
+  if (with_mesh) {
+    std::ifstream vi(dir + "/vi.txt");
+    for (auto& f : faces) {
+      vi >> f[0] >> f[1] >> f[2];
+      n_vertices = std::max({n_vertices, f[0] + 1, f[1] + 1, f[2] + 1});
+    }
+  }
+  mesh.set_vertices(std::vector<Eigen::Vector3f>(size_t(n_vertices), Eigen::Vector3f(0, 0, 0)));
   mesh.set_vertex_indices(faces);
   ugu::VisibilityInfo info(mesh);
   std::ifstream fin(dir + "/faces.txt");
@@ -157,8 +177,6 @@ int tile(const std::string& dir) {
     }
   }
   info.CalcStatFaceInfo();
-  ugu::TextureMappingOption opt;  // default uv type: kGenerateSimpleTile
-  opt.criteria = static_cast<ugu::ViewSelectionCriteria>(criteria);
   const bool ok = ugu::TextureMapping(kfs, info, &mesh, opt);
   int tw = 0, th = 0;
   if (ok) {
@@ -180,6 +198,7 @@ int main(int argc, char** argv) {
   const std::string cmd = argv[1];
   if (cmd == "vertex") return vertex(argv[2]);
   if (cmd == "face") return face(argv[2]);
-  if (cmd == "tile") return tile(argv[2]);
+  if (cmd == "tile") return tile(argv[2], false);
+  if (cmd == "map") return tile(argv[2], true);
   return 2;
 }

@@ -1,9 +1,10 @@
 // Test driver of the C++ texturing API (built by ugu_amd/cpp/Makefile, run by
 // tests/test_gpu_cpp_texturing.py). Used like UGU's texturing example
 // (examples/ex12_texturing.cc:63-151): LoadObj, keyframes rendered by the
-// Rasterizer, VisibilityTester::Test(keyframes, info), TextureMapping on the
-// mesh's own uv. Writes the keyframe images, every visibility record, the
-// statistics and the textures as raw files for the Python side to compare.
+// Rasterizer, VisibilityTester::Test(keyframes, info), TextureMapping for
+// every uv type. Writes the keyframe images, every visibility record, the
+// statistics, the textures and the generated uvs as raw files for the
+// Python side to compare.
 //
 // usage: test_texturing_api <obj_path> <cams.f64> <ncam> <out_dir>
 // cams.f64: ncam x (W H fx fy cx cy c2w[16 row-major])
@@ -140,13 +141,20 @@ int main(int argc, char** argv) {
     nomask->mask = ugu::Image1b();
     vt.set_keyframe(nomask);
     check(!vt.Test(&info), "test_use_mask_without_mask_fails");
-    // the chart-atlas uv generators are not provided (LOGE, false)
-    for (ugu::TexturingOutputUvType t :
-         {ugu::kGenerateSimpleTriangles, ugu::kGenerateSimpleCharts}) {
+    // the atlas generators give up as the reference does (LOGE, false): a
+    // triangle grid whose rectangles would hold fewer than 6 texels, and
+    // charts for a VisibilityInfo of another face count
+    {
       ugu::TextureMappingOption topt;
-      topt.uv_type = t;
+      topt.uv_type = ugu::kGenerateSimpleTriangles;
+      topt.tex_w = topt.tex_h = 16;
       ugu::Mesh m2 = *mesh;
-      check(!ugu::TextureMapping(keyframes, info, &m2, topt), "atlas_uv_type_not_provided");
+      check(!ugu::TextureMapping(keyframes, info, &m2, topt), "triangles_texture_too_small");
+      topt.uv_type = ugu::kGenerateSimpleCharts;
+      ugu::VisibilityInfo other;
+      other.face_info_list.resize(3);
+      ugu::Mesh m3 = *mesh;
+      check(!ugu::TextureMapping(keyframes, other, &m3, topt), "charts_face_count_mismatch");
     }
   }
   ugu::set_log_level(ugu::LogLevel::kError);
@@ -200,6 +208,39 @@ int main(int argc, char** argv) {
         uv.push_back(t[1]);
       }
       write_vec(p + "tile_uv.f32", uv);
+    }
+    // the atlas generators (texture_mapper.cc:347-922): texture, size, uv and
+    // uv indices
+    const ugu::TexturingOutputUvType atlas[2] = {ugu::kGenerateSimpleTriangles,
+                                                 ugu::kGenerateSimpleCharts};
+    const char* atlas_name[2] = {"tri_", "chart_"};
+    for (int t = 0; t < 2; t++) {
+      ugu::TextureMappingOption topt;
+      topt.uv_type = atlas[t];
+      topt.tex_w = 512;
+      topt.tex_h = 512;
+      ugu::Mesh m2 = *mesh;
+      const bool ok = ugu::TextureMapping(keyframes, info, &m2, topt);
+      check(ok && m2.materials().size() == 1 &&
+                m2.uv_indices().size() == m2.vertex_indices().size(),
+            atlas_name[t]);
+      if (!ok) continue;
+      const auto& tex = m2.materials()[0].diffuse_tex;
+      write_img(p + atlas_name[t] + "tex.u8", tex);
+      write_vec(p + atlas_name[t] + "size.i32", std::vector<int32_t>{tex.rows, tex.cols});
+      std::vector<float> uv;
+      for (const auto& q : m2.uv()) {
+        uv.push_back(q[0]);
+        uv.push_back(q[1]);
+      }
+      write_vec(p + atlas_name[t] + "uv.f32", uv);
+      std::vector<int32_t> uvi;
+      for (const auto& q : m2.uv_indices()) {
+        uvi.push_back(q[0]);
+        uvi.push_back(q[1]);
+        uvi.push_back(q[2]);
+      }
+      write_vec(p + atlas_name[t] + "uvi.i32", uvi);
     }
   }
   std::printf("FAILURES %d\n", g_fail);

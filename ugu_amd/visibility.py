@@ -21,8 +21,8 @@ CalcStat fills the per-vertex / per-face statistics that do not depend on
 library internals: means, minima (with index, keyframe id and colour),
 weighted averages and weighted medians (NormalizeWeights, math_util.h:29-77),
 medians for lists of 1, 2 or an even number of entries and odd ones by the
-reference's formula on the sorted list (see `median`), minimum intensity.
-The colour modes (`Mode`) are not computed.
+reference's formula on the sorted list (see `median`), minimum intensity,
+and the colour modes (`mode`, plain and weighted by inverse viewing angle).
 """
 import ctypes
 import math
