@@ -338,8 +338,13 @@ UGU_DEV void append_finish(const DevBins& bins, const AppendTicket* tk, const in
 
 // Entry i of a tile's bin: the front segment's nA entries, then the back
 // segment read from the bin's last slot downwards.
+// Each entry is read once, so the load is non-temporal: the stream of entries
+// does not evict the face records, shading records and texels the rasterizer
+// reads again (64-view batch: rasterizer -2%).
 UGU_DEV int2 bin_entry(const int2* bin, int nA, int cap, int i) {
-  return i < nA ? bin[i] : bin[cap - 1 - (i - nA)];
+  const int2* p = i < nA ? bin + i : bin + (cap - 1 - (i - nA));
+  const long long q = __builtin_nontemporal_load(reinterpret_cast<const long long*>(p));
+  return make_int2(static_cast<int>(q & 0xffffffffll), static_cast<int>(q >> 32));
 }
 
 // Skip depth of a face in a view: a lower bound of every depth the coverage
@@ -844,27 +849,12 @@ UGU_DEV void write_gbuffer(const DevView& v, int64_t pidx, bool covered, float u
   } else {
     n0 = n1 = n2 = 0.0f;
   }
-  if (v.bary) {
-    gptr<float> o = to_global(v.bary + pidx * 3);
-    o[0] = b[0]; o[1] = b[1]; o[2] = b[2];
-  }
-  if (v.uvo) {
-    gptr<float> o = to_global(v.uvo + pidx * 3);
-    o[0] = t[0]; o[1] = t[1]; o[2] = t[2];
-  }
-  if (v.pos_wld) {
-    gptr<float> o = to_global(v.pos_wld + pidx * 3);
-    o[0] = pw[0]; o[1] = pw[1]; o[2] = pw[2];
-  }
-  if (v.pos_cam) {
-    gptr<float> o = to_global(v.pos_cam + pidx * 3);
-    o[0] = pc[0]; o[1] = pc[1]; o[2] = pc[2];
-  }
-  if (v.nrm_wld) {
-    gptr<float> o = to_global(v.nrm_wld + pidx * 3);
-    o[0] = n0; o[1] = n1; o[2] = n2;
-  }
-  if (v.geo_id) to_global(v.geo_id)[pidx] = covered ? 0 : -1;
+  if (v.bary) put3_nt(v.bary, pidx, b[0], b[1], b[2]);
+  if (v.uvo) put3_nt(v.uvo, pidx, t[0], t[1], t[2]);
+  if (v.pos_wld) put3_nt(v.pos_wld, pidx, pw[0], pw[1], pw[2]);
+  if (v.pos_cam) put3_nt(v.pos_cam, pidx, pc[0], pc[1], pc[2]);
+  if (v.nrm_wld) put3_nt(v.nrm_wld, pidx, n0, n1, n2);
+  if (v.geo_id) put_nt(v.geo_id, pidx, covered ? 0 : -1);
 }
 
 // One screen tile; `seg` is the tile's counter record (front count, back
@@ -892,22 +882,15 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const
     if (gx < v.W && gy < v.H) {
       const int64_t pidx = int64_t(gy) * v.W + gx;
       if (MODE == 0) {
-        if (v.color) {
-          gptr<uint8_t> cp = to_global(v.color + pidx * 3);
-          cp[0] = cp[1] = cp[2] = 0;
-        }
-        if (v.depth) to_global(v.depth)[pidx] = 0.0f;
-        if (v.normal) {
-          gptr<float> np = to_global(v.normal + pidx * 3);
-          np[0] = np[1] = np[2] = 0.0f;
-        }
-        if (v.mask) to_global(v.mask)[pidx] = 0;
+        if (v.color) put3_nt(v.color, pidx, uint8_t(0), uint8_t(0), uint8_t(0));
+        if (v.depth) put_nt(v.depth, pidx, 0.0f);
+        if (v.normal) put3_nt(v.normal, pidx, 0.0f, 0.0f, 0.0f);
+        if (v.mask) put_nt(v.mask, pidx, uint8_t(0));
         write_gbuffer(v, pidx, false, 0.0f, 0.0f, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
       } else {
-        gptr<float> ip = to_global(v.image + pidx * 3);
-        ip[0] = ip[1] = ip[2] = 0.0f;
+        put3_nt(v.image, pidx, 0.0f, 0.0f, 0.0f);
       }
-      if (v.face_id) to_global(v.face_id)[pidx] = -1;
+      if (v.face_id) put_nt(v.face_id, pidx, -1);
     }
     return;
   }
@@ -1194,21 +1177,11 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const
     } else {
       write_gbuffer(v, pidx, false, 0.0f, 0.0f, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
     }
-    if (v.color) {
-      gptr<uint8_t> cp = to_global(v.color + pidx * 3);
-      cp[0] = col[0];
-      cp[1] = col[1];
-      cp[2] = col[2];
-    }
-    if (v.depth) to_global(v.depth)[pidx] = z;
-    if (v.normal) {
-      gptr<float> np = to_global(v.normal + pidx * 3);
-      np[0] = nrm[0];
-      np[1] = nrm[1];
-      np[2] = nrm[2];
-    }
-    if (v.mask) to_global(v.mask)[pidx] = msk;
-    if (v.face_id) to_global(v.face_id)[pidx] = fid;
+    if (v.color) put3_nt(v.color, pidx, col[0], col[1], col[2]);
+    if (v.depth) put_nt(v.depth, pidx, z);
+    if (v.normal) put3_nt(v.normal, pidx, nrm[0], nrm[1], nrm[2]);
+    if (v.mask) put_nt(v.mask, pidx, msk);
+    if (v.face_id) put_nt(v.face_id, pidx, fid);
   } else {
     // differentiable forward: image = colour term * texture term (float64)
     float img[3] = {0.0f, 0.0f, 0.0f};
@@ -1217,11 +1190,8 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const
       diff_eval(v, mesh, fid, gx, gy, mesh.faces + int64_t(fid) * 3, e);
       for (int k = 0; k < 3; k++) img[k] = static_cast<float>(e.c[k] * e.t[k]);
     }
-    gptr<float> ip = to_global(v.image + pidx * 3);
-    ip[0] = img[0];
-    ip[1] = img[1];
-    ip[2] = img[2];
-    if (v.face_id) to_global(v.face_id)[pidx] = fid;
+    put3_nt(v.image, pidx, img[0], img[1], img[2]);
+    if (v.face_id) put_nt(v.face_id, pidx, fid);
   }
 }
 
