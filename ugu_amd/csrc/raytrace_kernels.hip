@@ -133,20 +133,28 @@ __global__ __launch_bounds__(64 * kRtWaves) void k_raytrace(const DevView* __res
   if (hit && o.culling && backface_of(m.fnormals, bf, r0, r1, r2)) hit = false;
 
   if (!hit) {
-    if (v.color) put3_nt(v.color, pi, uint8_t(0), uint8_t(0), uint8_t(0));
-    if (v.depth) put_nt(v.depth, pi, 0.0f);
-    if (v.normal) put3_nt(v.normal, pi, 0.0f, 0.0f, 0.0f);
-    if (v.mask) put_nt(v.mask, pi, uint8_t(0));
-    if (v.face_id) put_nt(v.face_id, pi, -1);
+    if (v.color) {
+      v.color[pi * 3 + 0] = 0;
+      v.color[pi * 3 + 1] = 0;
+      v.color[pi * 3 + 2] = 0;
+    }
+    if (v.depth) v.depth[pi] = 0.0f;
+    if (v.normal) {
+      v.normal[pi * 3 + 0] = 0.0f;
+      v.normal[pi * 3 + 1] = 0.0f;
+      v.normal[pi * 3 + 2] = 0.0f;
+    }
+    if (v.mask) v.mask[pi] = 0;
+    if (v.face_id) v.face_id[pi] = -1;
     return;
   }
-  if (v.face_id) put_nt(v.face_id, pi, bf);
-  if (v.mask) put_nt(v.mask, pi, uint8_t(255));
+  if (v.face_id) v.face_id[pi] = bf;
+  if (v.mask) v.mask[pi] = 255;
   if (v.depth) {
     // hit_pos_c = w2c_R * (org + ray * t) + w2c_t; depth = z * depth_scale
     const float h0 = ox + r0 * bt, h1 = oy + r1 * bt, h2 = oz + r2 * bt;
     const float zc = sum3(v.w2c_R[6] * h0, v.w2c_R[7] * h1, v.w2c_R[8] * h2) + v.w2c_t[2];
-    put_nt(v.depth, pi, zc * o.depth_scale);
+    v.depth[pi] = zc * o.depth_scale;
   }
   float n0, n1, n2;
   FaceShade fs;
@@ -166,14 +174,17 @@ __global__ __launch_bounds__(64 * kRtWaves) void k_raytrace(const DevView* __res
     n1 = w * na[1] + bu * nb[1] + bv * nc[1];
     n2 = w * na[2] + bu * nb[2] + bv * nc[2];
   }
-  if (v.normal)
-    put3_nt(v.normal, pi, sum3(v.w2c_R[0] * n0, v.w2c_R[1] * n1, v.w2c_R[2] * n2),
-            sum3(v.w2c_R[3] * n0, v.w2c_R[4] * n1, v.w2c_R[5] * n2),
-            sum3(v.w2c_R[6] * n0, v.w2c_R[7] * n1, v.w2c_R[8] * n2));
+  if (v.normal) {
+    v.normal[pi * 3 + 0] = sum3(v.w2c_R[0] * n0, v.w2c_R[1] * n1, v.w2c_R[2] * n2);
+    v.normal[pi * 3 + 1] = sum3(v.w2c_R[3] * n0, v.w2c_R[4] * n1, v.w2c_R[5] * n2);
+    v.normal[pi * 3 + 2] = sum3(v.w2c_R[6] * n0, v.w2c_R[7] * n1, v.w2c_R[8] * n2);
+  }
   if (v.color) {
     uint8_t c[3];
     shade_u8(m, o, bf, fs, bu, bv, r0, r1, r2, n0, n1, n2, c);
-    put3_nt(v.color, pi, c[0], c[1], c[2]);
+    v.color[pi * 3 + 0] = c[0];
+    v.color[pi * 3 + 1] = c[1];
+    v.color[pi * 3 + 2] = c[2];
   }
 }
 
