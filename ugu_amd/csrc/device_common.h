@@ -29,23 +29,28 @@ UGU_DEV gptr<T> to_global(T* p) {
 }
 
 // Output planes are written once and never read back by the kernel that
-// writes them: non-temporal stores stream them to memory instead of
-// allocating them in L2, where they would evict the records and texels the
-// kernel reads again (64-view batch: rasterizer -6.5%).
-template <typename T>
-UGU_DEV void put_nt(T* plane, int64_t i, T value) {
-  __builtin_nontemporal_store(value, to_global(plane) + i);
+// writes them. With STREAM they are stored non-temporally: streamed to memory
+// instead of allocated in L2, where they would evict the records and texels
+// the kernel reads again (64-view batch: rasterizer -6.5%). That pays when a
+// wavefront's row segments are long; short ones (8-pixel tiles: 24 B of
+// colour) turn into partial-line writes that L2 would otherwise have merged
+// with the neighbouring tile's.
+template <bool STREAM, typename T>
+UGU_DEV void put(T* plane, int64_t i, T value) {
+  if (STREAM)
+    __builtin_nontemporal_store(value, to_global(plane) + i);
+  else
+    to_global(plane)[i] = value;
 }
 
 // Pixel i of a three-channel plane, its channels stored back to back (one
 // 12-byte store for floats; stores to another plane in between would keep
 // the compiler from merging them, as the planes may alias).
-template <typename T>
-UGU_DEV void put3_nt(T* plane, int64_t i, T c0, T c1, T c2) {
-  gptr<T> p = to_global(plane) + i * 3;
-  __builtin_nontemporal_store(c0, p);
-  __builtin_nontemporal_store(c1, p + 1);
-  __builtin_nontemporal_store(c2, p + 2);
+template <bool STREAM, typename T>
+UGU_DEV void put3(T* plane, int64_t i, T c0, T c1, T c2) {
+  put<STREAM>(plane, i * 3, c0);
+  put<STREAM>(plane, i * 3 + 1, c1);
+  put<STREAM>(plane, i * 3 + 2, c2);
 }
 
 // ---------------------------------------------------------------------------

@@ -821,6 +821,7 @@ UGU_DEV void diff_eval(const DevView& v, const DevMesh& m, int fid, int px, int 
 // winner's perspective-correct weights of its second and third vertex; P: its
 // world vertex positions (null when no position plane is written); uvm: the
 // mesh when the uv plane is written and the mesh has uvs; n: the shading normal.
+template <bool STREAM>
 UGU_DEV void write_gbuffer(const DevView& v, int64_t pidx, bool covered, float u, float vv,
                            const float* P, const DevMesh* uvm, float n0, float n1, float n2,
                            int fid = 0) {
@@ -849,12 +850,12 @@ UGU_DEV void write_gbuffer(const DevView& v, int64_t pidx, bool covered, float u
   } else {
     n0 = n1 = n2 = 0.0f;
   }
-  if (v.bary) put3_nt(v.bary, pidx, b[0], b[1], b[2]);
-  if (v.uvo) put3_nt(v.uvo, pidx, t[0], t[1], t[2]);
-  if (v.pos_wld) put3_nt(v.pos_wld, pidx, pw[0], pw[1], pw[2]);
-  if (v.pos_cam) put3_nt(v.pos_cam, pidx, pc[0], pc[1], pc[2]);
-  if (v.nrm_wld) put3_nt(v.nrm_wld, pidx, n0, n1, n2);
-  if (v.geo_id) put_nt(v.geo_id, pidx, covered ? 0 : -1);
+  if (v.bary) put3<STREAM>(v.bary, pidx, b[0], b[1], b[2]);
+  if (v.uvo) put3<STREAM>(v.uvo, pidx, t[0], t[1], t[2]);
+  if (v.pos_wld) put3<STREAM>(v.pos_wld, pidx, pw[0], pw[1], pw[2]);
+  if (v.pos_cam) put3<STREAM>(v.pos_cam, pidx, pc[0], pc[1], pc[2]);
+  if (v.nrm_wld) put3<STREAM>(v.nrm_wld, pidx, n0, n1, n2);
+  if (v.geo_id) put<STREAM>(v.geo_id, pidx, covered ? 0 : -1);
 }
 
 // One screen tile; `seg` is the tile's counter record (front count, back
@@ -864,6 +865,8 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const
                          const DevMesh& mesh,
                          const DevBins& bins, const DevOptions& opt) {
   constexpr int NT = TW * TH;
+  // output planes streamed past L2 when the tile's rows are long enough
+  constexpr bool S = TW >= 16;
   const int t = threadIdx.x;
 
   const int count = seg.x + seg.y;
@@ -882,15 +885,15 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const
     if (gx < v.W && gy < v.H) {
       const int64_t pidx = int64_t(gy) * v.W + gx;
       if (MODE == 0) {
-        if (v.color) put3_nt(v.color, pidx, uint8_t(0), uint8_t(0), uint8_t(0));
-        if (v.depth) put_nt(v.depth, pidx, 0.0f);
-        if (v.normal) put3_nt(v.normal, pidx, 0.0f, 0.0f, 0.0f);
-        if (v.mask) put_nt(v.mask, pidx, uint8_t(0));
-        write_gbuffer(v, pidx, false, 0.0f, 0.0f, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
+        if (v.color) put3<S>(v.color, pidx, uint8_t(0), uint8_t(0), uint8_t(0));
+        if (v.depth) put<S>(v.depth, pidx, 0.0f);
+        if (v.normal) put3<S>(v.normal, pidx, 0.0f, 0.0f, 0.0f);
+        if (v.mask) put<S>(v.mask, pidx, uint8_t(0));
+        write_gbuffer<S>(v, pidx, false, 0.0f, 0.0f, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
       } else {
-        put3_nt(v.image, pidx, 0.0f, 0.0f, 0.0f);
+        put3<S>(v.image, pidx, 0.0f, 0.0f, 0.0f);
       }
-      if (v.face_id) put_nt(v.face_id, pidx, -1);
+      if (v.face_id) put<S>(v.face_id, pidx, -1);
     }
     return;
   }
@@ -1171,17 +1174,17 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const
         float P[9], N[3];
         const bool pos = v.pos_wld || v.pos_cam;
         if (pos) load_face_rec(mesh.soup, fid, P, N);
-        write_gbuffer(v, pidx, true, w1, w2, pos ? P : nullptr,
+        write_gbuffer<S>(v, pidx, true, w1, w2, pos ? P : nullptr,
                       v.uvo && mesh.uv && mesh.uvidx ? &mesh : nullptr, n0, n1, n2, fid);
       }
     } else {
-      write_gbuffer(v, pidx, false, 0.0f, 0.0f, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
+      write_gbuffer<S>(v, pidx, false, 0.0f, 0.0f, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
     }
-    if (v.color) put3_nt(v.color, pidx, col[0], col[1], col[2]);
-    if (v.depth) put_nt(v.depth, pidx, z);
-    if (v.normal) put3_nt(v.normal, pidx, nrm[0], nrm[1], nrm[2]);
-    if (v.mask) put_nt(v.mask, pidx, msk);
-    if (v.face_id) put_nt(v.face_id, pidx, fid);
+    if (v.color) put3<S>(v.color, pidx, col[0], col[1], col[2]);
+    if (v.depth) put<S>(v.depth, pidx, z);
+    if (v.normal) put3<S>(v.normal, pidx, nrm[0], nrm[1], nrm[2]);
+    if (v.mask) put<S>(v.mask, pidx, msk);
+    if (v.face_id) put<S>(v.face_id, pidx, fid);
   } else {
     // differentiable forward: image = colour term * texture term (float64)
     float img[3] = {0.0f, 0.0f, 0.0f};
@@ -1190,8 +1193,8 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const
       diff_eval(v, mesh, fid, gx, gy, mesh.faces + int64_t(fid) * 3, e);
       for (int k = 0; k < 3; k++) img[k] = static_cast<float>(e.c[k] * e.t[k]);
     }
-    put3_nt(v.image, pidx, img[0], img[1], img[2]);
-    if (v.face_id) put_nt(v.face_id, pidx, fid);
+    put3<S>(v.image, pidx, img[0], img[1], img[2]);
+    if (v.face_id) put<S>(v.face_id, pidx, fid);
   }
 }
 
