@@ -54,6 +54,48 @@ UGU_DEV void put3(T* plane, int64_t i, T c0, T c1, T c2) {
 }
 
 // ---------------------------------------------------------------------------
+// Wavefront scans and reductions through the DPP cross-lane paths: four
+// shifts within each row of 16 lanes, then the row broadcasts of lanes 15
+// and 31 -- register moves, where __shfl_* is an LDS round trip per step.
+// Lanes whose source lies outside their row (or in a row the mask leaves
+// out) take `id`, the operation's identity.
+
+template <typename Op>
+UGU_DEV int wave_scan_dpp(int x, int id, Op op) {
+  x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+  x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+  x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+  x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+  x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return x;
+}
+
+// Inclusive prefix sum in lane order.
+UGU_DEV int wave_incl_sum(int x) {
+  return wave_scan_dpp(x, 0, [](int a, int b) { return a + b; });
+}
+
+// Inclusive running maximum of non-negative values.
+UGU_DEV int wave_incl_max(int x) {
+  return wave_scan_dpp(x, 0, [](int a, int b) { return max(a, b); });
+}
+
+// Maximum / minimum over the wavefront, the same in every lane.
+UGU_DEV uint32_t wave_max_u32(uint32_t v) {
+  const int r = wave_scan_dpp(static_cast<int>(v), 0, [](int a, int b) {
+    return static_cast<int>(max(static_cast<uint32_t>(a), static_cast<uint32_t>(b)));
+  });
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(r, 63));
+}
+UGU_DEV uint32_t wave_min_u32(uint32_t v) {
+  const int r = wave_scan_dpp(static_cast<int>(v), -1, [](int a, int b) {
+    return static_cast<int>(min(static_cast<uint32_t>(a), static_cast<uint32_t>(b)));
+  });
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(r, 63));
+}
+
+// ---------------------------------------------------------------------------
 // scalar helpers
 
 // Eigen's unrolled reduction of a fixed 3-vector: x0 + (x1 + x2)

@@ -278,11 +278,8 @@ struct AppendTicket {
 // fold it into their tile's back-segment minimum.
 UGU_DEV uint32_t back_zmin(bool back, float zskip) {
   uint32_t zmin = 0xffffffffu;
-  if (__ballot(back)) {
-    zmin = back ? (zskip >= 0.0f ? __float_as_uint(zskip) : 0u) : 0xffffffffu;
-    for (int off = 32; off > 0; off >>= 1)
-      zmin = min(zmin, static_cast<uint32_t>(__shfl_xor(static_cast<int>(zmin), off)));
-  }
+  if (__ballot(back))
+    zmin = wave_min_u32(back ? (zskip >= 0.0f ? __float_as_uint(zskip) : 0u) : 0xffffffffu);
   return zmin;
 }
 
@@ -981,14 +978,12 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const
     // block exclusive scan of the pair counts; a chunk without any pair
     // (every face skipped or off the tile) ends at the barrier
     const int lane = t & 63, wid = t >> 6;
-    int x = c, cmax = c;
-    for (int off = 1; off < 64; off <<= 1) {
-      const int y = __shfl_up(x, off);
-      if (lane >= off) x += y;
-      cmax = max(cmax, __shfl_xor(cmax, off));
+    const int x = wave_incl_sum(c);
+    const int cmax = wave_incl_max(c);
+    if (lane == 63) {
+      s.wsum[wid] = x;
+      s.wbig[wid] = cmax;
     }
-    if (lane == 63) s.wsum[wid] = x;
-    if (lane == 0) s.wbig[wid] = cmax;
     if (!__syncthreads_or(c > 0)) continue;
     int wbase = 0, total = 0, big = 0;
     for (int w = 0; w < NT / 64; w++) {
@@ -1077,7 +1072,7 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const
       // depth bits of this pixel: all ones when it has no key yet, 0 outside
       // the image (no face reaches those pixels)
       uint32_t db = (gx < v.W && gy < v.H) ? static_cast<uint32_t>(k >> 32) : 0u;
-      for (int off = 32; off > 0; off >>= 1) db = max(db, static_cast<uint32_t>(__shfl_xor(static_cast<int>(db), off)));
+      db = wave_max_u32(db);
       if ((t & 63) == 0) s.wmax[t >> 6] = db;
     }
     if (slot_key && k != seen) {
