@@ -81,6 +81,14 @@ UGU_DEV int wave_incl_max(int x) {
   return wave_scan_dpp(x, 0, [](int a, int b) { return max(a, b); });
 }
 
+// The previous / next lane's value (lane 0 / lane 63 get `edge`).
+UGU_DEV int wave_prev(int v, int edge) {
+  return __builtin_amdgcn_update_dpp(edge, v, 0x138, 0xf, 0xf, false);  // wave_shr:1
+}
+UGU_DEV int wave_next(int v, int edge) {
+  return __builtin_amdgcn_update_dpp(edge, v, 0x130, 0xf, 0xf, false);  // wave_shl:1
+}
+
 // Maximum / minimum over the wavefront, the same in every lane.
 UGU_DEV uint32_t wave_max_u32(uint32_t v) {
   const int r = wave_scan_dpp(static_cast<int>(v), 0, [](int a, int b) {

@@ -1270,7 +1270,7 @@ struct FaceRun {
 
 UGU_DEV FaceRun face_run(int key) {
   const int lane = __lane_id();
-  const int prev = __shfl_up(key, 1);
+  const int prev = wave_prev(key, key);
   FaceRun r;
   r.head = lane == 0 || prev != key;
   const uint64_t heads = __ballot(r.head);
@@ -1327,11 +1327,7 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const int32_t* __restrict_
     v[k] = b0 + k < n ? in[b0 + k] : 0;
     tot += v[k];
   }
-  int x = tot;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
+  const int x = wave_incl_sum(tot);
   if (lane == 63) wsum[wid] = x;
   __syncthreads();
   int base = 0, all = 0;
@@ -1356,11 +1352,7 @@ __global__ __launch_bounds__(1024) void k_scan_sums(int32_t* __restrict__ sums, 
   __syncthreads();
   for (int c0 = 0; c0 < nb; c0 += 1024) {
     const int val = c0 + t < nb ? sums[c0 + t] : 0;
-    int x = val;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o);
-      if (lane >= o) x += y;
-    }
+    const int x = wave_incl_sum(val);
     if (lane == 63) wsum[wid] = x;
     __syncthreads();
     int base = carry, all = 0;
@@ -1383,15 +1375,25 @@ __global__ void k_scan_add(int32_t* __restrict__ out, int n, const int32_t* __re
   if (i == 0) out[n] = sums[nb];
 }
 
-// Sums `val` over contiguous runs of equal `run` ids; the run's first lane
-// ends up with the run total.
+// Sums `val` over contiguous runs of equal `run` ids (an inclusive scan
+// segmented by run, through the DPP paths of wave_scan_dpp); the run's last
+// lane ends up with the run total. A source lane outside the row or the
+// row mask brings run INT_MIN, which no run has.
 UGU_DEV float run_sum(float val, int run) {
-  const int lane = __lane_id();
-  for (int off = 1; off < 64; off <<= 1) {
-    const float o = __shfl_down(val, off);
-    const int orun = __shfl_down(run, off);
-    if (lane + off < 64 && orun == run) val += o;
+  constexpr int kNone = INT_MIN;
+#define UGU_RUN_STEP(ctrl, rows)                                                          \
+  {                                                                                     \
+    const int o = __builtin_amdgcn_update_dpp(0, __float_as_int(val), ctrl, rows, 0xf, false); \
+    const int orun = __builtin_amdgcn_update_dpp(kNone, run, ctrl, rows, 0xf, false);   \
+    if (orun == run) val += __int_as_float(o);                                          \
   }
+  UGU_RUN_STEP(0x111, 0xf)  // row_shr:1
+  UGU_RUN_STEP(0x112, 0xf)  // row_shr:2
+  UGU_RUN_STEP(0x114, 0xf)  // row_shr:4
+  UGU_RUN_STEP(0x118, 0xf)  // row_shr:8
+  UGU_RUN_STEP(0x142, 0xa)  // row_bcast:15
+  UGU_RUN_STEP(0x143, 0xc)  // row_bcast:31
+#undef UGU_RUN_STEP
   return val;
 }
 
@@ -1567,9 +1569,12 @@ __global__ __launch_bounds__(256) void k_bwd_items(
     }
 
     // wavefront pre-reduction over runs of equal face id, then atomics
+    // from each run's last lane
     const int lane = __lane_id();
-    const int prev_fid = __shfl_up(fid, 1);
+    const int prev_fid = wave_prev(fid, INT_MIN);
+    const int next_fid = wave_next(fid, INT_MIN);
     const bool head = valid && (lane == 0 || prev_fid != fid);
+    const bool tail = valid && (lane == 63 || next_fid != fid);
     const uint64_t heads = __ballot(head);
     const uint64_t vmask = __ballot(valid);
     const bool reduce = heads != vmask;  // some run longer than one lane
@@ -1577,7 +1582,7 @@ __global__ __launch_bounds__(256) void k_bwd_items(
     if (g.colors && mesh.vcolors) {
       for (int k = 0; k < 9; k++) {
         const float val = reduce ? run_sum(gcol[k], run) : gcol[k];
-        if (head) unsafeAtomicAdd(&g.colors[int64_t(vid[k / 3]) * 3 + (k % 3)], val);
+        if (tail) unsafeAtomicAdd(&g.colors[int64_t(vid[k / 3]) * 3 + (k % 3)], val);
       }
     }
     if (tex && valid) {
@@ -1653,15 +1658,17 @@ __global__ __launch_bounds__(256) void k_bwd_items(
       }
     }
     const int lane = __lane_id();
-    const int prev_fid = __shfl_up(fid, 1);
+    const int prev_fid = wave_prev(fid, INT_MIN);
+    const int next_fid = wave_next(fid, INT_MIN);
     const bool head = valid && (lane == 0 || prev_fid != fid);
+    const bool tail = valid && (lane == 63 || next_fid != fid);
     const uint64_t heads = __ballot(head);
     const uint64_t vmask = __ballot(valid);
     const bool reduce = heads != vmask;
     const int run = valid ? __popcll(heads & ((2ull << lane) - 1ull)) : -1 - lane;
     for (int k = 0; k < 9; k++) {
       const float val = reduce ? run_sum(gpos[k], run) : gpos[k];
-      if (head) unsafeAtomicAdd(&g.verts[int64_t(vid[k / 3]) * 3 + (k % 3)], val);
+      if (tail) unsafeAtomicAdd(&g.verts[int64_t(vid[k / 3]) * 3 + (k % 3)], val);
     }
   }
   }  // rounds
