@@ -820,8 +820,8 @@ UGU_DEV void diff_eval(const DevView& v, const DevMesh& m, int fid, int px, int 
 // mesh when the uv plane is written and the mesh has uvs; n: the shading normal.
 template <bool STREAM>
 UGU_DEV void write_gbuffer(const DevView& v, int64_t pidx, bool covered, float u, float vv,
-                           const float* P, const DevMesh* uvm, float n0, float n1, float n2,
-                           int fid = 0) {
+                           bool has_p, const float* P, const DevMesh* uvm, float n0, float n1,
+                           float n2, int fid = 0) {
   float b[3] = {0.0f, 0.0f, 0.0f}, t[3] = {0.0f, 0.0f, 0.0f};
   float pw[3] = {0.0f, 0.0f, 0.0f}, pc[3] = {0.0f, 0.0f, 0.0f};
   if (covered) {
@@ -838,7 +838,7 @@ UGU_DEV void write_gbuffer(const DevView& v, int64_t pidx, bool covered, float u
       t[1] = w * a[1] + u * bb[1] + vv * c[1];
       t[2] = 1.0f - t[0] - t[1];
     }
-    if (P) {
+    if (has_p) {
       for (int k = 0; k < 3; k++) pw[k] = w * P[k] + u * P[3 + k] + vv * P[6 + k];
       for (int r = 0; r < 3; r++)
         pc[r] = v.w2c_t[r] + sum3(v.w2c_R[r * 3 + 0] * pw[0], v.w2c_R[r * 3 + 1] * pw[1],
@@ -886,7 +886,7 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const
         if (v.depth) put<S>(v.depth, pidx, 0.0f);
         if (v.normal) put3<S>(v.normal, pidx, 0.0f, 0.0f, 0.0f);
         if (v.mask) put<S>(v.mask, pidx, uint8_t(0));
-        write_gbuffer<S>(v, pidx, false, 0.0f, 0.0f, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
+        write_gbuffer<S>(v, pidx, false, 0.0f, 0.0f, false, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
       } else {
         put3<S>(v.image, pidx, 0.0f, 0.0f, 0.0f);
       }
@@ -1166,14 +1166,16 @@ UGU_DEV void raster_tile(RasterLds<TW * TH>& s, int gtile, const int4 seg, const
       }
       if (v.color) shade_u8(mesh, opt, fid, fs, w1, w2, r0, r1, r2, n0, n1, n2, col);
       if (v.bary || v.uvo || v.pos_wld || v.pos_cam || v.nrm_wld || v.geo_id) {
-        float P[9], N[3];
+        // P is passed with a flag, not as a pointer that may be null: a select
+        // between pointers would keep the array out of registers
+        float P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, N[3];
         const bool pos = v.pos_wld || v.pos_cam;
         if (pos) load_face_rec(mesh.soup, fid, P, N);
-        write_gbuffer<S>(v, pidx, true, w1, w2, pos ? P : nullptr,
-                      v.uvo && mesh.uv && mesh.uvidx ? &mesh : nullptr, n0, n1, n2, fid);
+        write_gbuffer<S>(v, pidx, true, w1, w2, pos, P,
+                         v.uvo && mesh.uv && mesh.uvidx ? &mesh : nullptr, n0, n1, n2, fid);
       }
     } else {
-      write_gbuffer<S>(v, pidx, false, 0.0f, 0.0f, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
+      write_gbuffer<S>(v, pidx, false, 0.0f, 0.0f, false, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
     }
     if (v.color) put3<S>(v.color, pidx, col[0], col[1], col[2]);
     if (v.depth) put<S>(v.depth, pidx, z);
