@@ -81,6 +81,28 @@ UGU_DEV int wave_incl_max(int x) {
   return wave_scan_dpp(x, 0, [](int a, int b) { return max(a, b); });
 }
 
+// Sums `val` over contiguous runs of equal `run` ids (an inclusive scan
+// segmented by run, through the DPP paths of wave_scan_dpp); the run's last
+// lane ends up with the run total. A source lane outside the row or the
+// row mask brings run INT_MIN, which no run has.
+UGU_DEV float run_sum(float val, int run) {
+  constexpr int kNone = INT_MIN;
+#define UGU_RUN_STEP(ctrl, rows)                                                          \
+  {                                                                                     \
+    const int o = __builtin_amdgcn_update_dpp(0, __float_as_int(val), ctrl, rows, 0xf, false); \
+    const int orun = __builtin_amdgcn_update_dpp(kNone, run, ctrl, rows, 0xf, false);   \
+    if (orun == run) val += __int_as_float(o);                                          \
+  }
+  UGU_RUN_STEP(0x111, 0xf)  // row_shr:1
+  UGU_RUN_STEP(0x112, 0xf)  // row_shr:2
+  UGU_RUN_STEP(0x114, 0xf)  // row_shr:4
+  UGU_RUN_STEP(0x118, 0xf)  // row_shr:8
+  UGU_RUN_STEP(0x142, 0xa)  // row_bcast:15
+  UGU_RUN_STEP(0x143, 0xc)  // row_bcast:31
+#undef UGU_RUN_STEP
+  return val;
+}
+
 // The previous / next lane's value (lane 0 / lane 63 get `edge`).
 UGU_DEV int wave_prev(int v, int edge) {
   return __builtin_amdgcn_update_dpp(edge, v, 0x138, 0xf, 0xf, false);  // wave_shr:1

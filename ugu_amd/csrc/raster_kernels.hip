@@ -1377,27 +1377,6 @@ __global__ void k_scan_add(int32_t* __restrict__ out, int n, const int32_t* __re
   if (i == 0) out[n] = sums[nb];
 }
 
-// Sums `val` over contiguous runs of equal `run` ids (an inclusive scan
-// segmented by run, through the DPP paths of wave_scan_dpp); the run's last
-// lane ends up with the run total. A source lane outside the row or the
-// row mask brings run INT_MIN, which no run has.
-UGU_DEV float run_sum(float val, int run) {
-  constexpr int kNone = INT_MIN;
-#define UGU_RUN_STEP(ctrl, rows)                                                          \
-  {                                                                                     \
-    const int o = __builtin_amdgcn_update_dpp(0, __float_as_int(val), ctrl, rows, 0xf, false); \
-    const int orun = __builtin_amdgcn_update_dpp(kNone, run, ctrl, rows, 0xf, false);   \
-    if (orun == run) val += __int_as_float(o);                                          \
-  }
-  UGU_RUN_STEP(0x111, 0xf)  // row_shr:1
-  UGU_RUN_STEP(0x112, 0xf)  // row_shr:2
-  UGU_RUN_STEP(0x114, 0xf)  // row_shr:4
-  UGU_RUN_STEP(0x118, 0xf)  // row_shr:8
-  UGU_RUN_STEP(0x142, 0xa)  // row_bcast:15
-  UGU_RUN_STEP(0x143, 0xc)  // row_bcast:31
-#undef UGU_RUN_STEP
-  return val;
-}
 
 // Texel gradients of a block's items are first summed in an LDS hash table
 // (open addressing on the texel index, linear probing); a block's items cover
