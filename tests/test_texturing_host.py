@@ -519,3 +519,31 @@ def test_triangle_grid_by_hand():
     assert T.triangle_grid(10, 4, 4) is None
     assert T.triangle_grid(40, 20, 20) is None
     assert T.triangle_grid(0, 64, 64) is None
+
+
+@pytest.mark.parametrize("impl", ["cpp", "python"])
+def test_simple_charts_shared_directed_edge(impl, tmp_path):
+    """Faces 1 and 2 both hold the directed edge 0 -> 1 (inconsistent
+    winding): its entry is the sum of their ids + 1, (1 + 1) + (2 + 1) = 5, as
+    the reference's setFromTriplets sums duplicates. Face 0 (edge 1 -> 0) then
+    finds "face 4" across that edge -- an isolated face -- and grows the chart
+    {0, 4}; faces 1, 2 and 3 are charts of their own (face 0's edges are
+    cleared before they are reached). Uv entries: 6 + 3 + 3 + 3 chart
+    vertices and the 3 shared unseen entries = 18. Keeping the first or the
+    last face of the edge instead would pair face 0 with face 1 or 2 (16)."""
+    vi = [(1, 0, 4), (0, 1, 2), (0, 1, 3), (5, 6, 7), (8, 9, 10)]
+    kfs = [0] * 5
+    proj = {(v, 0): (1.0 + v % 6, 1.0 + v // 6) for v in range(11)}
+    imgs = [np.random.default_rng(4).integers(0, 256, (CH, CW, 3), dtype=np.uint8)]
+    got = _map(impl, tmp_path, imgs, vi, kfs, proj,
+               T.TexturingOutputUvType.kGenerateSimpleCharts, 32, 32)
+    assert got is not None
+    tex, uv, uvi = got
+    assert len(uv) == 18
+    # face 0 and face 4 share one chart, whose patch spans the bbox of both:
+    # x 1..5, y 1..2 -> 4 x 7 = 28 texels with the border. By patch area the
+    # charts are {3} (x 1..6, y 1..2: 32), {0, 4} (28), {2} (x 1..4: 18),
+    # {1} (x 1..3: 15), so {0, 4} holds uv entries 3..8 (vertex ids in order)
+    assert set(uvi[0]) | set(uvi[4]) == set(range(3, 9))
+    assert tuple(uvi[3]) == (0, 1, 2) and set(uvi[2]) == set(range(9, 12))
+    assert set(uvi[1]) == set(range(12, 15))
