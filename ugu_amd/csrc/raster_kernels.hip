@@ -555,13 +555,9 @@ __global__ __launch_bounds__(256) void k_setup_bin(
     // last lane whose exclusive prefix is <= the pair) and the tile ordinal.
     if (__any(rest > 0)) {
       const int lane = __lane_id();
-      int incl = rest;
-      for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(incl, off);
-        if (lane >= off) incl += y;
-      }
+      const int incl = wave_incl_sum(rest);
       const int excl = incl - rest;
-      const int total = __shfl(incl, 63);
+      const int total = __builtin_amdgcn_readlane(incl, 63);
       for (int base = 0; base < total; base += 64) {
         const int p = base + lane;
         int src = 0;
