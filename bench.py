@@ -149,10 +149,14 @@ def diff_benchmarks(torch, np, scenes, mesh, cams, args):
     from ugu_amd.diff import DiffRenderer
     out = {}
     dev = torch.device("cuda")
+    # the L2 loss as one reduction (and one gradient kernel) and Adam as one
+    # fused update: the torch side of a step stays a few passes over the
+    # images and the texture
+    l2 = lambda a, b: torch.nn.functional.mse_loss(a, b, reduction="sum")
 
     def fit_step(dr, cam_list, v, c, t, targets, opt=None):
         imgs = dr.render(cam_list, v, c, t)
-        loss = sum(((im - tg) ** 2).sum() for im, tg in zip(imgs, targets))
+        loss = sum(l2(im, tg) for im, tg in zip(imgs, targets))
         if opt is not None:
             opt.zero_grad(set_to_none=True)
         loss.backward()
@@ -197,7 +201,7 @@ def diff_benchmarks(torch, np, scenes, mesh, cams, args):
 
     def batched_step():
         flat, _ = dr.render_flat(abi, pv, pc, pt)
-        ((flat - tg_flat) ** 2).sum().backward()
+        l2(flat, tg_flat).backward()
     tb = time_calls(torch, batched_step, max(3, args.steps // 2), 2)
 
     def per_view():
@@ -218,7 +222,7 @@ def diff_benchmarks(torch, np, scenes, mesh, cams, args):
     with torch.no_grad():
         tg5, _ = dr.render_flat(a5, vv, None, gt_tex)
     learn = torch.full_like(gt_tex, 0.5).requires_grad_(True)
-    opt = torch.optim.Adam([learn], lr=1e-2)
+    opt = torch.optim.Adam([learn], lr=1e-2, fused=True)
     seen = None
     n_iter = 200
     for it in range(n_iter + 3):
@@ -227,7 +231,7 @@ def diff_benchmarks(torch, np, scenes, mesh, cams, args):
             t0 = time.perf_counter()
         flat, _ = dr.render_flat(a5, vv, None, learn)
         opt.zero_grad(set_to_none=True)
-        ((flat - tg5) ** 2).sum().backward()
+        l2(flat, tg5).backward()
         opt.step()
         if seen is None:
             # texels that receive gradient from at least one view
@@ -240,7 +244,7 @@ def diff_benchmarks(torch, np, scenes, mesh, cams, args):
 
     def tex_step():
         flat, _ = dr.render_flat(a5, vv, None, learn)
-        ((flat - tg5) ** 2).sum().backward()
+        l2(flat, tg5).backward()
     roof5 = backward_roofline(dr, tex_step, mesh, len(mesh.vertex_indices),
                               gt_tex.shape[0] * gt_tex.shape[1], False, False)
     out["inverse_texturing_32view"] = {"iters_per_s": 1.0 / t5, "ms_per_iter": t5 * 1e3,
