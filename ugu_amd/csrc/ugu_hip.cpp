@@ -1477,7 +1477,7 @@ int ugu_hip_autotune_cached(ugu_hip_ctx* c, const UguCamera* cams, int n, const 
   }
   // kTuneRevision changes whenever the kernels' binning or cost changes, so
   // choices tuned against older kernels are not reused
-  constexpr int kTuneRevision = 3;
+  constexpr int kTuneRevision = 4;
   char key[256];
   std::snprintf(key, sizeof(key), "r%d|%s|cu%d|F%d|n%d|px%lld|s%016llx|o%d%d%d%d", kTuneRevision,
                 c->arch.empty() ? "unknown" : c->arch.c_str(), c->cu_count, c->F, n,
