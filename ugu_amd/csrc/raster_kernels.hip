@@ -156,6 +156,7 @@ __global__ void k_clear(int32_t* counts, int32_t n, int32_t* status,
     reinterpret_cast<int4*>(counts)[i] = make_int4(0, 0, 0x7f800000, lo);
   }
   if (i < 4) status[i] = 0;
+  if (i < kQueues) status[kQueueStride * (i + 1)] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1209,18 +1210,32 @@ __global__ __launch_bounds__(TW * TH, MODE == 0 ? 7 : 6) void k_raster(
     return;
   }
   // persistent: pull runs of kQueueRun consecutive (view, tile) items until
-  // the queue is drained; every block exits once the head passes total_tiles.
-  // The next run is claimed when the last tile of the current one starts, so
-  // the atomic's round trip overlaps that tile. The barrier before reading
-  // s.item publishes thread 0's write (an empty tile returns from raster_tile
-  // without a barrier); the one after keeps thread 0 from overwriting it
-  // before every thread has read it. `item` is the same in every thread, so
-  // every thread leaves the loop together.
-  // The counter records of a run's tiles are read together into LDS, one
-  // round trip per run (most tiles of a batch are empty and need nothing else).
-  constexpr int kQueueRun = 4;
+  // every queue is drained. Run r belongs to queue r % kQueues; a block starts
+  // at queue blockIdx.x % kQueues and, once that one is drained, takes the
+  // next queue's runs. Each head only grows and every run below n_runs is
+  // handed out once, so a block that finds all kQueues heads past their
+  // runs (one atomic each, at most kQueues in a row) leaves with every run
+  // claimed. The next run is claimed when the last tile of the current one
+  // starts. The barrier before reading s.item publishes thread 0's write (an
+  // empty tile returns from raster_tile without a barrier); the one after
+  // keeps thread 0 from overwriting it before every thread has read it.
+  // `item` is the same in every thread, so every thread leaves the loop
+  // together. The counter records of a run's tiles are read together into
+  // LDS, one round trip per run (most tiles of a batch are empty and need
+  // nothing else).
+  constexpr int kQueueRun = 2;
   static_assert(kQueueRun <= 4, "RasterLds::runseg holds one run");
-  if (threadIdx.x == 0) s.item = atomicAdd(&bins.status[1], kQueueRun);
+  const int n_runs = (bins.total_tiles + kQueueRun - 1) / kQueueRun;
+  int queue = static_cast<int>(blockIdx.x % kQueues);  // used by thread 0
+  auto claim = [&]() {
+    for (int k = 0; k < kQueues; k++) {
+      const int r = atomicAdd(&bins.status[kQueueStride * (queue + 1)], 1);
+      if (r < (n_runs - queue + kQueues - 1) / kQueues) return (queue + kQueues * r) * kQueueRun;
+      queue = (queue + 1) % kQueues;
+    }
+    return bins.total_tiles;
+  };
+  if (threadIdx.x == 0) s.item = claim();
   __syncthreads();
   int item = s.item;
   __syncthreads();
@@ -1230,7 +1245,7 @@ __global__ __launch_bounds__(TW * TH, MODE == 0 ? 7 : 6) void k_raster(
       s.runseg[threadIdx.x] = reinterpret_cast<const int4*>(bins.counts)[item + threadIdx.x];
     __syncthreads();
     for (int it = item; it < end; it++) {
-      if (it == end - 1 && threadIdx.x == 0) s.item = atomicAdd(&bins.status[1], kQueueRun);
+      if (it == end - 1 && threadIdx.x == 0) s.item = claim();
       raster_tile<TW, TH, MODE>(s, it, s.runseg[it - item], views, mesh, bins, opt);
     }
     __syncthreads();

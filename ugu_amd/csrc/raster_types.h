@@ -115,10 +115,19 @@ struct DevBins {
                      // entry = (face id, bits of its skip depth)
   int32_t cap;
   int32_t total_tiles;
-  int32_t* status;   // [0] max bin count seen, [1] work-queue head
+  int32_t* status;   // [0] max bin count seen; [kQueueStride (q + 1)] head of
+                     // work queue q (kStatusBytes in all)
   const int32_t* tile_base;  // n_views + 1 prefix of tiles per view
   int32_t n_views;
 };
+
+// Work queues of the persistent rasterizer: run r (kQueueRun consecutive
+// tiles) belongs to queue r % kQueues; the queue heads sit 128 B apart so
+// their atomics spread over the memory channels instead of queueing on one
+// address.
+constexpr int kQueues = 8;
+constexpr int kQueueStride = 32;  // int32 words between queue heads
+constexpr int kStatusBytes = 4 * kQueueStride * (kQueues + 1);
 
 // Gradient targets of the differentiable backward pass.
 struct DevGrads {

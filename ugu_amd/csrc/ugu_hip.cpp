@@ -62,9 +62,9 @@ namespace {
 constexpr int kNoAllocMaxViews = 1024;
 constexpr int kMaxDevices = 64;
 // Layout of the scratch array (offsets 256-B aligned): status words (max bin
-// count, work-queue head), the view descriptors, the views' first global
+// count, work-queue heads), the view descriptors, the views' first global
 // tiles, then per render group the tiles' counters and the bins.
-constexpr size_t kNaViewsOff = 256;
+constexpr size_t kNaViewsOff = (size_t(kStatusBytes) + 255) / 256 * 256;
 constexpr size_t kNaTileBaseOff =
     (kNaViewsOff + size_t(kNoAllocMaxViews) * sizeof(DevView) + 255) / 256 * 256;
 constexpr size_t kNaCountsOff =
@@ -517,7 +517,7 @@ int run_pipeline(ugu_hip_ctx* c, const UguCamera* cams, int n, const DevMesh& dm
 
   int err;
   if ((err = ensure(c, c->counts, size_t(total_tiles) * 16))) return err;  // 4 per tile
-  if ((err = ensure(c, c->status, 64))) return err;
+  if ((err = ensure(c, c->status, kStatusBytes))) return err;
   if ((err = ensure(c, c->views, size_t(n) * sizeof(DevView)))) return err;
   if ((err = ensure(c, c->tile_base, size_t(n + 1) * 4))) return err;
   if (stage_bytes > 0 && (err = ensure(c, c->stage_out, size_t(stage_bytes)))) return err;
